@@ -1,0 +1,290 @@
+/*
+ * bcs.h -- C-ABI of the MI355X (gfx950) blood-cell physics step.
+ *
+ * The reference exposes this path as C++ classes whose sizes are compile-time
+ * constants (src/simulation/simulation_controller.cuh:24-67,
+ * src/grids/uniform_grid.cuh:15-65, src/objects/blood_cells.cuh:16-74,
+ * src/objects/vein_triangles.cuh:24-124).  This header replaces them with an
+ * opaque-handle C interface: plain pointers and sizes, no C++ or torch types,
+ * every call returns a status code instead of exiting the process
+ * (reference: CUDACHECK -> exit(EXIT_FAILURE), src/utilities/cuda_handle_error.cuh:18-25).
+ *
+ * Conventions
+ *  - Vector quantities are structure-of-arrays: x[n], y[n], z[n] contiguous,
+ *    i.e. one float buffer of 3*n values (reference cudaVec3,
+ *    src/utilities/cuda_vec3.cuh:11-73).
+ *  - Host pointers are caller-owned and copied; device pointers returned by
+ *    bcs_sim_device_state() stay owned by the simulation.
+ *  - One simulation = one HIP stream = one host thread at a time.
+ */
+#ifndef BCS_H
+#define BCS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define BCS_OK                    0
+#define BCS_ERR_INVALID_ARGUMENT  1
+#define BCS_ERR_HIP               2
+#define BCS_ERR_OUT_OF_MEMORY     3
+#define BCS_ERR_GRID_TOO_LARGE    4
+#define BCS_ERR_NO_DEVICE         5
+#define BCS_ERR_STATE             6
+#define BCS_ERR_UNSUPPORTED       7
+
+/* ---- device error word (bit flags, sticky until cleared) ---------------
+ * Reference: device printf on out-of-grid positions
+ * (src/grids/uniform_grid.cu:26-29) and on NaN in normalize
+ * (src/utilities/math.cuh:98-101). */
+#define BCS_FLAG_OUT_OF_GRID        0x1u  /* position outside [min,max] or key clamped */
+#define BCS_FLAG_NAN_NORMALIZE      0x2u  /* normalize() produced NaN and returned 0 */
+#define BCS_FLAG_SCAN_TIMEOUT       0x4u  /* a bounded device wait gave up */
+#define BCS_FLAG_RADIUS_CLAMPED     0x8u  /* mixed-type radius index left the model table */
+#define BCS_FLAG_VEIN_SLOT_OVERFLOW 0x10u /* per-vertex hit slots spilled to the overflow list */
+#define BCS_FLAG_OVERFLOW_LOST      0x20u /* overflow list itself was full: a contribution was dropped */
+
+/* ---- grid selector ----------------------------------------------------- */
+#define BCS_GRID_PARTICLES 0
+#define BCS_GRID_TRIANGLES 1
+
+/* ---- physics / simulation configuration --------------------------------
+ * Field-for-field runtime mirror of src/config/simulation.hpp:3-21 and
+ * src/config/physics.hpp:4-34 (+ cylinderRadius, src/meta_factory/vein_factory.hpp:88).
+ * bcs_default_physics() fills the reference's default values. */
+typedef struct bcs_physics {
+    /* src/config/simulation.hpp */
+    float dt;                          /* dt = 0.008 */
+    int32_t use_blood_flow;            /* useBloodFlow = 1 (vein-end teleport) */
+    int32_t enable_reaction_force;     /* enableReactionForce = 1 */
+    int32_t enable_big_cells_brake;    /* enableBigCellsBrake = 1 */
+    float particle_cell[3];            /* cellWidth/Height/Depth = 2 */
+    float triangle_cell[3];            /* cellWidthTriangles... = 25 */
+    int32_t bounding_spheres_coeff;    /* boundingSpheresCoeff = 3 */
+    float grid_y_margin;               /* gridYMargin = 40 */
+    float grid_xz_margin;              /* gridXZMargin = 40 */
+    float min_spawn_y;                 /* minSpawnY = -20 */
+    /* src/config/physics.hpp */
+    float velocity_collision_damping;  /* 0.96 */
+    float particle_k_sniff;            /* 25 */
+    float vein_k_sniff;                /* 0.1 */
+    float particle_d_fact;             /* 10 */
+    float vein_d_fact;                 /* 0.7 */
+    float vein_collision_force_intensity; /* 0.005 */
+    float viscous_damping;             /* 0.39 */
+    float collision_spring_coeff;      /* 6 */
+    float collision_damping_coeff;     /* 6 */
+    float collision_shear_coeff;       /* 4 */
+    float max_cell_size_factor_before_brake; /* 1.5 */
+    float big_particle_braking_intensity;    /* 0.8 */
+    float init_velocity[3];            /* (0,-80,0): teleport velocity */
+    float vein_impact_distance;        /* 6 */
+    float vein_impact_minimal_force_distance; /* 0.001 */
+    float gravity[3];                  /* (0,-25,0) */
+    /* src/meta_factory/vein_factory.hpp:88 */
+    float cylinder_radius;             /* 50: teleport spread */
+    /* defined by this implementation (no reference counterpart) */
+    uint64_t seed;                     /* counter-based RNG seed for the vein-end teleport */
+    int32_t dynamic_triangle_grid;     /* 0: triangle centres frozen after creation
+                                          (reference single-GPU behaviour, src/main.cu:164-176);
+                                          1: recompute centres + triangle grid every frame */
+    int32_t reserved;
+} bcs_physics;
+
+/* ---- blood-cell type ---------------------------------------------------
+ * Runtime form of BloodCellDef<Count, ParticlesInCell, ..., Springs, Vertices>
+ * (src/meta_factory/blood_cells_def_type.hpp:20-46).  Spring i joins vertices
+ * spring_start[i] and spring_end[i] with rest length spring_length[i]
+ * (0 means "no spring", as in src/objects/blood_cells.cu:14,93). */
+typedef struct bcs_cell_type {
+    int32_t count;                 /* number of cells of this type */
+    int32_t particles_in_cell;     /* ppc */
+    int32_t spring_count;
+    const int32_t* spring_start;
+    const int32_t* spring_end;
+    const float* spring_length;
+    const float* vertices;         /* 3*ppc floats, xyz interleaved: model vertex positions */
+    float contact_radius;          /* > 0: every vertex uses this collision radius;
+                                      <= 0: derived like the reference (min neighbour
+                                      distance / (2*boundingSpheresCoeff),
+                                      src/simulation/simulation_controller.cu:93-153) */
+    int32_t reserved;
+} bcs_cell_type;
+
+/* ---- vein mesh ---------------------------------------------------------
+ * Runtime form of veinPositions / veinIndices / VeinEndingCenters /
+ * VeinEndingRadii (src/config/vein_definition.hpp:12,15965,31768-31776). */
+typedef struct bcs_vein_mesh {
+    int32_t vertex_count;
+    const float* positions;        /* 3*vertex_count, xyz interleaved */
+    int32_t triangle_count;
+    const uint32_t* indices;       /* 3*triangle_count */
+    int32_t ending_count;
+    const float* ending_centers;   /* 3*ending_count, xyz interleaved */
+    const float* ending_radii;     /* ending_count */
+} bcs_vein_mesh;
+
+/* Derived, read-only description of a created simulation (the tables the
+ * reference folds at compile time in src/meta_factory/blood_cell_factory.hpp:172-328
+ * and src/meta_factory/vein_factory.hpp:46-88, plus grid geometry from
+ * src/grids/uniform_grid.cu:83-90). */
+typedef struct bcs_sim_info {
+    int32_t particle_count;          /* particleCount */
+    int32_t blood_cell_count;        /* bloodCellCount */
+    int32_t type_count;              /* bloodCellTypeCount */
+    int32_t model_vertex_count;      /* particleDistinctCellsCount */
+    int32_t vertex_count;            /* vein vertices */
+    int32_t triangle_count;
+    float bounds_min[3];             /* minX, minY, minZ */
+    float bounds_max[3];             /* maxX, maxY, maxZ */
+    int32_t particle_cells[3];       /* cellCountX/Y/Z of the particle grid */
+    int32_t triangle_cells[3];       /* cellCountX/Y/Z of the triangle grid */
+    uint64_t particle_cell_count;
+    uint64_t triangle_cell_count;
+    int32_t particle_starts[16];     /* particleStarts */
+    int32_t cell_type_starts[16];    /* bloodCellTypesStarts */
+    int32_t model_starts[16];        /* bloodCellModelStarts */
+    int32_t graph_starts[16];        /* accumulatedGraphSizes */
+    int32_t particles_in_cell[16];
+    int32_t cells_of_type[16];
+    uint64_t step_index;             /* frames completed (drives the teleport RNG) */
+} bcs_sim_info;
+
+/* Raw device pointers (for zero-copy use by a caller that already holds the
+ * state on the GPU, e.g. torch tensors built over them).  All SoA float[3*n]. */
+typedef struct bcs_device_state {
+    float* positions;     /* particles: x[N] y[N] z[N] */
+    float* velocities;
+    float* forces;
+    float* centers;       /* blood-cell centres, 3*bloodCellCount */
+    float* vein_positions;
+    float* vein_velocities;
+    float* vein_forces;
+    float* triangle_centers;
+    void* stream;         /* hipStream_t the simulation enqueues on */
+} bcs_device_state;
+
+typedef struct bcs_sim bcs_sim;
+
+/* ---- configuration helpers (host only, no GPU needed) ------------------- */
+
+/* Reference defaults of src/config/physics.hpp:4-34 and src/config/simulation.hpp:3-21. */
+int bcs_default_physics(bcs_physics* out);
+
+/* Reference type folding: merge user definitions that share (ppc, spring list)
+ * and sort them with the reference comparator
+ * (src/meta_factory/blood_cell_factory.hpp:21-162, boost::mp11 mp_sort quicksort).
+ * order_out[k] = index into `user` of the k-th folded type, counts_out[k] = its
+ * merged count; *n_out = number of folded types. */
+int bcs_fold_reference_types(const bcs_cell_type* user, int32_t n_user,
+                             int32_t* order_out, int32_t* counts_out, int32_t* n_out);
+
+/* Grid bounds = vein AABB +/- margins (src/meta_factory/vein_factory.hpp:21-86). */
+int bcs_grid_bounds(const bcs_vein_mesh* mesh, const bcs_physics* phys,
+                    float bounds_min[3], float bounds_max[3]);
+
+/* Vein spring-neighbour tables: 9 slots per vertex, slot-major
+ * (ids[s*nv + v], lengths[s*nv + v]); unused = -1 / -1.0f
+ * (src/meta_factory/vein_factory.hpp:130-174). */
+int bcs_vein_neighbors(const bcs_vein_mesh* mesh, int32_t* ids, float* lengths);
+
+/* Per-model-vertex collision radii and initial radii for a type list in final
+ * order (src/simulation/simulation_controller.cu:93-153).  Arrays sized
+ * sum(ppc). */
+int bcs_model_tables(const bcs_physics* phys, const bcs_cell_type* types, int32_t n_types,
+                     float* bounding_spheres, float* initial_radii);
+
+/* ---- simulation lifecycle (replaces SimulationController + BloodCells +
+ *      VeinTriangles + two UniformGrid objects, src/main.cu:122-138) -------- */
+
+/* Types are taken in final order (no re-sorting; use bcs_fold_reference_types
+ * to reproduce the reference order).  At most 16 types. */
+int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* types, int32_t n_types,
+                   const bcs_vein_mesh* mesh, int32_t device, bcs_sim** out);
+int bcs_sim_destroy(bcs_sim* sim);
+int bcs_sim_get_info(bcs_sim* sim, bcs_sim_info* out);
+int bcs_sim_device_state(bcs_sim* sim, bcs_device_state* out);
+
+/* Host <-> device state transfer (SoA, 3*n floats each; NULL = skip). */
+int bcs_sim_set_particles(bcs_sim* sim, const float* positions, const float* velocities,
+                          const float* forces);
+int bcs_sim_get_particles(bcs_sim* sim, float* positions, float* velocities, float* forces);
+int bcs_sim_set_vein(bcs_sim* sim, const float* positions, const float* velocities,
+                     const float* forces);
+int bcs_sim_get_vein(bcs_sim* sim, float* positions, float* velocities, float* forces);
+int bcs_sim_get_centers(bcs_sim* sim, float* centers);
+int bcs_sim_set_step_index(bcs_sim* sim, uint64_t step_index);
+
+/* Reference step API ---------------------------------------------------- */
+
+/* UniformGrid::calculateGrid (src/grids/uniform_grid.cu:129-155) on the
+ * particle positions (BCS_GRID_PARTICLES) or the triangle centres
+ * (BCS_GRID_TRIANGLES; src/main.cu:175-176). */
+int bcs_sim_calculate_grid(bcs_sim* sim, int32_t which);
+
+/* SimulationController::calculateNextFrame
+ * (src/simulation/simulation_controller.cu:246-313): vein springs, cell
+ * centres, cell springs, particle collisions, vein collisions. */
+int bcs_sim_calculate_next_frame(bcs_sim* sim);
+
+/* SimulationController::propagateAll
+ * (src/simulation/simulation_controller.cu:315-331): particle integrate, vein
+ * integrate (+ zero vein forces), vein-end teleport. */
+int bcs_sim_propagate_all(bcs_sim* sim);
+
+/* One whole frame of src/main.cu:175-208 (grids + calculateNextFrame +
+ * propagateAll) repeated n times through fused kernels replayed from a HIP
+ * graph.  Asynchronous: returns after enqueueing. */
+int bcs_sim_step(bcs_sim* sim, int32_t n_steps);
+
+int bcs_sim_synchronize(bcs_sim* sim);
+
+/* Time n fused steps with HIP events on the simulation's stream (ms total). */
+int bcs_sim_time_steps(bcs_sim* sim, int32_t n_steps, float* total_ms);
+
+/* Per-kernel average durations (ms) of the fused step over n steps, measured
+ * with HIP events around each launch on the simulation's stream.  names_out
+ * receives a ';'-separated list of kernel names matching ms_out order. */
+int bcs_sim_profile_kernels(bcs_sim* sim, int32_t n_steps, float* ms_out, int32_t capacity,
+                            int32_t* n_kernels, char* names_out, int32_t names_capacity);
+
+/* Parity views ------------------------------------------------------------ */
+
+/* Sorted grid of the last bcs_sim_calculate_grid / step: keys_sorted[n] and
+ * perm[n] (gridCellIds / particleIds after thrust::stable_sort_by_key), and
+ * the occupied cells with inclusive [start, end] ranges
+ * (gridCellStarts / gridCellEnds).  Any output may be NULL.  occ arrays need
+ * capacity n. */
+int bcs_sim_get_grid(bcs_sim* sim, int32_t which, uint32_t* keys_sorted, int32_t* perm,
+                     int32_t* n_occupied, uint32_t* occ_keys, int32_t* occ_start,
+                     int32_t* occ_end);
+
+/* Contact pairs of the particle-collision phase on the current grid and
+ * state: pairs (i, j) with d^2 <= (r_i + r_j)^2 and d^2 >= 1e-4, in visit
+ * order per i (src/simulation/particle_collisions.cuh:26-83).  counts[i] per
+ * particle; pairs_j flattened by particle index.  *n_total is always set;
+ * pairs_j may be NULL to only count. */
+int bcs_sim_get_contacts(bcs_sim* sim, int32_t* counts, int32_t* pairs_j, int64_t capacity,
+                         int64_t* n_total);
+
+/* Device error word (BCS_FLAG_*); clear != 0 resets it. */
+int bcs_sim_error_flags(bcs_sim* sim, int32_t clear, uint32_t* flags);
+
+/* Last error message of the calling thread ("" if none). */
+const char* bcs_last_error(void);
+
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int bcs_device_count(int32_t* count);
+
+/* Plain device-to-device copy of n_bytes through this library's own copy
+ * kernel, on a fresh allocation (HBM counter calibration).  Returns elapsed ms. */
+int bcs_calibration_copy(int64_t n_bytes, int32_t repeats, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BCS_H */

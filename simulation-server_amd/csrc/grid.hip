@@ -1,0 +1,373 @@
+// grid.hip -- uniform-grid build for gfx950: cell hash, stable counting sort
+// and cell-start table.
+//
+// Replaces UniformGrid::calculateGrid (src/grids/uniform_grid.cu:129-155):
+// calculateCellIdKernel (:38-49), thrust::stable_sort_by_key (:144-147) and
+// calculateStartAndEndOfCellKernel (:51-80).
+//
+// Sort: keys are split into buckets of 2^shift consecutive cell ids.
+//   k_key      key per object, bucket histogram (atomics on a small table)
+//   k_scan     exclusive scan of the histogram (single pass, decoupled look-back)
+//   k_scatter  (key, index) pairs into their bucket segments
+//   k_rank     in-bucket rank by counting over (key, index): the output order is
+//              exactly thrust's stable order (ties by object index); run starts
+//              write the dense cell-start table
+//   k_big      the same for buckets larger than the k_rank threshold (LDS tiles)
+// The dense start table is never cleared: an entry is valid only if
+// keys_sorted[start] == key, which holds for every occupied cell of the current
+// build and for no empty one.
+#include "grid.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace bcs {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr uint32_t kRankThreshold = 256;
+constexpr uint64_t kStatusAgg = 1ull << 62;
+constexpr uint64_t kStatusInc = 2ull << 62;
+constexpr uint64_t kSpinLimit = 1ull << 22;
+
+__global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict__ pos, int64_t stride, int32_t n,
+                                             int shift, uint32_t* __restrict__ keys, uint32_t* __restrict__ bcount,
+                                             uint64_t* __restrict__ status, int32_t ntiles,
+                                             uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = gid; t < ntiles; t += nthreads) status[t] = 0ull;
+    if (gid == 0) {
+        ctrl[0] = 0u;  // scan ticket
+        ctrl[1] = 0u;  // big-bucket count
+    }
+    uint32_t fl = 0;
+    if (gid < n) {
+        uint32_t k = cell_key(g, pos[gid], pos[stride + gid], pos[2 * stride + gid], fl);
+        keys[gid] = k;
+        atomicAdd(&bcount[k >> shift], 1u);
+    }
+    if (fl) atomicOr(flags, fl);
+}
+
+// Exclusive scan of `count[0..m)` into out[0..m] (out[m] = total), single
+// pass with decoupled look-back over tiles taken in ticket order.  Entries
+// larger than big_thr are appended to biglist (when biglist != nullptr).
+__global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restrict__ count, uint32_t* __restrict__ out,
+                                                       int64_t m, uint64_t* __restrict__ status, int32_t ntiles,
+                                                       uint32_t* __restrict__ ctrl, uint32_t* __restrict__ biglist,
+                                                       uint32_t big_thr, uint32_t* __restrict__ flags)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    __shared__ uint32_t s_prefix;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(&ctrl[0], 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile >= ntiles) {  // stale ticket counter: never index past the status array
+        if (tid == 0) atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);
+        return;
+    }
+    const int64_t base = tile * kScanTile + (int64_t)tid * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int64_t i = base + k;
+        v[k] = (i < m) ? count[i] : 0u;
+        if (biglist && i < m && v[k] > big_thr) {
+            uint32_t slot = atomicAdd(&ctrl[1], 1u);
+            biglist[slot] = (uint32_t)i;
+        }
+        sum += v[k];
+    }
+    // wave64 inclusive scan of the per-thread sums
+    const int lane = tid & 63, wave = tid >> 6;
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0, agg = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wave) wave_off += s_wave[w];
+        agg += s_wave[w];
+    }
+    if (tid == 0) {
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&status[0], kStatusInc | (uint64_t)agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&status[tile], kStatusAgg | (uint64_t)agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = tile - 1;
+            uint64_t spins = 0;
+            while (j >= 0) {
+                uint64_t w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t st = w & (3ull << 62);
+                if (st == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += (uint32_t)(w & 0xFFFFFFFFull);
+                if (st == kStatusInc) break;
+                --j;
+            }
+            __hip_atomic_store(&status[tile], kStatusInc | (uint64_t)(prefix + agg), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = prefix;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + wave_off + (inc - sum);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int64_t i = base + k;
+        if (i <= m) out[i] = run;
+        run += v[k];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ keys, int32_t n, int shift,
+                                                 const uint32_t* __restrict__ bstart, uint32_t* __restrict__ bcount,
+                                                 uint32_t* __restrict__ items_k, int32_t* __restrict__ items_i,
+                                                 uint32_t* __restrict__ flags)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    const uint32_t b = k >> shift;
+    // counts return to zero: the histogram is ready for the next build
+    const uint32_t old = atomicSub(&bcount[b], 1u);
+    const uint32_t pos = bstart[b] + old - 1u;
+    if (old == 0u || pos >= (uint32_t)n) {  // inconsistent histogram: never write out of bounds
+        atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);
+        return;
+    }
+    items_k[pos] = k;
+    items_i[pos] = i;
+}
+
+struct RankOut {
+    uint32_t* keys_sorted;
+    int32_t* perm;
+    int32_t* slot_of;      // nullable
+    uint32_t* start;
+    const float* src_pos;  // sorted copies (nullable)
+    const float* src_vel;
+    int64_t src_stride;
+    float* xs;
+    float* vs;
+    int64_t dst_stride;
+};
+
+__device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32_t idx, uint32_t pos, bool first)
+{
+    o.keys_sorted[pos] = key;
+    o.perm[pos] = idx;
+    if (o.slot_of) o.slot_of[idx] = (int32_t)pos;
+    if (first) o.start[key] = pos;
+    if (o.xs) {
+        const int64_t s = o.src_stride, d = o.dst_stride;
+        o.xs[pos] = o.src_pos[idx];
+        o.xs[d + pos] = o.src_pos[s + idx];
+        o.xs[2 * d + pos] = o.src_pos[2 * s + idx];
+        o.vs[pos] = o.src_vel[idx];
+        o.vs[d + pos] = o.src_vel[s + idx];
+        o.vs[2 * d + pos] = o.src_vel[2 * s + idx];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_rank(const uint32_t* __restrict__ items_k, const int32_t* __restrict__ items_i,
+                                              int32_t n, int shift, const uint32_t* __restrict__ bstart, RankOut o)
+{
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t key = items_k[q];
+    const int32_t idx = items_i[q];
+    const uint32_t b = key >> shift;
+    const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
+    if (s1 < s0 || s1 > (uint32_t)n) return;  // inconsistent offsets (flagged by k_scatter)
+    if (s1 - s0 > kRankThreshold) return;     // handled by k_big
+    uint32_t rank = 0;
+    bool first = true;
+    for (uint32_t u = s0; u < s1; ++u) {
+        const uint32_t k2 = items_k[u];
+        if (k2 < key) {
+            ++rank;
+        } else if (k2 == key) {
+            if (items_i[u] < idx) {
+                ++rank;
+                first = false;
+            }
+        }
+    }
+    rank_write(o, key, idx, s0 + rank, first);
+}
+
+constexpr int kBigChunk = 1024;
+
+__global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_k, const int32_t* __restrict__ items_i,
+                                             int32_t n, int shift, const uint32_t* __restrict__ bstart,
+                                             const uint32_t* __restrict__ biglist, const uint32_t* __restrict__ ctrl,
+                                             RankOut o)
+{
+    __shared__ uint32_t sk[kBigChunk];
+    __shared__ int32_t si[kBigChunk];
+    const uint32_t nbig = ctrl[1];
+    for (uint32_t e = blockIdx.x; e < nbig; e += gridDim.x) {
+        const uint32_t b = biglist[e];
+        const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
+        if (s1 < s0 || s1 > (uint32_t)n) continue;
+        for (uint32_t gbase = s0; gbase < s1; gbase += blockDim.x) {
+            const uint32_t q = gbase + threadIdx.x;
+            const bool own = q < s1;
+            const uint32_t key = own ? items_k[q] : 0u;
+            const int32_t idx = own ? items_i[q] : 0;
+            uint32_t rank = 0;
+            bool first = true;
+            for (uint32_t cbase = s0; cbase < s1; cbase += kBigChunk) {
+                const uint32_t cn = min((uint32_t)kBigChunk, s1 - cbase);
+                __syncthreads();
+                for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
+                    sk[t] = items_k[cbase + t];
+                    si[t] = items_i[cbase + t];
+                }
+                __syncthreads();
+                if (own) {
+                    for (uint32_t t = 0; t < cn; ++t) {
+                        const uint32_t k2 = sk[t];
+                        if (k2 < key) {
+                            ++rank;
+                        } else if (k2 == key && si[t] < idx) {
+                            ++rank;
+                            first = false;
+                        }
+                    }
+                }
+            }
+            if (own) rank_write(o, key, idx, s0 + rank, first);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host --
+
+static int choose_shift(uint64_t cell_count, int64_t cap)
+{
+    // aim for ~8 objects per bucket if they filled the whole grid uniformly
+    const double target = std::max<double>(1.0, (double)std::max<int64_t>(cap, 1) / 8.0);
+    int s = 0;
+    while (s < 24 && (double)(cell_count >> s) > target) ++s;
+    return s;
+}
+
+#define GRID_HIP(call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return fail(BCS_ERR_HIP, std::string("grid: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_copies, hipStream_t st)
+{
+    G = DeviceGrid{};
+    G.g = geom;
+    G.cap = cap;
+    if (geom.cell_count == 0 || geom.cell_count > 0xFFFFFFFFull)
+        return fail(BCS_ERR_GRID_TOO_LARGE, "grid: cell count " + std::to_string(geom.cell_count) +
+                                                " outside [1, 2^32) (choose a larger cell size)");
+    G.shift = choose_shift(geom.cell_count, cap);
+    G.nb = ((geom.cell_count - 1) >> G.shift) + 1;
+    G.ntiles = (int32_t)((G.nb + 1 + kScanTile - 1) / kScanTile);
+    const size_t n = (size_t)std::max(cap, 1);
+    GRID_HIP(hipMalloc(&G.keys, n * 4));
+    GRID_HIP(hipMalloc(&G.items_k, n * 4));
+    GRID_HIP(hipMalloc(&G.items_i, n * 4));
+    GRID_HIP(hipMalloc(&G.keys_sorted, n * 4));
+    GRID_HIP(hipMalloc(&G.perm, n * 4));
+    if (with_slot) GRID_HIP(hipMalloc(&G.slot_of, n * 4));
+    if (with_copies) {
+        GRID_HIP(hipMalloc(&G.xs, n * 12));
+        GRID_HIP(hipMalloc(&G.vs, n * 12));
+    }
+    hipError_t e = hipMalloc(&G.start, geom.cell_count * 4);
+    if (e != hipSuccess)
+        return fail(BCS_ERR_OUT_OF_MEMORY, "grid: cell-start table of " + std::to_string(geom.cell_count * 4) + " bytes");
+    GRID_HIP(hipMemsetAsync(G.start, 0xFF, geom.cell_count * 4, st));
+    GRID_HIP(hipMalloc(&G.bcount, (G.nb + 1) * 4));
+    GRID_HIP(hipMemsetAsync(G.bcount, 0, (G.nb + 1) * 4, st));
+    GRID_HIP(hipMalloc(&G.bstart, (G.nb + 2) * 4));
+    GRID_HIP(hipMalloc(&G.status, (size_t)std::max(G.ntiles, 1) * 8));
+    GRID_HIP(hipMalloc(&G.ctrl, 16 * 4));
+    GRID_HIP(hipMemsetAsync(G.ctrl, 0, 16 * 4, st));
+    G.big_cap = (uint32_t)std::min<uint64_t>(G.nb, (uint64_t)n / kRankThreshold + 1);
+    GRID_HIP(hipMalloc(&G.biglist, (size_t)(G.big_cap + 1) * 4));
+    return BCS_OK;
+}
+
+void grid_free(DeviceGrid& G)
+{
+    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.perm, G.slot_of, G.xs, G.vs,
+                    G.start, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    G = DeviceGrid{};
+}
+
+int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
+               hipStream_t st, const KernelTimer* timer)
+{
+    if (n < 0 || n > G.cap) return fail(BCS_ERR_INVALID_ARGUMENT, "grid: object count exceeds capacity");
+    G.n = n;
+    if (n == 0) return BCS_OK;
+    const int blocks = (n + 255) / 256;
+    auto tb = [&](int id) { if (timer) timer->begin(id, st); };
+    auto te = [&](int id) { if (timer) timer->end(id, st); };
+    tb(0);
+    hipLaunchKernelGGL(k_key, dim3(std::max(blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride, n,
+                       G.shift, G.keys, G.bcount, G.status, G.ntiles, G.ctrl, flags);
+    te(0);
+    tb(1);
+    hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,
+                       G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold, flags);
+    te(1);
+    tb(2);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.bcount, G.items_k,
+                       G.items_i, flags);
+    te(2);
+    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.start, vel ? pos : nullptr, vel, stride, vel ? G.xs : nullptr,
+              vel ? G.vs : nullptr, (int64_t)G.cap};
+    tb(3);
+    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, o);
+    te(3);
+    tb(4);
+    hipLaunchKernelGGL(k_big, dim3(64), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, G.biglist, G.ctrl,
+                       o);
+    te(4);
+    GRID_HIP(hipGetLastError());
+    return BCS_OK;
+}
+
+// Generic exclusive scan used by the vein-hit segments (same kernel).
+int scan_exclusive(const uint32_t* count, uint32_t* out, int64_t m, uint64_t* status, int32_t ntiles, uint32_t* ctrl,
+                   uint32_t* flags, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, out, m, status, ntiles, ctrl,
+                       (uint32_t*)nullptr, 0u, flags);
+    GRID_HIP(hipGetLastError());
+    return BCS_OK;
+}
+
+int scan_tiles(int64_t m) { return (int)((m + 1 + kScanTile - 1) / kScanTile); }
+
+}  // namespace bcs

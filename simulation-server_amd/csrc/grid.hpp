@@ -1,0 +1,61 @@
+// grid.hpp -- device uniform grid (replaces the UniformGrid buffers of
+// src/grids/uniform_grid.cuh:30-35) and the shared scan.
+#pragma once
+
+#include "bcs_internal.hpp"
+#include "device_math.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace bcs {
+
+// Optional per-kernel HIP-event timing (profiling mode only).
+struct KernelTimer {
+    std::vector<hipEvent_t>* ev_begin = nullptr;
+    std::vector<hipEvent_t>* ev_end = nullptr;
+    int offset = 0;  // kernel id offset of this group
+    void begin(int id, hipStream_t st) const { (void)hipEventRecord((*ev_begin)[offset + id], st); }
+    void end(int id, hipStream_t st) const { (void)hipEventRecord((*ev_end)[offset + id], st); }
+};
+
+struct DeviceGrid {
+    GridGeom g{};
+    int32_t cap = 0;            // object capacity
+    int32_t n = 0;              // objects in the last build
+    int shift = 0;              // bucket = key >> shift
+    uint64_t nb = 0;            // buckets
+    int32_t ntiles = 0;         // scan tiles over nb+1 entries
+    uint32_t big_cap = 0;
+    uint32_t* keys = nullptr;         // per object, input order
+    uint32_t* items_k = nullptr;      // bucketed (key, index)
+    int32_t* items_i = nullptr;
+    uint32_t* keys_sorted = nullptr;  // gridCellIds after the sort
+    int32_t* perm = nullptr;          // particleIds after the sort
+    int32_t* slot_of = nullptr;       // inverse of perm (optional)
+    uint32_t* start = nullptr;        // dense cell-start table [cell_count], validated by keys_sorted
+    uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds
+    uint32_t* bstart = nullptr;       // bucket offsets [nb+2]
+    uint64_t* status = nullptr;       // scan tile status
+    uint32_t* ctrl = nullptr;         // [0] scan ticket, [1] big-bucket count
+    uint32_t* biglist = nullptr;
+    float* xs = nullptr;              // positions in sorted order (optional), stride cap
+    float* vs = nullptr;              // velocities in sorted order (optional)
+};
+
+// Initialisation is enqueued on `st` (the owner's stream, which does not
+// synchronise with the legacy NULL stream).
+int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_copies, hipStream_t st);
+void grid_free(DeviceGrid& G);
+// pos: SoA base pointer (x, y, z at +0, +stride, +2*stride).  vel != nullptr
+// also writes the sorted copies xs/vs.
+int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
+               hipStream_t st, const KernelTimer* timer = nullptr);
+constexpr int kGridKernels = 5;
+
+int scan_exclusive(const uint32_t* count, uint32_t* out, int64_t m, uint64_t* status, int32_t ntiles, uint32_t* ctrl,
+                   uint32_t* flags, hipStream_t st);
+int scan_tiles(int64_t m);
+
+}  // namespace bcs

@@ -1,0 +1,773 @@
+// sim.hip -- the C-ABI of include/bcs.h: device state, phase sequencing and
+// the graph-replayed fused frame.
+//
+// Replaces, for one GPU: SimulationController (src/simulation/simulation_controller.cu:42-331),
+// BloodCells (src/objects/blood_cells.cu:16-42), VeinTriangles
+// (src/objects/vein_triangles.cu:37-86), VeinNeighbors
+// (src/objects/vein_neighbors.cu:8-54) and the two UniformGrid objects of
+// src/main.cu:130-133.  All device memory is owned by the bcs_sim object and
+// all work is enqueued on its single stream (default priority).
+#include "step.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace bcs {
+void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st);
+void launch_integrate(const StepArgs& A, hipStream_t st);
+void launch_vein_end(const StepArgs& A, int nchunks, hipStream_t st);
+void launch_bump(uint64_t* step, hipStream_t st);
+void launch_vein_springs(const VeinArgs& V, hipStream_t st);
+void launch_vein_scatter(const VeinArgs& V, hipStream_t st);
+void launch_vein(const VeinArgs& V, int mode, hipStream_t st);
+void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t ntri, float* c, hipStream_t st);
+void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st);
+void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
+void launch_gather_u32(const uint32_t* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
+}  // namespace bcs
+
+using namespace bcs;
+
+#define HIPCHK(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) return fail(BCS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct bcs_sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bcs_physics phys{};
+    int32_t nt = 0, np = 0, nc = 0, nm = 0, nv = 0, ntri = 0;
+    float bmin[3]{}, bmax[3]{};
+    TypeTable tt{};
+    PhysConst pc{};
+    int nchunks = 0;
+    std::vector<void*> allocs;
+    // particles
+    float *pos = nullptr, *vel = nullptr, *frc = nullptr, *centers = nullptr;
+    float *bs = nullptr, *r0 = nullptr, *model = nullptr, *sp_L = nullptr;
+    int32_t *sp_off = nullptr, *sp_b = nullptr, *chunk_type = nullptr, *chunk_first = nullptr, *chunk_n = nullptr;
+    // vein
+    float *X = nullptr, *V = nullptr, *Fv = nullptr, *nbl = nullptr, *tcent = nullptr;
+    int32_t* nbr = nullptr;
+    uint32_t* tidx = nullptr;
+    uint32_t *vcount = nullptr, *vstart = nullptr, *hctrl = nullptr;
+    float4 *hits = nullptr, *seg = nullptr;
+    uint64_t* vstatus = nullptr;
+    int32_t vntiles = 0;
+    // grids
+    DeviceGrid pg, tg;
+    bool tg_built = false;
+    // misc
+    uint32_t* flags = nullptr;
+    uint64_t* step = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<float> host_bs;
+    std::vector<float> host_smallest;
+
+    ~bcs_sim()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        grid_free(pg);
+        grid_free(tg);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <class T>
+    int dalloc(T** p, size_t count)
+    {
+        hipError_t e = hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(BCS_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back((void*)*p);
+        return BCS_OK;
+    }
+
+    StepArgs step_args() const
+    {
+        StepArgs A{};
+        A.tt = tt;
+        A.pc = pc;
+        A.pos = pos; A.vel = vel; A.frc = frc; A.centers = centers;
+        A.bs = bs; A.r0 = r0; A.model = model; A.sp_off = sp_off; A.sp_b = sp_b; A.sp_L = sp_L;
+        A.chunk_type = chunk_type; A.chunk_first = chunk_first; A.chunk_n = chunk_n;
+        A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_start = pg.start;
+        A.xs = pg.xs; A.vs = pg.vs; A.s_stride = pg.cap;
+        A.tg = tg.g; A.t_keys = tg.keys_sorted; A.t_perm = tg.perm; A.t_start = tg.start; A.ntri = ntri;
+        A.nv = nv; A.X = X; A.tidx = tidx;
+        A.vcount = vcount; A.hits = hits; A.hctrl = hctrl; A.vstatus = vstatus; A.vntiles = vntiles;
+        A.flags = flags; A.step = step;
+        return A;
+    }
+    VeinArgs vein_args() const
+    {
+        VeinArgs Va{};
+        Va.nv = nv; Va.X = X; Va.V = V; Va.Fv = Fv; Va.nbr = nbr; Va.nbl = nbl;
+        Va.dt = phys.dt; Va.vk = phys.vein_k_sniff; Va.vd = phys.vein_d_fact;
+        Va.vcount = vcount; Va.vstart = vstart; Va.hits = hits; Va.seg = seg; Va.hctrl = hctrl; Va.tidx = tidx;
+        Va.flags = flags; Va.step = step;
+        return Va;
+    }
+};
+
+namespace {
+
+GridGeom make_geom(const float bmin[3], const float bmax[3], const float cell[3])
+{
+    GridGeom g{};
+    for (int d = 0; d < 3; ++d) {
+        g.minv[d] = bmin[d];
+        g.maxv[d] = bmax[d];
+        g.extent[d] = bmax[d] - bmin[d];
+        g.cell[d] = cell[d];
+        float q = g.extent[d] / g.cell[d];  // static_cast<int>(width / cellWidth), uniform_grid.cu:86-88
+        g.cnt[d] = (q >= 2147483647.0f) ? 2147483647 : (q > 0 ? (int32_t)q : 0);
+    }
+    g.cell_count = (uint64_t)g.cnt[0] * (uint64_t)g.cnt[1] * (uint64_t)g.cnt[2];
+    return g;
+}
+
+int set_dev(bcs_sim* s)
+{
+    if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
+    HIPCHK(hipSetDevice(s->device));
+    return BCS_OK;
+}
+
+#define RC(x)                 \
+    do {                      \
+        int rc_ = (x);        \
+        if (rc_) return rc_;  \
+    } while (0)
+
+int build_triangle_grid(bcs_sim* s)
+{
+    if (s->phys.dynamic_triangle_grid || !s->tg_built)
+        launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
+    RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, nullptr, s->flags, s->stream));
+    s->tg_built = true;
+    return BCS_OK;
+}
+
+int enqueue_particle_grid(bcs_sim* s, const KernelTimer* timer = nullptr)
+{
+    return grid_build(s->pg, s->pos, s->np, s->np, s->vel, s->flags, s->stream, timer);
+}
+
+int enqueue_vein_hits(bcs_sim* s, const VeinArgs& Va)
+{
+    RC(scan_exclusive(s->vcount, s->vstart, s->nv, s->vstatus, s->vntiles, s->hctrl + 1, s->flags, s->stream));
+    launch_vein_scatter(Va, s->stream);
+    return BCS_OK;
+}
+
+// One fused frame: particle grid, vein springs, k_cells (springs, contacts,
+// vein ray, integrate, vein end), vein-hit segments, vein sum + integrate.
+int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
+{
+    hipStream_t st = s->stream;
+    auto tb = [&](int id) { if (timer) timer->begin(id, st); };
+    auto te = [&](int id) { if (timer) timer->end(id, st); };
+    if (s->phys.dynamic_triangle_grid) RC(build_triangle_grid(s));
+    RC(enqueue_particle_grid(s, timer));
+    const StepArgs A = s->step_args();
+    const VeinArgs Va = s->vein_args();
+    tb(kGridKernels + 0);
+    launch_vein_springs(Va, st);
+    te(kGridKernels + 0);
+    tb(kGridKernels + 1);
+    launch_cells(A, s->nchunks, true, st);
+    te(kGridKernels + 1);
+    tb(kGridKernels + 2);
+    RC(scan_exclusive(s->vcount, s->vstart, s->nv, s->vstatus, s->vntiles, s->hctrl + 1, s->flags, st));
+    te(kGridKernels + 2);
+    tb(kGridKernels + 3);
+    launch_vein_scatter(Va, st);
+    te(kGridKernels + 3);
+    tb(kGridKernels + 4);
+    launch_vein(Va, 7, st);
+    te(kGridKernels + 4);
+    HIPCHK(hipGetLastError());
+    return BCS_OK;
+}
+
+const char* kKernelNames = "k_key;k_scan;k_scatter;k_rank;k_big;k_vein_springs;k_cells;k_scan(vein);k_vein_scatter;k_vein";
+constexpr int kFusedKernels = kGridKernels + 5;
+
+}  // namespace
+
+extern "C" int bcs_device_count(int32_t* count)
+{
+    if (!count) return fail(BCS_ERR_INVALID_ARGUMENT, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* types, int32_t n_types,
+                              const bcs_vein_mesh* mesh, int32_t device, bcs_sim** out)
+{
+    if (!phys || !types || !mesh || !out) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: null argument");
+    *out = nullptr;
+    if (n_types < 1 || n_types > kMaxTypes) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: 1..16 cell types");
+    if (mesh->vertex_count < 1 || mesh->triangle_count < 1 || !mesh->positions || !mesh->indices)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: empty vein mesh");
+    if (mesh->ending_count < 0 || mesh->ending_count > kMaxEndings)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: at most 16 vein endings");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(BCS_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(BCS_ERR_NO_DEVICE, "device index out of range");
+    std::unique_ptr<bcs_sim> s(new bcs_sim());
+    s->device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->phys = *phys;
+    s->nt = n_types;
+    // ---- type tables (blood_cell_factory.hpp:172-282)
+    TypeTable& tt = s->tt;
+    tt.nt = n_types;
+    int64_t np = 0, nc = 0, nm = 0;
+    for (int t = 0; t < n_types; ++t) {
+        const bcs_cell_type& c = types[t];
+        if (c.count < 0 || c.particles_in_cell < 1 || c.particles_in_cell > kBlock || !c.vertices)
+            return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: bad cell type (1 <= ppc <= 256)");
+        for (int k = 0; k < c.spring_count; ++k)
+            if (c.spring_start[k] < 0 || c.spring_end[k] < 0 || c.spring_start[k] >= c.particles_in_cell ||
+                c.spring_end[k] >= c.particles_in_cell)
+                return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: ill-formed spring");
+        tt.ppc[t] = c.particles_in_cell;
+        tt.cnt[t] = c.count;
+        tt.pstart[t] = (int32_t)np;
+        tt.cstart[t] = (int32_t)nc;
+        tt.mstart[t] = (int32_t)nm;
+        tt.end_spheres[t] = !((int64_t)c.count * c.particles_in_cell <= 32 || (32 % c.particles_in_cell) == 0);
+        np += (int64_t)c.count * c.particles_in_cell;
+        nc += c.count;
+        nm += c.particles_in_cell;
+    }
+    if (np < 1) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: no particles");
+    if (np > (1 << 29)) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: more than 2^29 particles");
+    tt.np = (int32_t)np; tt.nc = (int32_t)nc; tt.nm = (int32_t)nm;
+    s->np = tt.np; s->nc = tt.nc; s->nm = tt.nm;
+    s->nv = mesh->vertex_count;
+    s->ntri = mesh->triangle_count;
+    for (int64_t k = 0; k < 3 * (int64_t)s->ntri; ++k)
+        if (mesh->indices[k] >= (uint32_t)s->nv) return fail(BCS_ERR_INVALID_ARGUMENT, "vein index out of range");
+    // ---- host tables
+    std::vector<float> hbs(nm), hr0(nm), hmodel(3 * nm);
+    RC(bcs_model_tables(phys, types, n_types, hbs.data(), hr0.data()));
+    s->host_bs = hbs;
+    std::vector<int32_t> hoff(nm + 1, 0), hb;
+    std::vector<float> hL;
+    for (int t = 0; t < n_types; ++t) {
+        const bcs_cell_type& c = types[t];
+        const int p = c.particles_in_cell;
+        std::memcpy(hmodel.data() + 3 * tt.mstart[t], c.vertices, sizeof(float) * 3 * p);
+        // dense springGraph, blood_cell_factory.hpp:292-328 (later springs overwrite)
+        std::vector<float> dense((size_t)p * p, 0.f);
+        for (int k = 0; k < c.spring_count; ++k) {
+            float L = c.spring_length[k] * 1.0f;
+            dense[(size_t)c.spring_start[k] * p + c.spring_end[k]] = L;
+            dense[(size_t)c.spring_end[k] * p + c.spring_start[k]] = L;
+        }
+        // CSR of column a in ascending b (the order of gatherForcesKernel's loop)
+        for (int a = 0; a < p; ++a) {
+            for (int b = 0; b < p; ++b) {
+                float L = dense[(size_t)b * p + a];
+                if (L != 0.f) {
+                    hb.push_back(b);
+                    hL.push_back(L);
+                }
+            }
+            hoff[tt.mstart[t] + a + 1] = (int32_t)hb.size();
+        }
+    }
+    float bmin[3], bmax[3];
+    RC(bcs_grid_bounds(mesh, phys, bmin, bmax));
+    std::memcpy(s->bmin, bmin, sizeof(bmin));
+    std::memcpy(s->bmax, bmax, sizeof(bmax));
+    // ---- physics constants (physics.hpp, simulation.hpp, vein_end.cu:12-18)
+    PhysConst& pc = s->pc;
+    pc.dt = phys->dt; pc.k = phys->particle_k_sniff; pc.d = phys->particle_d_fact;
+    pc.vk = phys->vein_k_sniff; pc.vd = phys->vein_d_fact;
+    pc.vcd = phys->velocity_collision_damping; pc.vcfi = phys->vein_collision_force_intensity;
+    pc.vdamp = phys->viscous_damping; pc.cs = phys->collision_spring_coeff; pc.cd = phys->collision_damping_coeff;
+    pc.csh = phys->collision_shear_coeff; pc.brake = phys->max_cell_size_factor_before_brake;
+    pc.bint = phys->big_particle_braking_intensity;
+    for (int d = 0; d < 3; ++d) { pc.initv[d] = phys->init_velocity[d]; pc.G[d] = phys->gravity[d]; }
+    pc.impact2 = phys->vein_impact_distance * phys->vein_impact_distance;
+    pc.minimal2 = phys->vein_impact_minimal_force_distance * phys->vein_impact_minimal_force_distance;
+    pc.cyl_r = phys->cylinder_radius;
+    pc.spawn_y = phys->min_spawn_y;
+    const float my = phys->grid_y_margin, mxz = phys->grid_xz_margin;
+    pc.thr_upper = bmax[1] - 3.0f * my / 4.0f;
+    pc.thr_lower = bmin[1] + my / 2.0f;
+    pc.thr_right = bmax[0] - mxz / 2.0f;
+    pc.thr_left = bmin[0] + mxz / 2.0f;
+    pc.thr_front = bmax[2] - mxz / 2.0f;
+    pc.thr_back = bmin[2] + mxz / 2.0f;
+    pc.use_blood_flow = phys->use_blood_flow;
+    pc.reaction = phys->enable_reaction_force;
+    pc.brake_on = phys->enable_big_cells_brake;
+    pc.seed = phys->seed;
+    pc.nend = mesh->ending_count;
+    for (int e = 0; e < mesh->ending_count; ++e) {
+        for (int d = 0; d < 3; ++d) pc.endc[3 * e + d] = mesh->ending_centers[3 * e + d];
+        pc.endr[e] = mesh->ending_radii[e];
+    }
+    // smallestRadiusInType (simulation_controller.cuh:41)
+    for (int t = 0; t < n_types; ++t) {
+        float m = 3.4e38f;
+        for (int a = 0; a < tt.ppc[t]; ++a) m = std::min(m, hbs[tt.mstart[t] + a]);
+        s->host_smallest.push_back(m);
+    }
+    // ---- work chunks (whole cells of one type per block)
+    std::vector<int32_t> ct, cf, cn;
+    for (int t = 0; t < n_types; ++t) {
+        const int cpb = kBlock / tt.ppc[t];
+        for (int c0 = 0; c0 < tt.cnt[t]; c0 += cpb) {
+            ct.push_back(t);
+            cf.push_back(c0);
+            cn.push_back(std::min(cpb, tt.cnt[t] - c0));
+        }
+    }
+    s->nchunks = (int)ct.size();
+    // ---- device allocations
+    const size_t N = (size_t)np, NV = (size_t)s->nv, NT = (size_t)s->ntri;
+    RC(s->dalloc(&s->pos, 3 * N)); RC(s->dalloc(&s->vel, 3 * N)); RC(s->dalloc(&s->frc, 3 * N));
+    RC(s->dalloc(&s->centers, 3 * (size_t)nc));
+    RC(s->dalloc(&s->bs, nm)); RC(s->dalloc(&s->r0, nm)); RC(s->dalloc(&s->model, 3 * nm));
+    RC(s->dalloc(&s->sp_off, nm + 1)); RC(s->dalloc(&s->sp_b, hb.size())); RC(s->dalloc(&s->sp_L, hL.size()));
+    RC(s->dalloc(&s->chunk_type, ct.size())); RC(s->dalloc(&s->chunk_first, cf.size())); RC(s->dalloc(&s->chunk_n, cn.size()));
+    RC(s->dalloc(&s->X, 3 * NV)); RC(s->dalloc(&s->V, 3 * NV)); RC(s->dalloc(&s->Fv, 3 * NV));
+    RC(s->dalloc(&s->nbr, kVeinSlots * NV)); RC(s->dalloc(&s->nbl, kVeinSlots * NV));
+    RC(s->dalloc(&s->tidx, 3 * NT)); RC(s->dalloc(&s->tcent, 3 * NT));
+    RC(s->dalloc(&s->vcount, NV + 1)); RC(s->dalloc(&s->vstart, NV + 2));
+    RC(s->dalloc(&s->hits, 3 * N)); RC(s->dalloc(&s->seg, 3 * N));
+    RC(s->dalloc(&s->hctrl, 16));
+    s->vntiles = scan_tiles(s->nv);
+    RC(s->dalloc(&s->vstatus, (size_t)s->vntiles));
+    RC(s->dalloc(&s->flags, 4)); RC(s->dalloc(&s->step, 1));
+    // ---- uploads
+    HIPCHK(hipMemsetAsync(s->pos, 0, 12 * std::max<size_t>(N, 1), s->stream));
+    HIPCHK(hipMemsetAsync(s->vel, 0, 12 * std::max<size_t>(N, 1), s->stream));
+    HIPCHK(hipMemsetAsync(s->frc, 0, 12 * std::max<size_t>(N, 1), s->stream));
+    HIPCHK(hipMemsetAsync(s->centers, 0, 12 * std::max<int64_t>(nc, 1), s->stream));
+    HIPCHK(hipMemcpyAsync(s->bs, hbs.data(), 4 * nm, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->r0, hr0.data(), 4 * nm, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->model, hmodel.data(), 12 * nm, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->sp_off, hoff.data(), 4 * (nm + 1), hipMemcpyHostToDevice, s->stream));
+    if (!hb.empty()) {
+        HIPCHK(hipMemcpyAsync(s->sp_b, hb.data(), 4 * hb.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->sp_L, hL.data(), 4 * hL.size(), hipMemcpyHostToDevice, s->stream));
+    }
+    if (!ct.empty()) {
+        HIPCHK(hipMemcpyAsync(s->chunk_type, ct.data(), 4 * ct.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->chunk_first, cf.data(), 4 * cf.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->chunk_n, cn.data(), 4 * cn.size(), hipMemcpyHostToDevice, s->stream));
+    }
+    {
+        std::vector<float> soa(3 * NV);
+        for (size_t v = 0; v < NV; ++v)
+            for (int d = 0; d < 3; ++d) soa[d * NV + v] = mesh->positions[3 * v + d];
+        HIPCHK(hipMemcpyAsync(s->X, soa.data(), 12 * NV, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemsetAsync(s->V, 0, 12 * NV, s->stream));
+        HIPCHK(hipMemsetAsync(s->Fv, 0, 12 * NV, s->stream));
+        std::vector<int32_t> ids(kVeinSlots * NV);
+        std::vector<float> lens(kVeinSlots * NV);
+        RC(bcs_vein_neighbors(mesh, ids.data(), lens.data()));
+        HIPCHK(hipMemcpyAsync(s->nbr, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->nbl, lens.data(), 4 * lens.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->tidx, mesh->indices, 12 * NT, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));  // host staging vectors die at the end of this block
+    }
+    HIPCHK(hipMemsetAsync(s->vcount, 0, 4 * (NV + 1), s->stream));
+    HIPCHK(hipMemsetAsync(s->hctrl, 0, 64, s->stream));
+    HIPCHK(hipMemsetAsync(s->flags, 0, 4, s->stream));
+    HIPCHK(hipMemsetAsync(s->step, 0, 8, s->stream));
+    // ---- grids
+    const GridGeom pgeom = make_geom(bmin, bmax, phys->particle_cell);
+    const GridGeom tgeom = make_geom(bmin, bmax, phys->triangle_cell);
+    RC(grid_alloc(s->pg, pgeom, s->np, true, true, s->stream));
+    RC(grid_alloc(s->tg, tgeom, s->ntri, false, false, s->stream));
+    RC(build_triangle_grid(s.get()));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    *out = s.release();
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_destroy(bcs_sim* sim)
+{
+    if (!sim) return BCS_OK;
+    (void)hipSetDevice(sim->device);
+    (void)hipStreamSynchronize(sim->stream);
+    delete sim;
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_get_info(bcs_sim* s, bcs_sim_info* out)
+{
+    RC(set_dev(s));
+    if (!out) return fail(BCS_ERR_INVALID_ARGUMENT, "null info");
+    std::memset(out, 0, sizeof(*out));
+    out->particle_count = s->np;
+    out->blood_cell_count = s->nc;
+    out->type_count = s->nt;
+    out->model_vertex_count = s->nm;
+    out->vertex_count = s->nv;
+    out->triangle_count = s->ntri;
+    for (int d = 0; d < 3; ++d) {
+        out->bounds_min[d] = s->bmin[d];
+        out->bounds_max[d] = s->bmax[d];
+        out->particle_cells[d] = s->pg.g.cnt[d];
+        out->triangle_cells[d] = s->tg.g.cnt[d];
+    }
+    out->particle_cell_count = s->pg.g.cell_count;
+    out->triangle_cell_count = s->tg.g.cell_count;
+    int g = 0;
+    for (int t = 0; t < s->nt; ++t) {
+        out->particle_starts[t] = s->tt.pstart[t];
+        out->cell_type_starts[t] = s->tt.cstart[t];
+        out->model_starts[t] = s->tt.mstart[t];
+        out->graph_starts[t] = g;
+        out->particles_in_cell[t] = s->tt.ppc[t];
+        out->cells_of_type[t] = s->tt.cnt[t];
+        g += s->tt.ppc[t] * s->tt.ppc[t];
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpyAsync(&out->step_index, s->step, 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_device_state(bcs_sim* s, bcs_device_state* out)
+{
+    RC(set_dev(s));
+    if (!out) return fail(BCS_ERR_INVALID_ARGUMENT, "null output");
+    out->positions = s->pos; out->velocities = s->vel; out->forces = s->frc; out->centers = s->centers;
+    out->vein_positions = s->X; out->vein_velocities = s->V; out->vein_forces = s->Fv;
+    out->triangle_centers = s->tcent;
+    out->stream = (void*)s->stream;
+    return BCS_OK;
+}
+
+static int copy_in(bcs_sim* s, float* dst, const float* src, size_t n)
+{
+    if (!src || n == 0) return BCS_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, 4 * n, hipMemcpyHostToDevice, s->stream));
+    return BCS_OK;
+}
+static int copy_out(bcs_sim* s, float* dst, const float* src, size_t n)
+{
+    if (!dst || n == 0) return BCS_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, 4 * n, hipMemcpyDeviceToHost, s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_set_particles(bcs_sim* s, const float* p, const float* v, const float* f)
+{
+    RC(set_dev(s));
+    const size_t n = 3 * (size_t)s->np;
+    RC(copy_in(s, s->pos, p, n)); RC(copy_in(s, s->vel, v, n)); RC(copy_in(s, s->frc, f, n));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+extern "C" int bcs_sim_get_particles(bcs_sim* s, float* p, float* v, float* f)
+{
+    RC(set_dev(s));
+    const size_t n = 3 * (size_t)s->np;
+    RC(copy_out(s, p, s->pos, n)); RC(copy_out(s, v, s->vel, n)); RC(copy_out(s, f, s->frc, n));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+extern "C" int bcs_sim_set_vein(bcs_sim* s, const float* p, const float* v, const float* f)
+{
+    RC(set_dev(s));
+    const size_t n = 3 * (size_t)s->nv;
+    RC(copy_in(s, s->X, p, n)); RC(copy_in(s, s->V, v, n)); RC(copy_in(s, s->Fv, f, n));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+extern "C" int bcs_sim_get_vein(bcs_sim* s, float* p, float* v, float* f)
+{
+    RC(set_dev(s));
+    const size_t n = 3 * (size_t)s->nv;
+    RC(copy_out(s, p, s->X, n)); RC(copy_out(s, v, s->V, n)); RC(copy_out(s, f, s->Fv, n));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+extern "C" int bcs_sim_get_centers(bcs_sim* s, float* c)
+{
+    RC(set_dev(s));
+    RC(copy_out(s, c, s->centers, 3 * (size_t)s->nc));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+extern "C" int bcs_sim_set_step_index(bcs_sim* s, uint64_t step)
+{
+    RC(set_dev(s));
+    HIPCHK(hipMemcpyAsync(s->step, &step, 8, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_calculate_grid(bcs_sim* s, int32_t which)
+{
+    RC(set_dev(s));
+    if (which == BCS_GRID_PARTICLES) RC(enqueue_particle_grid(s));
+    else if (which == BCS_GRID_TRIANGLES) RC(build_triangle_grid(s));
+    else return fail(BCS_ERR_INVALID_ARGUMENT, "unknown grid");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
+{
+    RC(set_dev(s));
+    if (s->pg.n != s->np) return fail(BCS_ERR_STATE, "particle grid not built (call bcs_sim_calculate_grid)");
+    const StepArgs A = s->step_args();
+    const VeinArgs Va = s->vein_args();
+    launch_vein_springs(Va, s->stream);
+    launch_cells(A, s->nchunks, false, s->stream);
+    RC(enqueue_vein_hits(s, Va));
+    launch_vein(Va, 1, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_propagate_all(bcs_sim* s)
+{
+    RC(set_dev(s));
+    const StepArgs A = s->step_args();
+    const VeinArgs Va = s->vein_args();
+    launch_integrate(A, s->stream);
+    launch_vein(Va, 2, s->stream);
+    if (s->phys.use_blood_flow) launch_vein_end(A, s->nchunks, s->stream);
+    launch_bump(s->step, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
+static int ensure_graph(bcs_sim* s)
+{
+    if (s->graph_exec) return BCS_OK;
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_fused_frame(s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(BCS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    s->graph = g;
+    HIPCHK(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_step(bcs_sim* s, int32_t n_steps)
+{
+    RC(set_dev(s));
+    if (n_steps < 0) return fail(BCS_ERR_INVALID_ARGUMENT, "negative step count");
+    if (n_steps == 0) return BCS_OK;
+    RC(ensure_graph(s));
+    for (int32_t k = 0; k < n_steps; ++k) HIPCHK(hipGraphLaunch(s->graph_exec, s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_synchronize(bcs_sim* s)
+{
+    RC(set_dev(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_time_steps(bcs_sim* s, int32_t n_steps, float* total_ms)
+{
+    RC(set_dev(s));
+    if (!total_ms) return fail(BCS_ERR_INVALID_ARGUMENT, "null output");
+    RC(ensure_graph(s));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s->stream));
+    int rc = bcs_sim_step(s, n_steps);
+    HIPCHK(hipEventRecord(e1, s->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *total_ms = ms;
+    return rc;
+}
+
+extern "C" int bcs_sim_profile_kernels(bcs_sim* s, int32_t n_steps, float* ms_out, int32_t capacity,
+                                       int32_t* n_kernels, char* names_out, int32_t names_capacity)
+{
+    RC(set_dev(s));
+    if (!ms_out || !n_kernels || capacity < kFusedKernels || n_steps < 1)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_profile_kernels: bad arguments");
+    std::vector<hipEvent_t> eb(kFusedKernels), ee(kFusedKernels);
+    for (int k = 0; k < kFusedKernels; ++k) {
+        HIPCHK(hipEventCreate(&eb[k]));
+        HIPCHK(hipEventCreate(&ee[k]));
+    }
+    KernelTimer timer{&eb, &ee, 0};
+    std::vector<double> acc(kFusedKernels, 0.0);
+    int rc = BCS_OK;
+    for (int32_t it = 0; it < n_steps && !rc; ++it) {
+        rc = enqueue_fused_frame(s, &timer);
+        if (rc) break;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        for (int k = 0; k < kFusedKernels; ++k) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, eb[k], ee[k]) == hipSuccess) acc[k] += ms;
+        }
+    }
+    for (int k = 0; k < kFusedKernels; ++k) {
+        ms_out[k] = (float)(acc[k] / n_steps);
+        (void)hipEventDestroy(eb[k]);
+        (void)hipEventDestroy(ee[k]);
+    }
+    *n_kernels = kFusedKernels;
+    if (names_out && names_capacity > 0) {
+        std::strncpy(names_out, kKernelNames, (size_t)names_capacity - 1);
+        names_out[names_capacity - 1] = 0;
+    }
+    return rc;
+}
+
+extern "C" int bcs_sim_get_grid(bcs_sim* s, int32_t which, uint32_t* keys_sorted, int32_t* perm, int32_t* n_occupied,
+                                uint32_t* occ_keys, int32_t* occ_start, int32_t* occ_end)
+{
+    RC(set_dev(s));
+    const DeviceGrid& G = which == BCS_GRID_PARTICLES ? s->pg : s->tg;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const size_t n = (size_t)G.n;
+    std::vector<uint32_t> k(n);
+    if (n) HIPCHK(hipMemcpyAsync(k.data(), G.keys_sorted, 4 * n, hipMemcpyDeviceToHost, s->stream));
+    if (keys_sorted && n) std::memcpy(keys_sorted, k.data(), 4 * n);
+    if (perm && n) HIPCHK(hipMemcpyAsync(perm, G.perm, 4 * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // occupied ranges; starts are read back through the device start table
+    std::vector<uint32_t> okeys;
+    std::vector<int32_t> oend;
+    for (size_t i = 0; i < n; ++i) {
+        if (i == 0 || k[i] != k[i - 1]) okeys.push_back(k[i]);
+        if (!okeys.empty()) {
+            if (oend.size() < okeys.size()) oend.push_back(0);
+            oend.back() = (int32_t)i;
+        }
+    }
+    const int32_t no = (int32_t)okeys.size();
+    std::vector<uint32_t> ostart(no);
+    if (no) {
+        uint32_t *dk = nullptr, *dout = nullptr;
+        HIPCHK(hipMalloc(&dk, 4 * (size_t)no));
+        HIPCHK(hipMalloc(&dout, 4 * (size_t)no));
+        HIPCHK(hipMemcpyAsync(dk, okeys.data(), 4 * (size_t)no, hipMemcpyHostToDevice, s->stream));
+        launch_gather_u32(G.start, dk, no, dout, s->stream);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipMemcpyAsync(ostart.data(), dout, 4 * (size_t)no, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        (void)hipFree(dk);
+        (void)hipFree(dout);
+    }
+    for (int32_t q = 0; q < no; ++q) {
+        if (occ_keys) occ_keys[q] = okeys[q];
+        if (occ_start) occ_start[q] = (int32_t)ostart[q];
+        if (occ_end) occ_end[q] = oend[q];
+    }
+    if (n_occupied) *n_occupied = no;
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_get_contacts(bcs_sim* s, int32_t* counts, int32_t* pairs_j, int64_t capacity, int64_t* n_total)
+{
+    RC(set_dev(s));
+    if (!counts || !n_total) return fail(BCS_ERR_INVALID_ARGUMENT, "null output");
+    if (s->pg.n != s->np) return fail(BCS_ERR_STATE, "particle grid not built");
+    const StepArgs A = s->step_args();
+    int32_t* dcounts = nullptr;
+    int64_t* doffs = nullptr;
+    int32_t* dpairs = nullptr;
+    HIPCHK(hipMalloc(&dcounts, 4 * std::max<size_t>(s->np, 1)));
+    launch_contacts(A, 0, dcounts, nullptr, nullptr, s->stream);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpyAsync(counts, dcounts, 4 * (size_t)s->np, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    int64_t total = 0;
+    std::vector<int64_t> offs(s->np + 1, 0);
+    for (int32_t i = 0; i < s->np; ++i) {
+        offs[i] = total;
+        total += counts[i];
+    }
+    *n_total = total;
+    int rc = BCS_OK;
+    if (pairs_j && total <= capacity && total > 0) {
+        if (hipMalloc(&doffs, 8 * (size_t)(s->np + 1)) != hipSuccess || hipMalloc(&dpairs, 4 * (size_t)total) != hipSuccess)
+            rc = fail(BCS_ERR_OUT_OF_MEMORY, "contacts buffer");
+        if (!rc) {
+            (void)hipMemcpyAsync(doffs, offs.data(), 8 * (size_t)(s->np + 1), hipMemcpyHostToDevice, s->stream);
+            launch_contacts(A, 1, dcounts, doffs, dpairs, s->stream);
+            (void)hipStreamSynchronize(s->stream);
+            (void)hipMemcpyAsync(pairs_j, dpairs, 4 * (size_t)total, hipMemcpyDeviceToHost, s->stream);
+            (void)hipStreamSynchronize(s->stream);
+        }
+    }
+    (void)hipFree(dcounts);
+    if (doffs) (void)hipFree(doffs);
+    if (dpairs) (void)hipFree(dpairs);
+    return rc;
+}
+
+extern "C" int bcs_sim_error_flags(bcs_sim* s, int32_t clear, uint32_t* flags)
+{
+    RC(set_dev(s));
+    if (!flags) return fail(BCS_ERR_INVALID_ARGUMENT, "null output");
+    HIPCHK(hipMemcpyAsync(flags, s->flags, 4, hipMemcpyDeviceToHost, s->stream));
+    if (clear) HIPCHK(hipMemsetAsync(s->flags, 0, 4, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_calibration_copy(int64_t n_bytes, int32_t repeats, float* ms)
+{
+    if (n_bytes < 16 || repeats < 1 || !ms) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_calibration_copy");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(BCS_ERR_NO_DEVICE, "no HIP device");
+    void *a = nullptr, *b = nullptr;
+    const int64_t n16 = n_bytes / 16;
+    HIPCHK(hipMalloc(&a, n16 * 16));
+    HIPCHK(hipMalloc(&b, n16 * 16));
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipMemsetAsync(a, 1, n16 * 16, st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch_copy(a, b, n16, st);  // warm-up
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < repeats; ++r) launch_copy(a, b, n16, st);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return BCS_OK;
+}

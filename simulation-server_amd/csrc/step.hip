@@ -1,0 +1,598 @@
+// step.hip -- physics kernels of one frame for gfx950.
+//
+// k_cells is the fused hot kernel.  One block owns a chunk of whole blood
+// cells of one type (<= 256 particles); it stages their (x, v, F) in LDS and,
+// per particle:
+//   centres   calculateBloodCellsCenters      src/objects/blood_cells.cu:44-61
+//   springs   gatherForcesKernel (Heun)        src/objects/blood_cells.cu:66-120
+//   contacts  calculateParticleCollisions      src/simulation/particle_collisions.cuh:104-269
+//   vein ray  detectVeinCollisions             src/simulation/vein_collisions.cu:63-277
+//   integrate propagateParticleForcesKernel    src/objects/blood_cells.cu:155-179   (fused mode)
+//   vein end  handleVeinEnds*                  src/simulation/vein_end.cu:57-138   (fused mode)
+// Reads that the reference shares between threads come from snapshots: the
+// cell's F from LDS (loaded before any write), neighbour x/v from the sorted
+// copies made by the grid build.  Hence state is updated in place.
+// Vein-vertex contributions of ray hits are recorded and summed per vertex
+// in particle-index order by k_vein (deterministic; the reference adds them
+// without atomics).
+#include "step.hpp"
+
+namespace bcs {
+
+__device__ __forceinline__ int32_t model_index(const TypeTable& tt, int t, int32_t id, uint32_t& fl)
+{
+    // bloodCellmodelStart + (id - particlesStart) % particlesInBloodCell, C remainder
+    int32_t m = tt.mstart[t] + (id - tt.pstart[t]) % tt.ppc[t];
+    if (m < 0) { fl |= BCS_FLAG_RADIUS_CLAMPED; m = 0; }
+    if (m >= tt.nm) { fl |= BCS_FLAG_RADIUS_CLAMPED; m = tt.nm - 1; }
+    return m;
+}
+
+__device__ __forceinline__ int type_of_index(const TypeTable& tt, int32_t s)
+{
+    for (int t = 0; t < tt.nt; ++t)
+        if (s >= tt.pstart[t] && s < tt.pstart[t] + tt.cnt[t] * tt.ppc[t]) return t;
+    return tt.nt - 1;
+}
+
+// addResilientForceOnCollision, src/simulation/physics.cuh:133-145 (increment).
+__device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVel, float d2, float radius, float coef,
+                                        uint32_t& fl)
+{
+    f3 dir = normalize(relPos, fl);
+    f3 tang = sub(relVel, smul(dot(relVel, dir), dir));
+    f3 spring = smul(-pc.cs * (radius * 2.0f - sqrtf(d2)), dir);
+    f3 damp = smul(pc.cd, relVel);
+    f3 shear = smul(pc.csh, tang);
+    return smul(coef, add(add(spring, damp), shear));
+}
+
+// realCollisionDetection, src/simulation/vein_collisions.cu:11-45.
+__device__ __forceinline__ bool ray_tri(f3 v0, f3 v1, f3 v2, f3 orig, f3 dir, float& t_out, f3& normal, f3& refl,
+                                        uint32_t& fl)
+{
+    const float EPS = 0.000001f;
+    f3 edge1 = sub(v1, v0);
+    f3 edge2 = sub(v2, v0);
+    f3 h = cross(dir, edge2);
+    float a = dot(edge1, h);
+    if (a > -EPS && a < EPS) return false;
+    float f = 1.0f / a;
+    f3 s = sub(orig, v0);
+    float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 q = cross(s, edge1);
+    float v = f * dot(dir, q);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = f * dot(edge2, q);
+    if (t > EPS) {
+        t_out = t;
+        normal = normalize(cross(edge2, edge1), fl);
+        refl = sub(dir, smul(2.0f * dot(dir, normal), normal));
+        return true;
+    }
+    return false;
+}
+
+// calculateBaricentric (edges v1-v0, v2-v1), src/simulation/vein_collisions.cu:47-61.
+__device__ __forceinline__ f3 bary(f3 point, f3 v0, f3 v1, f3 v2)
+{
+    f3 e0 = sub(v1, v0), e1 = sub(v2, v1), e2 = sub(point, v0);
+    float d00 = dot(e0, e0), d01 = dot(e0, e1), d11 = dot(e1, e1), d20 = dot(e2, e0), d21 = dot(e2, e1);
+    float denom = d00 * d11 - d01 * d01;
+    f3 r;
+    r.x = (d11 * d20 - d01 * d21) / denom;
+    r.y = (d00 * d21 - d01 * d20) / denom;
+    r.z = 1.0f - r.x - r.y;
+    return r;
+}
+
+// Vein-end condition of one particle (src/simulation/vein_end.cu:57-90,
+// thresholds :12-18).
+__device__ __forceinline__ bool vein_end_flag(const PhysConst& pc, bool spheres, f3 p)
+{
+    bool tele = false;
+    if (spheres) {
+        for (int e = 0; e < pc.nend; ++e) {
+            f3 dd = mk(p.x - pc.endc[3 * e], p.y - pc.endc[3 * e + 1], p.z - pc.endc[3 * e + 2]);
+            tele = tele || dot(dd, dd) <= pc.endr[e] * pc.endr[e];
+        }
+    }
+    tele = tele || p.y <= pc.thr_lower || p.y >= pc.thr_upper || p.x <= pc.thr_left || p.x >= pc.thr_right ||
+           p.z <= pc.thr_back || p.z >= pc.thr_front;
+    return tele;
+}
+
+// Teleported position of model vertex a of a cell (vein_end.cu:103-106):
+// ((U - 0.5) * 1.2) * R + m_a - m_0 in x and z, minSpawnY + m_a.y - m_0.y in y.
+__device__ __forceinline__ f3 teleport_pos(const PhysConst& pc, const float* model, int m0, int a, float u1, float u2)
+{
+    const float* ma = model + 3 * (m0 + a);
+    const float* mz = model + 3 * m0;
+    f3 r;
+    r.x = (u1 - 0.5f) * 1.2f * pc.cyl_r + ma[0] - mz[0];
+    r.y = pc.spawn_y + ma[1] - mz[1];
+    r.z = (u2 - 0.5f) * 1.2f * pc.cyl_r + ma[2] - mz[2];
+    return r;
+}
+
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
+{
+    __shared__ float sx[kBlock], sy[kBlock], sz[kBlock];
+    __shared__ float svx[kBlock], svy[kBlock], svz[kBlock];
+    __shared__ float sfx[kBlock], sfy[kBlock], sfz[kBlock];
+    __shared__ int s_tele[kBlock];
+
+    const TypeTable& tt = A.tt;
+    const PhysConst& pc = A.pc;
+    const int l = threadIdx.x;
+    // reset the vein-hit scan control for this frame (k_scan runs after us)
+    {
+        const int64_t gid = (int64_t)blockIdx.x * blockDim.x + l;
+        for (int64_t t = gid; t < A.vntiles; t += (int64_t)gridDim.x * blockDim.x) A.vstatus[t] = 0ull;
+        if (gid == 0) A.hctrl[1] = 0u;
+    }
+    const int T = A.chunk_type[blockIdx.x];
+    const int first = A.chunk_first[blockIdx.x];
+    const int ncell = A.chunk_n[blockIdx.x];
+    const int p = tt.ppc[T];
+    const int nloc = ncell * p;
+    const bool act = l < nloc;
+    const int a = act ? l % p : 0;
+    const int cl = act ? l / p : 0;
+    const int64_t np = tt.np;
+    const int32_t i = tt.pstart[T] + first * p + l;
+    if (l < ncell) s_tele[l] = 0;
+    if (act) {
+        sx[l] = A.pos[i]; sy[l] = A.pos[np + i]; sz[l] = A.pos[2 * np + i];
+        svx[l] = A.vel[i]; svy[l] = A.vel[np + i]; svz[l] = A.vel[2 * np + i];
+        sfx[l] = A.frc[i]; sfy[l] = A.frc[np + i]; sfz[l] = A.frc[2 * np + i];
+    }
+    __syncthreads();
+
+    uint32_t fl = 0;
+    f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
+    if (act) {
+        x1 = mk(sx[l], sy[l], sz[l]);
+        v1 = mk(svx[l], svy[l], svz[l]);
+        const f3 F0 = mk(sfx[l], sfy[l], sfz[l]);
+        // ---- cell centre (sequential sum in vertex order)
+        const int base = cl * p;
+        f3 cen = mk(0.f, 0.f, 0.f);
+        for (int b = 0; b < p; ++b) cen = add(cen, mk(sx[base + b], sy[base + b], sz[base + b]));
+        cen = sdiv(cen, (float)p);
+        const int32_t cell = tt.cstart[T] + first + cl;
+        if (a == 0) {
+            A.centers[cell] = cen.x;
+            A.centers[tt.nc + cell] = cen.y;
+            A.centers[2 * (int64_t)tt.nc + cell] = cen.z;
+        }
+        // ---- springs + environment (F of neighbours from the LDS snapshot)
+        const int ma = tt.mstart[T] + a;
+        f3 newF = mk(0.f, 0.f, 0.f);
+        const int e0 = A.sp_off[ma], e1 = A.sp_off[ma + 1];
+        for (int e = e0; e < e1; ++e) {
+            const int nb = base + A.sp_b[e];
+            const float L = A.sp_L[e];
+            f3 dP = sub(x1, mk(sx[nb], sy[nb], sz[nb]));
+            f3 dv = sub(v1, mk(svx[nb], svy[nb], svz[nb]));
+            f3 f2 = mk(sfx[nb], sfy[nb], sfz[nb]);
+            f3 shift = normalize(smul(-1.0f, dP), fl);
+            f3 dvv = add(dv, smul(pc.dt, sub(F0, f2)));
+            float mag = (len(dP) - L) * pc.k + dot(normalize(dP, fl), dvv) * pc.d;
+            newF = add(newF, smul(mag, shift));
+        }
+        const float ratio = len(sub(x1, cen)) / A.r0[ma];
+        f3 env;
+        const f3 G = mk(pc.G[0], pc.G[1], pc.G[2]);
+        if (pc.brake_on && ratio > pc.brake) env = sub(G, smul(pc.vdamp * ratio * pc.bint, v1));
+        else env = sub(G, smul(pc.vdamp, v1));
+        newF = add(newF, env);
+        F = sdiv(add(F0, newF), 2.0f);
+
+        // ---- particle-particle contacts over the 27 neighbour cells
+        {
+            const GridGeom& g = A.pg;
+            int Tl = T;
+            if (tt.nt > 1) Tl = type_of_index(tt, A.p_slot[i]);
+            const float r1 = A.bs[model_index(tt, Tl, i, fl)];
+            uint32_t dummy = 0;
+            const uint32_t cellId = cell_key(g, x1.x, x1.y, x1.z, dummy);
+            const int32_t xId = sat_i32((x1.x - g.minv[0]) / g.cell[0]);
+            const int32_t yId = sat_i32((x1.y - g.minv[1]) / g.cell[1]);
+            const int32_t zId = sat_i32((x1.z - g.minv[2]) / g.cell[2]);
+            int xl, xh, yl, yh, zl, zh;
+            axis_range_i(xId, g.cnt[0], xl, xh);
+            axis_range_i(yId, g.cnt[1], yl, yh);
+            axis_range_i(zId, g.cnt[2], zl, zh);
+            const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
+            const int64_t ss = A.s_stride;
+            for (int dx = xl; dx <= xh; ++dx)
+                for (int dy = yl; dy <= yh; ++dy)
+                    for (int dz = zl; dz <= zh; ++dz) {
+                        const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
+                        if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
+                        const uint32_t s0 = A.p_start[nk];
+                        if (s0 >= (uint32_t)np || A.p_keys[s0] != (uint32_t)nk) continue;
+                        for (uint32_t u = s0; u < (uint32_t)np && A.p_keys[u] == (uint32_t)nk; ++u) {
+                            const int32_t j = A.p_perm[u];
+                            if (j == i) continue;
+                            const float r2 = A.bs[model_index(tt, Tl, j, fl)];
+                            const f3 p2 = mk(A.xs[u], A.xs[ss + u], A.xs[2 * ss + u]);
+                            const f3 rel = sub(x1, p2);
+                            const float d2 = dot(rel, rel);
+                            const float minD = r1 + r2;
+                            if (d2 <= minD * minD && d2 >= 0.0001f) {
+                                const f3 rv = sub(v1, mk(A.vs[u], A.vs[ss + u], A.vs[2 * ss + u]));
+                                F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
+                            }
+                        }
+                    }
+        }
+
+        // ---- particle-vein ray test (first hit in x->y->z cell order)
+        {
+            const GridGeom& g = A.tg;
+            const f3 dir = normalize(v1, fl);
+            float rt = 1e10f;
+            f3 normal = mk(0.f, 0.f, 0.f), refl = mk(0.f, 0.f, 0.f);
+            int32_t obj = 0;
+            bool hit = false;
+            uint32_t dummy = 0;
+            const uint32_t cellId = cell_key(g, x1.x, x1.y, x1.z, dummy);
+            const uint32_t xId = sat_u32((x1.x - g.minv[0]) / g.cell[0]);
+            const uint32_t yId = sat_u32((x1.y - g.minv[1]) / g.cell[1]);
+            const uint32_t zId = sat_u32((x1.z - g.minv[2]) / g.cell[2]);
+            int xl, xh, yl, yh, zl, zh;
+            axis_range_u(xId, g.cnt[0], xl, xh);
+            axis_range_u(yId, g.cnt[1], yl, yh);
+            axis_range_u(zId, g.cnt[2], zl, zh);
+            const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
+            const int64_t nv = A.nv;
+            for (int dx = xl; dx <= xh && !hit; ++dx)
+                for (int dy = yl; dy <= yh && !hit; ++dy)
+                    for (int dz = zl; dz <= zh && !hit; ++dz) {
+                        const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
+                        if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
+                        const uint32_t s0 = A.t_start[nk];
+                        if (s0 >= (uint32_t)A.ntri || A.t_keys[s0] != (uint32_t)nk) continue;
+                        for (uint32_t u = s0; u < (uint32_t)A.ntri && A.t_keys[u] == (uint32_t)nk; ++u) {
+                            const int32_t tri = A.t_perm[u];
+                            const uint32_t i0 = A.tidx[3 * tri], i1 = A.tidx[3 * tri + 1], i2 = A.tidx[3 * tri + 2];
+                            const f3 v0 = load3(A.X, nv, i0), va = load3(A.X, nv, i1), vb = load3(A.X, nv, i2);
+                            if (ray_tri(v0, va, vb, x1, dir, rt, normal, refl, fl)) {
+                                obj = tri;
+                                hit = true;
+                                break;
+                            }
+                        }
+                    }
+            const f3 relPos = sub(x1, add(x1, smul(rt, dir)));
+            const float d2 = dot(relPos, relPos);
+            if (hit && d2 <= pc.impact2) {
+                if (d2 > pc.minimal2) {
+                    if (pc.reaction) {
+                        f3 resp = sdiv(smul(-1.0f * dot(F, normal), normal), dot(normal, normal));
+                        F = add(F, resp);
+                    } else {
+                        const float rad = A.bs[model_index(tt, T, i, fl)];
+                        F = add(F, resilient(pc, relPos, v1, d2, rad, 0.5f, fl));
+                    }
+                }
+                const float speed = len(v1);
+                const f3 dv = smul(1.0f, sub(smul(pc.vcd * speed, refl), v1));
+                const f3 ds = smul(pc.vcfi, v1);
+                v1 = add(v1, dv);
+                const uint32_t i0 = A.tidx[3 * obj], i1 = A.tidx[3 * obj + 1], i2 = A.tidx[3 * obj + 2];
+                const f3 v0 = load3(A.X, nv, i0), va = load3(A.X, nv, i1), vb = load3(A.X, nv, i2);
+                const f3 bc = bary(add(x1, smul(rt, dir)), v0, va, vb);
+                const f3 c0 = smul(bc.x, ds), c1 = smul(bc.y, ds), c2 = smul(bc.z, ds);
+                const uint32_t h = atomicAdd(&A.hctrl[0], 1u);
+                A.hits[3 * (size_t)h + 0] = make_float4(c0.x, c0.y, c0.z, __int_as_float(i));
+                A.hits[3 * (size_t)h + 1] = make_float4(c1.x, c1.y, c1.z, __int_as_float(obj));
+                A.hits[3 * (size_t)h + 2] = make_float4(c2.x, c2.y, c2.z, 0.0f);
+                atomicAdd(&A.vcount[i0], 1u);
+                atomicAdd(&A.vcount[i1], 1u);
+                atomicAdd(&A.vcount[i2], 1u);
+            }
+        }
+    }
+
+    if (FUSED) {
+        // ---- integrate (Heun / trapezoid): v0 = v/gpuCount; v = v0 + dt*F; x += 0.5*dt*(v + v0)
+        if (act) {
+            const f3 v0 = sdiv(v1, 1.0f);
+            v1 = add(v0, smul(pc.dt, F));
+            x1 = add(x1, smul(0.5f * pc.dt, add(v1, v0)));
+            if (pc.use_blood_flow && vein_end_flag(pc, tt.end_spheres[T] != 0, x1)) s_tele[cl] = 1;
+        }
+        __syncthreads();
+        if (act && pc.use_blood_flow && s_tele[cl]) {
+            const int32_t cell = tt.cstart[T] + first + cl;
+            const uint64_t stp = *A.step;
+            const float u1 = uniform01(pc.seed, stp, (uint64_t)cell, 0u);
+            const float u2 = uniform01(pc.seed, stp, (uint64_t)cell, 1u);
+            x1 = teleport_pos(pc, A.model, tt.mstart[T], a, u1, u2);
+            v1 = mk(pc.initv[0], pc.initv[1], pc.initv[2]);
+        }
+        if (act) {
+            A.pos[i] = x1.x; A.pos[np + i] = x1.y; A.pos[2 * np + i] = x1.z;
+        }
+    }
+    if (act) {
+        A.vel[i] = v1.x; A.vel[np + i] = v1.y; A.vel[2 * np + i] = v1.z;
+        A.frc[i] = F.x; A.frc[np + i] = F.y; A.frc[2 * np + i] = F.z;
+    }
+    if (fl) atomicOr(A.flags, fl);
+}
+
+// propagateParticleForcesKernel, src/objects/blood_cells.cu:155-179 (API path).
+__global__ void __launch_bounds__(256) k_integrate(StepArgs A)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = A.tt.np;
+    if (i >= np) return;
+    const f3 F = load3(A.frc, np, i);
+    const f3 v0 = sdiv(load3(A.vel, np, i), 1.0f);
+    const f3 v = add(v0, smul(A.pc.dt, F));
+    store3(A.vel, np, i, v);
+    store3(A.pos, np, i, add(load3(A.pos, np, i), smul(0.5f * A.pc.dt, add(v, v0))));
+}
+
+// HandleVeinEnd, src/simulation/vein_end.cu:141-173 (API path; one block per
+// chunk of whole cells so the per-cell OR is a block-local reduction).
+__global__ void __launch_bounds__(kBlock) k_vein_end(StepArgs A)
+{
+    __shared__ int s_tele[kBlock];
+    const TypeTable& tt = A.tt;
+    const int l = threadIdx.x;
+    const int T = A.chunk_type[blockIdx.x], first = A.chunk_first[blockIdx.x], ncell = A.chunk_n[blockIdx.x];
+    const int p = tt.ppc[T];
+    const bool act = l < ncell * p;
+    const int a = act ? l % p : 0, cl = act ? l / p : 0;
+    const int64_t np = tt.np;
+    const int32_t i = tt.pstart[T] + first * p + l;
+    if (l < ncell) s_tele[l] = 0;
+    __syncthreads();
+    f3 x = mk(0.f, 0.f, 0.f);
+    if (act) {
+        x = load3(A.pos, np, i);
+        if (vein_end_flag(A.pc, tt.end_spheres[T] != 0, x)) s_tele[cl] = 1;
+    }
+    __syncthreads();
+    if (act && s_tele[cl]) {
+        const int32_t cell = tt.cstart[T] + first + cl;
+        const uint64_t stp = *A.step;
+        const float u1 = uniform01(A.pc.seed, stp, (uint64_t)cell, 0u);
+        const float u2 = uniform01(A.pc.seed, stp, (uint64_t)cell, 1u);
+        store3(A.pos, np, i, teleport_pos(A.pc, A.model, tt.mstart[T], a, u1, u2));
+        store3(A.vel, np, i, mk(A.pc.initv[0], A.pc.initv[1], A.pc.initv[2]));
+    }
+}
+
+__global__ void k_bump_step(uint64_t* step) { *step += 1ull; }
+
+// Vein springs: gatherForcesKernel, src/objects/vein_triangles.cu:126-154.
+// Forces were zeroed by the previous integrate, so Fv = 0 + spring force.
+__global__ void __launch_bounds__(256) k_vein_springs(VeinArgs A)
+{
+    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= A.nv) return;
+    const int64_t nv = A.nv;
+    uint32_t fl = 0;
+    const f3 P = load3(A.X, nv, id);
+    const f3 Vv = load3(A.V, nv, id);
+    f3 force = mk(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int s = 0; s < kVeinSlots; ++s) {
+        const int32_t nb = A.nbr[s * nv + id];
+        if (nb != -1) {
+            const float L = A.nbl[s * nv + id];
+            const f3 NP = load3(A.X, nv, nb);
+            const f3 NV = load3(A.V, nv, nb);
+            const float sf = (len(sub(P, NP)) - L) * A.vk + dot(normalize(sub(P, NP), fl), sub(Vv, NV)) * A.vd;
+            force = add(force, smul(sf, normalize(sub(NP, P), fl)));
+        }
+    }
+    const f3 F0 = load3(A.Fv, nv, id);
+    store3(A.Fv, nv, id, add(F0, force));
+    if (fl) atomicOr(A.flags, fl);
+}
+
+// Distribute recorded hits into per-vertex segments (order within a segment
+// is fixed later by the particle index).
+__global__ void __launch_bounds__(256) k_vein_scatter(VeinArgs A)
+{
+    const uint32_t nh = A.hctrl[0];
+    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+        const float4 r0 = A.hits[3 * (size_t)h], r1 = A.hits[3 * (size_t)h + 1], r2 = A.hits[3 * (size_t)h + 2];
+        const int32_t i = __float_as_int(r0.w);
+        const int32_t tri = __float_as_int(r1.w);
+        const float4 rr[3] = {r0, r1, r2};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = A.tidx[3 * tri + k];
+            const uint32_t old = atomicSub(&A.vcount[v], 1u);
+            const uint32_t pos = A.vstart[v] + old - 1u;
+            A.seg[pos] = make_float4(rr[k].x, rr[k].y, rr[k].z, __int_as_float(i * 4 + k));
+        }
+    }
+}
+
+// Per vertex: add the recorded contributions in (particle, corner) order, then
+// (mode & 2) integrate: V += dt*Fv; X += dt*V; Fv = 0
+// (propagateForcesIntoPositionsKernel, src/objects/vein_triangles.cu:88-117).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_vein(VeinArgs A)
+{
+    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (MODE & 1) {
+        if (id == 0) A.hctrl[0] = 0u;  // hits consumed (k_vein_scatter ran before)
+    }
+    if (MODE & 4) {
+        if (id == 0) *A.step += 1ull;
+    }
+    if (id >= A.nv) return;
+    const int64_t nv = A.nv;
+    f3 F = load3(A.Fv, nv, id);
+    if (MODE & 1) {
+        const uint32_t s0 = A.vstart[id], s1 = A.vstart[id + 1];
+        int32_t last = -1;
+        for (uint32_t r = s0; r < s1; ++r) {
+            int32_t best = 0x7FFFFFFF;
+            float4 bv = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (uint32_t u = s0; u < s1; ++u) {
+                const float4 e = A.seg[u];
+                const int32_t key = __float_as_int(e.w);
+                if (key > last && key < best) {
+                    best = key;
+                    bv = e;
+                }
+            }
+            F = add(F, mk(bv.x, bv.y, bv.z));
+            last = best;
+        }
+    }
+    if (MODE & 2) {
+        const f3 V = add(load3(A.V, nv, id), smul(A.dt, F));
+        store3(A.V, nv, id, V);
+        store3(A.X, nv, id, add(load3(A.X, nv, id), smul(A.dt, V)));
+        store3(A.Fv, nv, id, mk(0.f, 0.f, 0.f));
+    } else {
+        store3(A.Fv, nv, id, F);
+    }
+}
+
+// Triangle centres, calculateCentersKernel, src/objects/vein_triangles.cu:14-27.
+__global__ void __launch_bounds__(256) k_tri_centers(const float* __restrict__ X, int32_t nv,
+                                                     const uint32_t* __restrict__ idx, int32_t ntri,
+                                                     float* __restrict__ c)
+{
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    const f3 a = load3(X, nv, idx[3 * t]), b = load3(X, nv, idx[3 * t + 1]), d = load3(X, nv, idx[3 * t + 2]);
+    c[t] = (a.x + b.x + d.x) / 3.0f;
+    c[ntri + t] = (a.y + b.y + d.y) / 3.0f;
+    c[2 * (int64_t)ntri + t] = (a.z + b.z + d.z) / 3.0f;
+}
+
+// Contact listing in sorted-slot order (calculateParticleCollisions visit
+// order; the force is not applied).  pass 0 counts, pass 1 writes pairs at
+// offs[i].
+__global__ void __launch_bounds__(256) k_contacts(StepArgs A, int pass, int32_t* counts, const int64_t* offs,
+                                                  int32_t* pairs)
+{
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const TypeTable& tt = A.tt;
+    if (s >= tt.np) return;
+    uint32_t fl = 0;
+    const int64_t np = tt.np;
+    const GridGeom& g = A.pg;
+    const int Tl = type_of_index(tt, s);
+    const int32_t i = A.p_perm[s];
+    const f3 x1 = load3(A.pos, np, i);
+    const float r1 = A.bs[model_index(tt, Tl, i, fl)];
+    const uint32_t cellId = A.p_keys[s];
+    const int32_t xId = sat_i32((x1.x - g.minv[0]) / g.cell[0]);
+    const int32_t yId = sat_i32((x1.y - g.minv[1]) / g.cell[1]);
+    const int32_t zId = sat_i32((x1.z - g.minv[2]) / g.cell[2]);
+    int xl, xh, yl, yh, zl, zh;
+    axis_range_i(xId, g.cnt[0], xl, xh);
+    axis_range_i(yId, g.cnt[1], yl, yh);
+    axis_range_i(zId, g.cnt[2], zl, zh);
+    const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
+    int32_t c = 0;
+    for (int dx = xl; dx <= xh; ++dx)
+        for (int dy = yl; dy <= yh; ++dy)
+            for (int dz = zl; dz <= zh; ++dz) {
+                const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
+                if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
+                const uint32_t s0 = A.p_start[nk];
+                if (s0 >= (uint32_t)np || A.p_keys[s0] != (uint32_t)nk) continue;
+                for (uint32_t u = s0; u < (uint32_t)np && A.p_keys[u] == (uint32_t)nk; ++u) {
+                    const int32_t j = A.p_perm[u];
+                    if (j == i) continue;
+                    const float r2 = A.bs[model_index(tt, Tl, j, fl)];
+                    const f3 rel = sub(x1, load3(A.pos, np, j));
+                    const float d2 = dot(rel, rel);
+                    const float minD = r1 + r2;
+                    if (d2 <= minD * minD && d2 >= 0.0001f) {
+                        if (pass == 1) pairs[offs[i] + c] = j;
+                        ++c;
+                    }
+                }
+            }
+    if (pass == 0) counts[i] = c;
+}
+
+// Gather of table entries (parity views of the cell-start table).
+__global__ void __launch_bounds__(256) k_gather_u32(const uint32_t* __restrict__ table, const uint32_t* __restrict__ idx,
+                                                    int32_t n, uint32_t* __restrict__ out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = table[idx[i]];
+}
+
+// Plain copy (HBM counter calibration).
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ------------------------------------------------------------ launchers --
+
+void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
+{
+    if (nchunks <= 0) return;
+    if (fused) hipLaunchKernelGGL(k_cells<true>, dim3(nchunks), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_cells<false>, dim3(nchunks), dim3(kBlock), 0, st, A);
+}
+void launch_integrate(const StepArgs& A, hipStream_t st)
+{
+    if (A.tt.np > 0) hipLaunchKernelGGL(k_integrate, dim3((A.tt.np + 255) / 256), dim3(256), 0, st, A);
+}
+void launch_vein_end(const StepArgs& A, int nchunks, hipStream_t st)
+{
+    if (nchunks > 0) hipLaunchKernelGGL(k_vein_end, dim3(nchunks), dim3(kBlock), 0, st, A);
+}
+void launch_bump(uint64_t* step, hipStream_t st) { hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(1), 0, st, step); }
+void launch_vein_springs(const VeinArgs& V, hipStream_t st)
+{
+    if (V.nv > 0) hipLaunchKernelGGL(k_vein_springs, dim3((V.nv + 255) / 256), dim3(256), 0, st, V);
+}
+void launch_vein_scatter(const VeinArgs& V, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_vein_scatter, dim3(1024), dim3(256), 0, st, V);
+}
+void launch_vein(const VeinArgs& V, int mode, hipStream_t st)
+{
+    const dim3 grid((std::max(V.nv, 1) + 255) / 256);
+    switch (mode) {
+    case 1: hipLaunchKernelGGL(k_vein<1>, grid, dim3(256), 0, st, V); break;
+    case 2: hipLaunchKernelGGL(k_vein<2>, grid, dim3(256), 0, st, V); break;
+    case 3: hipLaunchKernelGGL(k_vein<3>, grid, dim3(256), 0, st, V); break;
+    case 7: hipLaunchKernelGGL(k_vein<7>, grid, dim3(256), 0, st, V); break;
+    default: break;
+    }
+}
+void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t ntri, float* c, hipStream_t st)
+{
+    if (ntri > 0) hipLaunchKernelGGL(k_tri_centers, dim3((ntri + 255) / 256), dim3(256), 0, st, X, nv, idx, ntri, c);
+}
+void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st)
+{
+    if (A.tt.np > 0)
+        hipLaunchKernelGGL(k_contacts, dim3((A.tt.np + 255) / 256), dim3(256), 0, st, A, pass, counts, offs, pairs);
+}
+void launch_gather_u32(const uint32_t* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_gather_u32, dim3((n + 255) / 256), dim3(256), 0, st, table, idx, n, out);
+}
+void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, (const float4*)src, (float4*)dst, n16);
+}
+
+}  // namespace bcs
