@@ -1,0 +1,213 @@
+"""Benchmark of the MI355X blood-cell physics step (one JSON line on stdout).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-cpu]
+
+Workload (default): config 1 of the plain method -- 20,000 icosahedron cells
+(240,000 particles, FP32) in a Y-bifurcating vein (trunk r 60 x 600, branches
+r 42 x 400 at +/-30 deg, ~200k triangles), dt = 0.005, seeded placement.
+A step is one full frame (particle grid, vein springs, cell springs,
+particle contacts, vein ray collisions, integration, vein-end teleport),
+replayed from a HIP graph; all state is resident in HBM during the timed
+region.  `value` is particle-steps/s.
+
+Roofline (dominant kernel, HBM-bound): algorithmic bytes per launch / its
+average duration measured with HIP events on the library's stream.  The
+algorithmic bytes are stated in DESIGN.md; `traffic` comes from the committed
+rocprofv3 PMC summary when present.
+
+cpu_baseline: the CPU oracle (the C restatement of the reference frame,
+OpenMP) on a bounded sample of the same workload, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+WORKLOADS = {
+    0: "config 0: 500 icosahedron cells (6,000 particles) in a straight cylinder r 40 x 400 (12,800 triangles), dt 0.01",
+    1: "config 1: 20,000 icosahedron cells (240,000 particles) in a Y-bifurcating vein (trunk r 60 x 600, branches r 42 x 400 at +/-30 deg), dt 0.005",
+    2: "config 2: 200,000 cells (70% 42-particle discs, 30% 12-particle icosahedra) in the Y vein scaled 3x, 35% lattice packing, dt 0.005",
+    4: "config 4 geometry: 1,000,000 42-particle discs in the config-2 vein, dt 0.005 (plain method)",
+}
+
+
+def step_bytes(np_, n_occ, nc, nv, nt):
+    """Algorithmic HBM bytes of one plain frame:
+    88 N_p + 16 N_occ + 24 N_c + 144 N_v + 16 N_t."""
+    return 88 * np_ + 16 * n_occ + 24 * nc + 144 * nv + 16 * nt
+
+
+def kernel_bytes(name, np_, n_occ, nc, nv, nt):
+    """Algorithmic HBM bytes of one launch of each frame kernel (every datum
+    counted once, as a perfect cache would move it):
+      k_key        x read 12, previous key read 4, its table entry cleared 8,
+                   key written 4                                   28 / particle
+      k_scatter    key read 4, (key, index) written 8              12 / particle
+      k_rank       (key, index) read 8, sorted key, index, inverse written 12,
+                   x, v gathered 24, float4 sorted copies written 32
+                   + {start, count} per occupied cell 8            76 / particle + 8 / cell
+      k_cells_a    x, v, F read 36, F written 12, sorted copy read once as a
+                   neighbour 16, first-hit key written 4           68 / particle + 12 / blood cell
+      k_cells_b    x, v, F and first-hit key read 40, x, v, F written 36
+                                                                   76 / particle
+      k_vein_springs+k_tri_prep  X, V read 24, 9 neighbour ids + lengths 72,
+                   Fv read + written 24 per vertex; perm + indices 16 and the
+                   float4 triangle data written 48 per triangle    120 / vertex + 64 / triangle
+      k_vein       X, V, Fv, X0 read 48, X, V, Fv written 36, offsets 4
+                                                                   88 / vertex"""
+    table = {
+        "k_key": 28 * np_,
+        "k_scatter": 12 * np_,
+        "k_rank": 76 * np_ + 8 * n_occ,
+        "k_cells_a": 68 * np_ + 12 * nc,
+        "k_cells_b": 76 * np_,
+        "k_vein_springs+k_tri_prep": 120 * nv + 64 * nt,
+        "k_vein": 88 * nv,
+    }
+    return table.get(name)
+
+
+def cpu_baseline(scene, target_s=12.0, max_steps=400):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle_py.set_threads(threads)
+    o = oracle_py.Oracle(scene.physics, scene.types, scene.mesh)
+    o.set_particles(scene.positions, scene.velocities, scene.forces)
+    o.step(2)  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while n < max_steps and time.perf_counter() - t0 < target_s:
+        o.step(1)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {
+        "value": scene.particle_count * n / dt,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} frames (after 2 warm-up) of the same workload on {threads} OpenMP threads, {dt:.1f} s",
+    }
+
+
+def pmc_traffic(kernel, config_id):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
+    p = ROOT / "profiles" / f"{kernel}_config{config_id}_pmc.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", type=int, default=1, choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    bcs = importlib.import_module("simulation-server_amd")
+    if bcs.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    scene = bcs.Scene(args.config, 1.0, seed=args.seed)
+    sim = bcs.Simulation.from_scene(scene, device=0)
+    info = sim.info
+    np_, nc, nv, nt = info.particle_count, info.blood_cell_count, info.vertex_count, info.triangle_count
+
+    # warm-up (also builds and instantiates the frame graph)
+    sim.step(max(args.warmup, 1))
+    # timed region: K graph-replayed frames between two synchronisations
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps, sync=False)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = np_ * args.steps / wall
+
+    flags = sim.error_flags()
+    _, perm, occ_keys, _, _ = sim.get_grid(0)
+    n_occ = len(occ_keys)
+    # per-kernel durations (HIP events on the library stream, un-graphed frames)
+    kern = sim.profile_kernels(args.profile_steps)
+    dominant = max(kern, key=kern.get)
+    dom_ms = kern[dominant]
+    dom_bytes = kernel_bytes(dominant, np_, n_occ, nc, nv, nt)
+    achieved = dom_bytes / (dom_ms * 1e-3) if dom_bytes else None
+    roof = {
+        "kernel": dominant,
+        "bound": "hbm",
+        "achieved": achieved / 1e9 if achieved else None,
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK if achieved else None,
+        "traffic": pmc_traffic(dominant, args.config),
+        "algorithmic_bytes_per_launch": dom_bytes,
+        "avg_launch_ms": dom_ms,
+    }
+    sb = step_bytes(np_, n_occ, nc, nv, nt)
+    step_roof = {
+        "bytes_per_step": sb,
+        "achieved_GBps": sb / (ms_per_step * 1e-3) / 1e9,
+        "frac": sb / (ms_per_step * 1e-3) / HBM_PEAK,
+    }
+    out = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator, seed %d)" % args.seed,
+        "config": {
+            "workload": WORKLOADS[args.config],
+            "particles": np_,
+            "cells": nc,
+            "vein_vertices": nv,
+            "vein_triangles": nt,
+            "occupied_grid_cells": n_occ,
+            "particle_grid": list(info.particle_cells),
+            "triangle_grid": list(info.triangle_cells),
+            "dt": scene.physics.dt,
+        },
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "kernel_ms": kern,
+        "dominant_kernel": dominant,
+        "error_flags": flags,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

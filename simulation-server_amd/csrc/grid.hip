@@ -13,9 +13,9 @@
 //              exactly thrust's stable order (ties by object index); run starts
 //              write the dense cell-start table
 //   k_big      the same for buckets larger than the k_rank threshold (LDS tiles)
-// The dense start table is never cleared: an entry is valid only if
-// keys_sorted[start] == key, which holds for every occupied cell of the current
-// build and for no empty one.
+// Cell table: dense {start, count} per cell id (8 bytes, one load per lookup).
+// Only occupied cells are written; k_key first clears the entries of the
+// previous build's keys, so every entry is either current or {0, 0}.
 #include "grid.hpp"
 
 #include <algorithm>
@@ -32,7 +32,8 @@ constexpr uint64_t kStatusInc = 2ull << 62;
 constexpr uint64_t kSpinLimit = 1ull << 22;
 
 __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict__ pos, int64_t stride, int32_t n,
-                                             int shift, uint32_t* __restrict__ keys, uint32_t* __restrict__ bcount,
+                                             int32_t n_prev, int shift, uint32_t* __restrict__ keys,
+                                             uint2* __restrict__ table, uint32_t* __restrict__ bcount,
                                              uint64_t* __restrict__ status, int32_t ntiles,
                                              uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags)
 {
@@ -42,6 +43,10 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
     if (gid == 0) {
         ctrl[0] = 0u;  // scan ticket
         ctrl[1] = 0u;  // big-bucket count
+    }
+    if (gid < n_prev) {  // clear the previous build's cell of this object
+        const uint32_t old = keys[gid];
+        if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
     }
     uint32_t fl = 0;
     if (gid < n) {
@@ -162,29 +167,25 @@ struct RankOut {
     uint32_t* keys_sorted;
     int32_t* perm;
     int32_t* slot_of;      // nullable
-    uint32_t* start;
+    uint2* table;          // {start, count} per cell
     const float* src_pos;  // sorted copies (nullable)
     const float* src_vel;
     int64_t src_stride;
-    float* xs;
-    float* vs;
-    int64_t dst_stride;
+    float4* xs4;           // (x, y, z, index bits) in sorted order
+    float4* vs4;           // (vx, vy, vz, 0) in sorted order
 };
 
-__device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32_t idx, uint32_t pos, bool first)
+__device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32_t idx, uint32_t pos, bool first,
+                                           uint32_t count)
 {
     o.keys_sorted[pos] = key;
     o.perm[pos] = idx;
     if (o.slot_of) o.slot_of[idx] = (int32_t)pos;
-    if (first) o.start[key] = pos;
-    if (o.xs) {
-        const int64_t s = o.src_stride, d = o.dst_stride;
-        o.xs[pos] = o.src_pos[idx];
-        o.xs[d + pos] = o.src_pos[s + idx];
-        o.xs[2 * d + pos] = o.src_pos[2 * s + idx];
-        o.vs[pos] = o.src_vel[idx];
-        o.vs[d + pos] = o.src_vel[s + idx];
-        o.vs[2 * d + pos] = o.src_vel[2 * s + idx];
+    if (first) o.table[key] = make_uint2(pos, count);
+    if (o.xs4) {
+        const int64_t s = o.src_stride;
+        o.xs4[pos] = make_float4(o.src_pos[idx], o.src_pos[s + idx], o.src_pos[2 * s + idx], __int_as_float(idx));
+        o.vs4[pos] = make_float4(o.src_vel[idx], o.src_vel[s + idx], o.src_vel[2 * s + idx], 0.0f);
     }
 }
 
@@ -199,20 +200,21 @@ __global__ void __launch_bounds__(256) k_rank(const uint32_t* __restrict__ items
     const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
     if (s1 < s0 || s1 > (uint32_t)n) return;  // inconsistent offsets (flagged by k_scatter)
     if (s1 - s0 > kRankThreshold) return;     // handled by k_big
-    uint32_t rank = 0;
+    uint32_t rank = 0, same = 0;
     bool first = true;
     for (uint32_t u = s0; u < s1; ++u) {
         const uint32_t k2 = items_k[u];
         if (k2 < key) {
             ++rank;
         } else if (k2 == key) {
+            ++same;
             if (items_i[u] < idx) {
                 ++rank;
                 first = false;
             }
         }
     }
-    rank_write(o, key, idx, s0 + rank, first);
+    rank_write(o, key, idx, s0 + rank, first, same);
 }
 
 constexpr int kBigChunk = 1024;
@@ -234,7 +236,7 @@ __global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_
             const bool own = q < s1;
             const uint32_t key = own ? items_k[q] : 0u;
             const int32_t idx = own ? items_i[q] : 0;
-            uint32_t rank = 0;
+            uint32_t rank = 0, same = 0;
             bool first = true;
             for (uint32_t cbase = s0; cbase < s1; cbase += kBigChunk) {
                 const uint32_t cn = min((uint32_t)kBigChunk, s1 - cbase);
@@ -249,14 +251,17 @@ __global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_
                         const uint32_t k2 = sk[t];
                         if (k2 < key) {
                             ++rank;
-                        } else if (k2 == key && si[t] < idx) {
-                            ++rank;
-                            first = false;
+                        } else if (k2 == key) {
+                            ++same;
+                            if (si[t] < idx) {
+                                ++rank;
+                                first = false;
+                            }
                         }
                     }
                 }
             }
-            if (own) rank_write(o, key, idx, s0 + rank, first);
+            if (own) rank_write(o, key, idx, s0 + rank, first, same);
         }
     }
 }
@@ -297,13 +302,14 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     GRID_HIP(hipMalloc(&G.perm, n * 4));
     if (with_slot) GRID_HIP(hipMalloc(&G.slot_of, n * 4));
     if (with_copies) {
-        GRID_HIP(hipMalloc(&G.xs, n * 12));
-        GRID_HIP(hipMalloc(&G.vs, n * 12));
+        GRID_HIP(hipMalloc(&G.xs4, n * 16));
+        GRID_HIP(hipMalloc(&G.vs4, n * 16));
     }
-    hipError_t e = hipMalloc(&G.start, geom.cell_count * 4);
+    GRID_HIP(hipMemsetAsync(G.keys, 0xFF, n * 4, st));
+    hipError_t e = hipMalloc(&G.table, geom.cell_count * 8);
     if (e != hipSuccess)
-        return fail(BCS_ERR_OUT_OF_MEMORY, "grid: cell-start table of " + std::to_string(geom.cell_count * 4) + " bytes");
-    GRID_HIP(hipMemsetAsync(G.start, 0xFF, geom.cell_count * 4, st));
+        return fail(BCS_ERR_OUT_OF_MEMORY, "grid: cell table of " + std::to_string(geom.cell_count * 8) + " bytes");
+    GRID_HIP(hipMemsetAsync(G.table, 0, geom.cell_count * 8, st));
     GRID_HIP(hipMalloc(&G.bcount, (G.nb + 1) * 4));
     GRID_HIP(hipMemsetAsync(G.bcount, 0, (G.nb + 1) * 4, st));
     GRID_HIP(hipMalloc(&G.bstart, (G.nb + 2) * 4));
@@ -317,8 +323,8 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
 
 void grid_free(DeviceGrid& G)
 {
-    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.perm, G.slot_of, G.xs, G.vs,
-                    G.start, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
+    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.perm, G.slot_of, G.xs4, G.vs4,
+                    G.table, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     G = DeviceGrid{};
@@ -328,14 +334,17 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
                hipStream_t st, const KernelTimer* timer)
 {
     if (n < 0 || n > G.cap) return fail(BCS_ERR_INVALID_ARGUMENT, "grid: object count exceeds capacity");
+    // Every slot below the capacity clears its previous cell (slots never
+    // built hold the sentinel key): nothing here depends on the previous
+    // build's size, so a captured graph of this sequence stays valid.
+    const int32_t n_prev = G.cap;
     G.n = n;
-    if (n == 0) return BCS_OK;
-    const int blocks = (n + 255) / 256;
+    const int blocks = (std::max(std::max(n, n_prev), 1) + 255) / 256;
     auto tb = [&](int id) { if (timer) timer->begin(id, st); };
     auto te = [&](int id) { if (timer) timer->end(id, st); };
     tb(0);
     hipLaunchKernelGGL(k_key, dim3(std::max(blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride, n,
-                       G.shift, G.keys, G.bcount, G.status, G.ntiles, G.ctrl, flags);
+                       n_prev, G.shift, G.keys, G.table, G.bcount, G.status, G.ntiles, G.ctrl, flags);
     te(0);
     tb(1);
     hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,
@@ -345,8 +354,8 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.bcount, G.items_k,
                        G.items_i, flags);
     te(2);
-    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.start, vel ? pos : nullptr, vel, stride, vel ? G.xs : nullptr,
-              vel ? G.vs : nullptr, (int64_t)G.cap};
+    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, vel ? pos : nullptr, vel, stride, vel ? G.xs4 : nullptr,
+              vel ? G.vs4 : nullptr};
     tb(3);
     hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, o);
     te(3);

@@ -28,20 +28,20 @@ struct DeviceGrid {
     uint64_t nb = 0;            // buckets
     int32_t ntiles = 0;         // scan tiles over nb+1 entries
     uint32_t big_cap = 0;
-    uint32_t* keys = nullptr;         // per object, input order
+    uint32_t* keys = nullptr;         // per object, input order (last build; clears its cells next build)
     uint32_t* items_k = nullptr;      // bucketed (key, index)
     int32_t* items_i = nullptr;
     uint32_t* keys_sorted = nullptr;  // gridCellIds after the sort
     int32_t* perm = nullptr;          // particleIds after the sort
     int32_t* slot_of = nullptr;       // inverse of perm (optional)
-    uint32_t* start = nullptr;        // dense cell-start table [cell_count], validated by keys_sorted
+    uint2* table = nullptr;           // dense {start, count} per cell id; {0, 0} = empty
     uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds
     uint32_t* bstart = nullptr;       // bucket offsets [nb+2]
     uint64_t* status = nullptr;       // scan tile status
     uint32_t* ctrl = nullptr;         // [0] scan ticket, [1] big-bucket count
     uint32_t* biglist = nullptr;
-    float* xs = nullptr;              // positions in sorted order (optional), stride cap
-    float* vs = nullptr;              // velocities in sorted order (optional)
+    float4* xs4 = nullptr;            // (x, y, z, index bits) in sorted order (optional)
+    float4* vs4 = nullptr;            // (vx, vy, vz, 0) in sorted order (optional)
 };
 
 // Initialisation is enqueued on `st` (the owner's stream, which does not

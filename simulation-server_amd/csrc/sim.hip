@@ -16,7 +16,9 @@
 #include <vector>
 
 namespace bcs {
-void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st);
+void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st);
+void launch_ray(const StepArgs& A, hipStream_t st);
+void launch_cells_b(const StepArgs& A, int nchunks, bool fused, hipStream_t st);
 void launch_integrate(const StepArgs& A, hipStream_t st);
 void launch_vein_end(const StepArgs& A, int nchunks, hipStream_t st);
 void launch_bump(uint64_t* step, hipStream_t st);
@@ -26,7 +28,12 @@ void launch_vein(const VeinArgs& V, int mode, hipStream_t st);
 void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t ntri, float* c, hipStream_t st);
 void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st);
 void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
-void launch_gather_u32(const uint32_t* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
+void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
+void launch_tri_prep(const int32_t* perm, int32_t ntri, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
+                     hipStream_t st);
+void launch_tri_aabb(const uint32_t* keys, const int32_t* perm, int32_t ntri, const uint32_t* idx, const float* X,
+                     int32_t nv, float4* aabb, hipStream_t st);
+void launch_vdisp(const float* X, const float* X0, int32_t nv, uint32_t* vdisp, hipStream_t st);
 }  // namespace bcs
 
 using namespace bcs;
@@ -59,6 +66,14 @@ struct bcs_sim {
     float4 *hits = nullptr, *seg = nullptr;
     uint64_t* vstatus = nullptr;
     int32_t vntiles = 0;
+    float4* t_aabb = nullptr;  // triangle-cell boxes at run starts
+    float4* tdata = nullptr;   // per sorted triangle: (v0 | id), e1, e2
+    uint32_t* best = nullptr;  // per particle first-hit key
+    uint4* work = nullptr;     // ray work items
+    uint32_t* wctrl = nullptr;
+    uint32_t work_cap = 0;
+    float* X0 = nullptr;       // vein positions the boxes were made from
+    uint32_t* vdisp = nullptr; // max |X - X0| (float bits)
     // grids
     DeviceGrid pg, tg;
     bool tg_built = false;
@@ -97,11 +112,13 @@ struct bcs_sim {
         A.pos = pos; A.vel = vel; A.frc = frc; A.centers = centers;
         A.bs = bs; A.r0 = r0; A.model = model; A.sp_off = sp_off; A.sp_b = sp_b; A.sp_L = sp_L;
         A.chunk_type = chunk_type; A.chunk_first = chunk_first; A.chunk_n = chunk_n;
-        A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_start = pg.start;
-        A.xs = pg.xs; A.vs = pg.vs; A.s_stride = pg.cap;
-        A.tg = tg.g; A.t_keys = tg.keys_sorted; A.t_perm = tg.perm; A.t_start = tg.start; A.ntri = ntri;
+        A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table;
+        A.xs4 = pg.xs4; A.vs4 = pg.vs4;
+        A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.vcount = vcount; A.hits = hits; A.hctrl = hctrl; A.vstatus = vstatus; A.vntiles = vntiles;
+        A.t_aabb = t_aabb; A.vdisp = vdisp;
+        A.best = best; A.work = work; A.wctrl = wctrl; A.work_cap = work_cap;
         A.flags = flags; A.step = step;
         return A;
     }
@@ -111,6 +128,7 @@ struct bcs_sim {
         Va.nv = nv; Va.X = X; Va.V = V; Va.Fv = Fv; Va.nbr = nbr; Va.nbl = nbl;
         Va.dt = phys.dt; Va.vk = phys.vein_k_sniff; Va.vd = phys.vein_d_fact;
         Va.vcount = vcount; Va.vstart = vstart; Va.hits = hits; Va.seg = seg; Va.hctrl = hctrl; Va.tidx = tidx;
+        Va.X0 = X0; Va.vdisp = vdisp;
         Va.flags = flags; Va.step = step;
         return Va;
     }
@@ -146,11 +164,17 @@ int set_dev(bcs_sim* s)
         if (rc_) return rc_;  \
     } while (0)
 
+// Triangle grid on the triangle centres.  Frozen mode (the reference's
+// single-GPU behaviour): centres and cell boxes come from the creation-time
+// vein, and the boxes are inflated at use by the tracked displacement.
+// Dynamic mode: centres and boxes are recomputed from the current vein.
 int build_triangle_grid(bcs_sim* s)
 {
-    if (s->phys.dynamic_triangle_grid || !s->tg_built)
-        launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
+    const bool dyn = s->phys.dynamic_triangle_grid != 0;
+    if (dyn || !s->tg_built) launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
     RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, nullptr, s->flags, s->stream));
+    if (dyn || !s->tg_built)
+        launch_tri_aabb(s->tg.keys_sorted, s->tg.perm, s->ntri, s->tidx, s->X, s->nv, s->t_aabb, s->stream);
     s->tg_built = true;
     return BCS_OK;
 }
@@ -180,25 +204,34 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
     const VeinArgs Va = s->vein_args();
     tb(kGridKernels + 0);
     launch_vein_springs(Va, st);
+    launch_tri_prep(s->tg.perm, s->ntri, s->tidx, s->X, s->nv, s->tdata, st);
     te(kGridKernels + 0);
     tb(kGridKernels + 1);
-    launch_cells(A, s->nchunks, true, st);
+    launch_cells_a(A, s->nchunks, st);
     te(kGridKernels + 1);
     tb(kGridKernels + 2);
-    RC(scan_exclusive(s->vcount, s->vstart, s->nv, s->vstatus, s->vntiles, s->hctrl + 1, s->flags, st));
+    launch_ray(A, st);
     te(kGridKernels + 2);
     tb(kGridKernels + 3);
-    launch_vein_scatter(Va, st);
+    launch_cells_b(A, s->nchunks, true, st);
     te(kGridKernels + 3);
     tb(kGridKernels + 4);
-    launch_vein(Va, 7, st);
+    RC(scan_exclusive(s->vcount, s->vstart, s->nv, s->vstatus, s->vntiles, s->hctrl + 1, s->flags, st));
     te(kGridKernels + 4);
+    tb(kGridKernels + 5);
+    launch_vein_scatter(Va, st);
+    te(kGridKernels + 5);
+    tb(kGridKernels + 6);
+    launch_vein(Va, 7, st);
+    te(kGridKernels + 6);
     HIPCHK(hipGetLastError());
     return BCS_OK;
 }
 
-const char* kKernelNames = "k_key;k_scan;k_scatter;k_rank;k_big;k_vein_springs;k_cells;k_scan(vein);k_vein_scatter;k_vein";
-constexpr int kFusedKernels = kGridKernels + 5;
+const char* kKernelNames =
+    "k_key;k_scan;k_scatter;k_rank;k_big;k_vein_springs+k_tri_prep;k_cells_a;k_ray;k_cells_b;k_scan(vein);"
+    "k_vein_scatter;k_vein";
+constexpr int kFusedKernels = kGridKernels + 7;
 
 }  // namespace
 
@@ -258,6 +291,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     s->np = tt.np; s->nc = tt.nc; s->nm = tt.nm;
     s->nv = mesh->vertex_count;
     s->ntri = mesh->triangle_count;
+    if (s->ntri >= (1 << 27)) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: at most 2^27 - 1 triangles");
     for (int64_t k = 0; k < 3 * (int64_t)s->ntri; ++k)
         if (mesh->indices[k] >= (uint32_t)s->nv) return fail(BCS_ERR_INVALID_ARGUMENT, "vein index out of range");
     // ---- host tables
@@ -318,6 +352,12 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     pc.brake_on = phys->enable_big_cells_brake;
     pc.seed = phys->seed;
     pc.nend = mesh->ending_count;
+    pc.aabb_static = phys->dynamic_triangle_grid ? 0 : 1;
+    {
+        float mabs = 0.f;
+        for (int d = 0; d < 3; ++d) mabs = std::max(mabs, std::max(std::fabs(bmin[d]), std::fabs(bmax[d])));
+        pc.aabb_margin = 0.05f + 1e-5f * mabs;
+    }
     for (int e = 0; e < mesh->ending_count; ++e) {
         for (int d = 0; d < 3; ++d) pc.endc[3 * e + d] = mesh->ending_centers[3 * e + d];
         pc.endr[e] = mesh->ending_radii[e];
@@ -355,6 +395,9 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     s->vntiles = scan_tiles(s->nv);
     RC(s->dalloc(&s->vstatus, (size_t)s->vntiles));
     RC(s->dalloc(&s->flags, 4)); RC(s->dalloc(&s->step, 1));
+    RC(s->dalloc(&s->t_aabb, 2 * NT)); RC(s->dalloc(&s->tdata, 3 * NT));
+    s->work_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * N, 65536), 0x7FFFFFFFull);
+    RC(s->dalloc(&s->best, N)); RC(s->dalloc(&s->work, s->work_cap)); RC(s->dalloc(&s->wctrl, 4)); RC(s->dalloc(&s->X0, 3 * NV)); RC(s->dalloc(&s->vdisp, 1));
     // ---- uploads
     HIPCHK(hipMemsetAsync(s->pos, 0, 12 * std::max<size_t>(N, 1), s->stream));
     HIPCHK(hipMemsetAsync(s->vel, 0, 12 * std::max<size_t>(N, 1), s->stream));
@@ -378,6 +421,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
         for (size_t v = 0; v < NV; ++v)
             for (int d = 0; d < 3; ++d) soa[d * NV + v] = mesh->positions[3 * v + d];
         HIPCHK(hipMemcpyAsync(s->X, soa.data(), 12 * NV, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->X0, soa.data(), 12 * NV, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemsetAsync(s->V, 0, 12 * NV, s->stream));
         HIPCHK(hipMemsetAsync(s->Fv, 0, 12 * NV, s->stream));
         std::vector<int32_t> ids(kVeinSlots * NV);
@@ -392,6 +436,8 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     HIPCHK(hipMemsetAsync(s->hctrl, 0, 64, s->stream));
     HIPCHK(hipMemsetAsync(s->flags, 0, 4, s->stream));
     HIPCHK(hipMemsetAsync(s->step, 0, 8, s->stream));
+    HIPCHK(hipMemsetAsync(s->wctrl, 0, 16, s->stream));
+    HIPCHK(hipMemsetAsync(s->vdisp, 0, 4, s->stream));
     // ---- grids
     const GridGeom pgeom = make_geom(bmin, bmax, phys->particle_cell);
     const GridGeom tgeom = make_geom(bmin, bmax, phys->triangle_cell);
@@ -492,6 +538,10 @@ extern "C" int bcs_sim_set_vein(bcs_sim* s, const float* p, const float* v, cons
     RC(set_dev(s));
     const size_t n = 3 * (size_t)s->nv;
     RC(copy_in(s, s->X, p, n)); RC(copy_in(s, s->V, v, n)); RC(copy_in(s, s->Fv, f, n));
+    if (p) {  // keep the triangle-cell boxes conservative for the new positions
+        HIPCHK(hipMemsetAsync(s->vdisp, 0, 4, s->stream));
+        launch_vdisp(s->X, s->X0, s->nv, s->vdisp, s->stream);
+    }
     HIPCHK(hipStreamSynchronize(s->stream));
     return BCS_OK;
 }
@@ -535,7 +585,10 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
     const StepArgs A = s->step_args();
     const VeinArgs Va = s->vein_args();
     launch_vein_springs(Va, s->stream);
-    launch_cells(A, s->nchunks, false, s->stream);
+    launch_tri_prep(s->tg.perm, s->ntri, s->tidx, s->X, s->nv, s->tdata, s->stream);
+    launch_cells_a(A, s->nchunks, s->stream);
+    launch_ray(A, s->stream);
+    launch_cells_b(A, s->nchunks, false, s->stream);
     RC(enqueue_vein_hits(s, Va));
     launch_vein(Va, 1, s->stream);
     HIPCHK(hipGetLastError());
@@ -676,7 +729,7 @@ extern "C" int bcs_sim_get_grid(bcs_sim* s, int32_t which, uint32_t* keys_sorted
         HIPCHK(hipMalloc(&dk, 4 * (size_t)no));
         HIPCHK(hipMalloc(&dout, 4 * (size_t)no));
         HIPCHK(hipMemcpyAsync(dk, okeys.data(), 4 * (size_t)no, hipMemcpyHostToDevice, s->stream));
-        launch_gather_u32(G.start, dk, no, dout, s->stream);
+        launch_gather_start(G.table, dk, no, dout, s->stream);
         HIPCHK(hipStreamSynchronize(s->stream));
         HIPCHK(hipMemcpyAsync(ostart.data(), dout, 4 * (size_t)no, hipMemcpyDeviceToHost, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));

@@ -1,17 +1,23 @@
 // step.hip -- physics kernels of one frame for gfx950.
 //
-// k_cells is the fused hot kernel.  One block owns a chunk of whole blood
-// cells of one type (<= 256 particles); it stages their (x, v, F) in LDS and,
-// per particle:
-//   centres   calculateBloodCellsCenters      src/objects/blood_cells.cu:44-61
-//   springs   gatherForcesKernel (Heun)        src/objects/blood_cells.cu:66-120
-//   contacts  calculateParticleCollisions      src/simulation/particle_collisions.cuh:104-269
-//   vein ray  detectVeinCollisions             src/simulation/vein_collisions.cu:63-277
-//   integrate propagateParticleForcesKernel    src/objects/blood_cells.cu:155-179   (fused mode)
-//   vein end  handleVeinEnds*                  src/simulation/vein_end.cu:57-138   (fused mode)
+// The particle part of a frame is three kernels.  k_cells_a and k_cells_b
+// give one block a chunk of whole blood cells of one type (<= 256 particles).
+//   k_cells_a  centres   calculateBloodCellsCenters   src/objects/blood_cells.cu:44-61
+//              springs   gatherForcesKernel (Heun)     src/objects/blood_cells.cu:66-120
+//              contacts  calculateParticleCollisions   src/simulation/particle_collisions.cuh:104-269
+//              vein ray candidates: the triangle cells of the 27-cell
+//              neighbourhood whose box the ray can reach, as work items of
+//              <= 16 sorted triangles
+//   k_ray      one thread per work item: first Moller-Trumbore hit of the item,
+//              combined per particle with atomicMin over (cell-visit rank,
+//              sorted slot) = the reference's first hit in x->y->z order
+//              (src/simulation/vein_collisions.cuh:60-93)
+//   k_cells_b  hit response   detectVeinCollisions      src/simulation/vein_collisions.cu:234-275
+//              integrate      propagateParticleForcesKernel src/objects/blood_cells.cu:155-179 (fused)
+//              vein end       handleVeinEnds*           src/simulation/vein_end.cu:57-138 (fused)
 // Reads that the reference shares between threads come from snapshots: the
 // cell's F from LDS (loaded before any write), neighbour x/v from the sorted
-// copies made by the grid build.  Hence state is updated in place.
+// copies made by the grid build; x and v are only written by k_cells_b.
 // Vein-vertex contributions of ray hits are recorded and summed per vertex
 // in particle-index order by k_vein (deterministic; the reference adds them
 // without atomics).
@@ -74,6 +80,32 @@ __device__ __forceinline__ bool ray_tri(f3 v0, f3 v1, f3 v2, f3 orig, f3 dir, fl
     return false;
 }
 
+// The same test with edge1 = v1 - v0 and edge2 = v2 - v0 precomputed (the
+// identical float subtractions, done once per frame by k_tri_prep).
+__device__ __forceinline__ bool ray_tri_pre(f3 v0, f3 edge1, f3 edge2, f3 orig, f3 dir, float& t_out, f3& normal,
+                                            f3& refl, uint32_t& fl)
+{
+    const float EPS = 0.000001f;
+    f3 h = cross(dir, edge2);
+    float a = dot(edge1, h);
+    if (a > -EPS && a < EPS) return false;
+    float f = 1.0f / a;
+    f3 s = sub(orig, v0);
+    float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 q = cross(s, edge1);
+    float v = f * dot(dir, q);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = f * dot(edge2, q);
+    if (t > EPS) {
+        t_out = t;
+        normal = normalize(cross(edge2, edge1), fl);
+        refl = sub(dir, smul(2.0f * dot(dir, normal), normal));
+        return true;
+    }
+    return false;
+}
+
 // calculateBaricentric (edges v1-v0, v2-v1), src/simulation/vein_collisions.cu:47-61.
 __device__ __forceinline__ f3 bary(f3 point, f3 v0, f3 v1, f3 v2)
 {
@@ -85,6 +117,28 @@ __device__ __forceinline__ f3 bary(f3 point, f3 v0, f3 v1, f3 v2)
     r.y = (d00 * d21 - d01 * d20) / denom;
     r.z = 1.0f - r.x - r.y;
     return r;
+}
+
+// Conservative ray / box test (slab method) on a box inflated by `m`: false
+// only if the ray (t >= 0) certainly misses every point of the box.  Used to
+// skip triangle cells that cannot hold a hit; the Moller-Trumbore test itself
+// is unchanged, so the first hit in visit order is unchanged.
+__device__ __forceinline__ bool ray_box(f3 o, f3 inv, f3 dir, float4 lo, float4 hi, float m)
+{
+    float tmin = 0.0f, tmax = 3.0e38f;
+    const float os[3] = {o.x, o.y, o.z}, is[3] = {inv.x, inv.y, inv.z}, ds[3] = {dir.x, dir.y, dir.z};
+    const float ls[3] = {lo.x - m, lo.y - m, lo.z - m}, hs[3] = {hi.x + m, hi.y + m, hi.z + m};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (ds[d] == 0.0f) {
+            if (os[d] < ls[d] || os[d] > hs[d]) return false;
+        } else {
+            float t1 = (ls[d] - os[d]) * is[d], t2 = (hs[d] - os[d]) * is[d];
+            tmin = fmaxf(tmin, fminf(t1, t2));
+            tmax = fminf(tmax, fmaxf(t1, t2));
+        }
+    }
+    return tmax >= tmin;
 }
 
 // Vein-end condition of one particle (src/simulation/vein_end.cu:57-90,
@@ -116,18 +170,109 @@ __device__ __forceinline__ f3 teleport_pos(const PhysConst& pc, const float* mod
     return r;
 }
 
-template <bool FUSED>
-__global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+constexpr uint32_t kScanSerial = 0xFFFFFFFEu;  // work list full: k_cells_b scans serially
+constexpr uint32_t kSlotBits = 27;             // best = rank << 27 | sorted triangle slot
+constexpr uint32_t kItemTris = 16;
+
+// Triangle-grid neighbourhood of a particle (face clipped, unsigned ids as
+// in src/simulation/vein_collisions.cu:80-232).
+struct RayCells {
+    uint32_t cellId;
+    int xl, xh, yl, yh, zl, zh;
+    int64_t cX, cXY;
+};
+
+__device__ __forceinline__ RayCells ray_cells(const GridGeom& g, f3 x1)
+{
+    RayCells rc;
+    uint32_t dummy = 0;
+    rc.cellId = cell_key(g, x1.x, x1.y, x1.z, dummy);
+    const uint32_t xId = sat_u32((x1.x - g.minv[0]) / g.cell[0]);
+    const uint32_t yId = sat_u32((x1.y - g.minv[1]) / g.cell[1]);
+    const uint32_t zId = sat_u32((x1.z - g.minv[2]) / g.cell[2]);
+    axis_range_u(xId, g.cnt[0], rc.xl, rc.xh);
+    axis_range_u(yId, g.cnt[1], rc.yl, rc.yh);
+    axis_range_u(zId, g.cnt[2], rc.zl, rc.zh);
+    rc.cX = g.cnt[0];
+    rc.cXY = (int64_t)g.cnt[0] * g.cnt[1];
+    return rc;
+}
+
+// Table entries of the 9 cells of x-slab dx (independent loads) and the mask
+// of occupied cells whose box the ray can reach.
+__device__ __forceinline__ uint32_t slab_cells(const StepArgs& A, const RayCells& rc, int dx, f3 x1, f3 inv, f3 dir,
+                                               float bm, uint2 (&ent)[9])
+{
+    const GridGeom& g = A.tg;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        const int dy = c / 3 - 1, dz = c % 3 - 1;
+        const int64_t nk = (int64_t)rc.cellId + dz * rc.cXY + dy * rc.cX + dx;
+        const bool in = dx >= rc.xl && dx <= rc.xh && dy >= rc.yl && dy <= rc.yh && dz >= rc.zl && dz <= rc.zh &&
+                        nk >= 0 && (uint64_t)nk < g.cell_count;
+        ent[c] = in ? A.t_table[nk] : make_uint2(0u, 0u);
+    }
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        if (ent[c].y != 0u && ent[c].x < (uint32_t)A.ntri) {
+            const float4 lo = A.t_aabb[2 * ent[c].x], hi = A.t_aabb[2 * ent[c].x + 1];
+            if (ray_box(x1, inv, dir, lo, hi, bm)) mask |= 1u << c;
+        }
+    }
+    return mask;
+}
+
+__device__ __forceinline__ f3 inv_dir(f3 dir)
+{
+    return mk(dir.x != 0.0f ? 1.0f / dir.x : 0.0f, dir.y != 0.0f ? 1.0f / dir.y : 0.0f,
+              dir.z != 0.0f ? 1.0f / dir.z : 0.0f);
+}
+
+__device__ __forceinline__ float box_margin(const StepArgs& A)
+{
+    return A.pc.aabb_margin + (A.pc.aabb_static ? __uint_as_float(*A.vdisp) : 0.0f);
+}
+
+// Serial first-hit scan of one ray (used only when the work list is full).
+__device__ bool ray_first_hit_serial(const StepArgs& A, f3 x1, f3 dir, float& rt, f3& normal, f3& refl, int32_t& obj,
+                                     uint32_t& fl)
+{
+    if (!(dir.x != 0.0f || dir.y != 0.0f || dir.z != 0.0f)) return false;
+    const RayCells rc = ray_cells(A.tg, x1);
+    const f3 inv = inv_dir(dir);
+    const float bm = box_margin(A);
+    for (int dx = -1; dx <= 1; ++dx) {
+        uint2 ent[9];
+        const uint32_t mask = slab_cells(A, rc, dx, x1, inv, dir, bm, ent);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            if (!((mask >> c) & 1u)) continue;
+            const uint32_t u1 = min(ent[c].x + ent[c].y, (uint32_t)A.ntri);
+            for (uint32_t u = ent[c].x; u < u1; ++u) {
+                const float4 a = A.tdata[3 * (size_t)u], e1 = A.tdata[3 * (size_t)u + 1], e2 = A.tdata[3 * (size_t)u + 2];
+                if (ray_tri_pre(mk(a.x, a.y, a.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), x1, dir, rt, normal, refl,
+                                fl)) {
+                    obj = __float_as_int(a.w);
+                    return true;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
 {
     __shared__ float sx[kBlock], sy[kBlock], sz[kBlock];
     __shared__ float svx[kBlock], svy[kBlock], svz[kBlock];
     __shared__ float sfx[kBlock], sfy[kBlock], sfz[kBlock];
-    __shared__ int s_tele[kBlock];
 
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
     const int l = threadIdx.x;
-    // reset the vein-hit scan control for this frame (k_scan runs after us)
+    // reset the vein-hit scan control for this frame (its k_scan runs later)
     {
         const int64_t gid = (int64_t)blockIdx.x * blockDim.x + l;
         for (int64_t t = gid; t < A.vntiles; t += (int64_t)gridDim.x * blockDim.x) A.vstatus[t] = 0ull;
@@ -143,7 +288,6 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
     const int cl = act ? l / p : 0;
     const int64_t np = tt.np;
     const int32_t i = tt.pstart[T] + first * p + l;
-    if (l < ncell) s_tele[l] = 0;
     if (act) {
         sx[l] = A.pos[i]; sy[l] = A.pos[np + i]; sz[l] = A.pos[2 * np + i];
         svx[l] = A.vel[i]; svy[l] = A.vel[np + i]; svz[l] = A.vel[2 * np + i];
@@ -153,6 +297,10 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
 
     uint32_t fl = 0;
     f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
+    uint32_t nitems = 0;
+    f3 dir = x1, inv = x1;
+    RayCells rc{};
+    float bm = 0.0f;
     if (act) {
         x1 = mk(sx[l], sy[l], sz[l]);
         v1 = mk(svx[l], svy[l], svz[l]);
@@ -191,7 +339,9 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
         newF = add(newF, env);
         F = sdiv(add(F0, newF), 2.0f);
 
-        // ---- particle-particle contacts over the 27 neighbour cells
+        // ---- particle-particle contacts over the 27 neighbour cells.  The
+        // cell table is read one x-slab (9 cells) at a time with independent
+        // loads; cells are then visited in the reference order (x, y, z).
         {
             const GridGeom& g = A.pg;
             int Tl = T;
@@ -207,106 +357,196 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
             axis_range_i(yId, g.cnt[1], yl, yh);
             axis_range_i(zId, g.cnt[2], zl, zh);
             const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-            const int64_t ss = A.s_stride;
-            for (int dx = xl; dx <= xh; ++dx)
-                for (int dy = yl; dy <= yh; ++dy)
-                    for (int dz = zl; dz <= zh; ++dz) {
-                        const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
-                        if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
-                        const uint32_t s0 = A.p_start[nk];
-                        if (s0 >= (uint32_t)np || A.p_keys[s0] != (uint32_t)nk) continue;
-                        for (uint32_t u = s0; u < (uint32_t)np && A.p_keys[u] == (uint32_t)nk; ++u) {
-                            const int32_t j = A.p_perm[u];
-                            if (j == i) continue;
-                            const float r2 = A.bs[model_index(tt, Tl, j, fl)];
-                            const f3 p2 = mk(A.xs[u], A.xs[ss + u], A.xs[2 * ss + u]);
-                            const f3 rel = sub(x1, p2);
-                            const float d2 = dot(rel, rel);
-                            const float minD = r1 + r2;
-                            if (d2 <= minD * minD && d2 >= 0.0001f) {
-                                const f3 rv = sub(v1, mk(A.vs[u], A.vs[ss + u], A.vs[2 * ss + u]));
-                                F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
-                            }
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx < xl || dx > xh) continue;
+                uint2 ent[9];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    const int dy = c / 3 - 1, dz = c % 3 - 1;
+                    const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
+                    const bool in = dy >= yl && dy <= yh && dz >= zl && dz <= zh && nk >= 0 &&
+                                    (uint64_t)nk < g.cell_count;
+                    ent[c] = in ? A.p_table[nk] : make_uint2(0u, 0u);
+                }
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    const uint32_t u1 = min(ent[c].x + ent[c].y, (uint32_t)np);
+                    for (uint32_t u = ent[c].x; u < u1; ++u) {
+                        const float4 q = A.xs4[u];
+                        const int32_t j = __float_as_int(q.w);
+                        if (j == i) continue;
+                        const float r2 = A.bs[model_index(tt, Tl, j, fl)];
+                        const f3 rel = sub(x1, mk(q.x, q.y, q.z));
+                        const float d2 = dot(rel, rel);
+                        const float minD = r1 + r2;
+                        if (d2 <= minD * minD && d2 >= 0.0001f) {
+                            const float4 w = A.vs4[u];
+                            const f3 rv = sub(v1, mk(w.x, w.y, w.z));
+                            F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
                         }
-                    }
-        }
-
-        // ---- particle-vein ray test (first hit in x->y->z cell order)
-        {
-            const GridGeom& g = A.tg;
-            const f3 dir = normalize(v1, fl);
-            float rt = 1e10f;
-            f3 normal = mk(0.f, 0.f, 0.f), refl = mk(0.f, 0.f, 0.f);
-            int32_t obj = 0;
-            bool hit = false;
-            uint32_t dummy = 0;
-            const uint32_t cellId = cell_key(g, x1.x, x1.y, x1.z, dummy);
-            const uint32_t xId = sat_u32((x1.x - g.minv[0]) / g.cell[0]);
-            const uint32_t yId = sat_u32((x1.y - g.minv[1]) / g.cell[1]);
-            const uint32_t zId = sat_u32((x1.z - g.minv[2]) / g.cell[2]);
-            int xl, xh, yl, yh, zl, zh;
-            axis_range_u(xId, g.cnt[0], xl, xh);
-            axis_range_u(yId, g.cnt[1], yl, yh);
-            axis_range_u(zId, g.cnt[2], zl, zh);
-            const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-            const int64_t nv = A.nv;
-            for (int dx = xl; dx <= xh && !hit; ++dx)
-                for (int dy = yl; dy <= yh && !hit; ++dy)
-                    for (int dz = zl; dz <= zh && !hit; ++dz) {
-                        const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
-                        if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
-                        const uint32_t s0 = A.t_start[nk];
-                        if (s0 >= (uint32_t)A.ntri || A.t_keys[s0] != (uint32_t)nk) continue;
-                        for (uint32_t u = s0; u < (uint32_t)A.ntri && A.t_keys[u] == (uint32_t)nk; ++u) {
-                            const int32_t tri = A.t_perm[u];
-                            const uint32_t i0 = A.tidx[3 * tri], i1 = A.tidx[3 * tri + 1], i2 = A.tidx[3 * tri + 2];
-                            const f3 v0 = load3(A.X, nv, i0), va = load3(A.X, nv, i1), vb = load3(A.X, nv, i2);
-                            if (ray_tri(v0, va, vb, x1, dir, rt, normal, refl, fl)) {
-                                obj = tri;
-                                hit = true;
-                                break;
-                            }
-                        }
-                    }
-            const f3 relPos = sub(x1, add(x1, smul(rt, dir)));
-            const float d2 = dot(relPos, relPos);
-            if (hit && d2 <= pc.impact2) {
-                if (d2 > pc.minimal2) {
-                    if (pc.reaction) {
-                        f3 resp = sdiv(smul(-1.0f * dot(F, normal), normal), dot(normal, normal));
-                        F = add(F, resp);
-                    } else {
-                        const float rad = A.bs[model_index(tt, T, i, fl)];
-                        F = add(F, resilient(pc, relPos, v1, d2, rad, 0.5f, fl));
                     }
                 }
-                const float speed = len(v1);
-                const f3 dv = smul(1.0f, sub(smul(pc.vcd * speed, refl), v1));
-                const f3 ds = smul(pc.vcfi, v1);
-                v1 = add(v1, dv);
-                const uint32_t i0 = A.tidx[3 * obj], i1 = A.tidx[3 * obj + 1], i2 = A.tidx[3 * obj + 2];
-                const f3 v0 = load3(A.X, nv, i0), va = load3(A.X, nv, i1), vb = load3(A.X, nv, i2);
-                const f3 bc = bary(add(x1, smul(rt, dir)), v0, va, vb);
-                const f3 c0 = smul(bc.x, ds), c1 = smul(bc.y, ds), c2 = smul(bc.z, ds);
-                const uint32_t h = atomicAdd(&A.hctrl[0], 1u);
-                A.hits[3 * (size_t)h + 0] = make_float4(c0.x, c0.y, c0.z, __int_as_float(i));
-                A.hits[3 * (size_t)h + 1] = make_float4(c1.x, c1.y, c1.z, __int_as_float(obj));
-                A.hits[3 * (size_t)h + 2] = make_float4(c2.x, c2.y, c2.z, 0.0f);
-                atomicAdd(&A.vcount[i0], 1u);
-                atomicAdd(&A.vcount[i1], 1u);
-                atomicAdd(&A.vcount[i2], 1u);
+            }
+        }
+
+        // ---- vein ray candidates: count work items (cells the ray can reach,
+        // split into runs of <= 16 sorted triangles)
+        dir = normalize(v1, fl);
+        if (dir.x != 0.0f || dir.y != 0.0f || dir.z != 0.0f) {
+            rc = ray_cells(A.tg, x1);
+            inv = inv_dir(dir);
+            bm = box_margin(A);
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                uint2 ent[9];
+                const uint32_t mask = slab_cells(A, rc, dx, x1, inv, dir, bm, ent);
+#pragma unroll
+                for (int c = 0; c < 9; ++c)
+                    if ((mask >> c) & 1u) nitems += (ent[c].y + kItemTris - 1) / kItemTris;
+            }
+        }
+        A.frc[i] = F.x; A.frc[np + i] = F.y; A.frc[2 * np + i] = F.z;
+    }
+    // ---- reserve work-list space: one atomic per wave
+    const int lane = l & 63;
+    uint32_t incl = nitems;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    const uint32_t wave_total = __shfl(incl, 63, 64);
+    uint32_t wbase = 0;
+    if (lane == 63 && wave_total > 0) wbase = atomicAdd(&A.wctrl[0], wave_total);
+    wbase = __shfl(wbase, 63, 64);
+    if (act) {
+        const uint32_t off = wbase + incl - nitems;
+        const bool fits = (uint64_t)off + nitems <= (uint64_t)A.work_cap;
+        A.best[i] = (nitems > 0 && !fits) ? kScanSerial : kNoHit;
+        if (nitems > 0 && fits) {
+            uint32_t w = off;
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                uint2 ent[9];
+                const uint32_t mask = slab_cells(A, rc, dx, x1, inv, dir, bm, ent);
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    if (!((mask >> c) & 1u)) continue;
+                    const uint32_t rank = (uint32_t)((dx + 1) * 9 + c);
+                    for (uint32_t u = ent[c].x; u < ent[c].x + ent[c].y && w < off + nitems; u += kItemTris)
+                        A.work[w++] = make_uint4((uint32_t)i, u, rank, min(kItemTris, ent[c].x + ent[c].y - u));
+                }
             }
         }
     }
+    if (fl) atomicOr(A.flags, fl);
+}
 
+// One thread per work item: first hit among its <= 16 triangles, folded into
+// the particle's minimum (cell rank, sorted slot).
+__global__ void __launch_bounds__(256) k_ray(StepArgs A)
+{
+    const uint32_t n = min(A.wctrl[0], A.work_cap);
+    const int64_t np = A.tt.np;
+    uint32_t fl = 0;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+        const uint4 it = A.work[w];
+        const int32_t i = (int32_t)it.x;
+        const f3 x1 = load3(A.pos, np, i);
+        const f3 dir = normalize(load3(A.vel, np, i), fl);
+        const uint32_t u1 = min(it.y + it.w, (uint32_t)A.ntri);
+        for (uint32_t u = it.y; u < u1; ++u) {
+            const float4 a = A.tdata[3 * (size_t)u], e1 = A.tdata[3 * (size_t)u + 1], e2 = A.tdata[3 * (size_t)u + 2];
+            float rt;
+            f3 nrm, refl;
+            if (ray_tri_pre(mk(a.x, a.y, a.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), x1, dir, rt, nrm, refl,
+                            fl)) {
+                atomicMin(&A.best[i], (it.z << kSlotBits) | u);
+                break;
+            }
+        }
+    }
+}
+
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
+{
+    __shared__ int s_tele[kBlock];
+    const TypeTable& tt = A.tt;
+    const PhysConst& pc = A.pc;
+    const int l = threadIdx.x;
+    if (blockIdx.x == 0 && l == 0) A.wctrl[0] = 0u;  // work list consumed by k_ray
+    const int T = A.chunk_type[blockIdx.x];
+    const int first = A.chunk_first[blockIdx.x];
+    const int ncell = A.chunk_n[blockIdx.x];
+    const int p = tt.ppc[T];
+    const bool act = l < ncell * p;
+    const int a = act ? l % p : 0;
+    const int cl = act ? l / p : 0;
+    const int64_t np = tt.np;
+    const int32_t i = tt.pstart[T] + first * p + l;
+    if (l < ncell) s_tele[l] = 0;
+    uint32_t fl = 0;
+    f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
+    if (act) {
+        x1 = load3(A.pos, np, i);
+        v1 = load3(A.vel, np, i);
+        F = load3(A.frc, np, i);
+        const uint32_t best = A.best[i];
+        const f3 dir = normalize(v1, fl);
+        float rt = 1e10f;
+        f3 normal = mk(0.f, 0.f, 0.f), refl = mk(0.f, 0.f, 0.f);
+        int32_t obj = 0;
+        bool hit = false;
+        if (best == kScanSerial) {
+            hit = ray_first_hit_serial(A, x1, dir, rt, normal, refl, obj, fl);
+        } else if (best != kNoHit) {
+            const uint32_t u = best & ((1u << kSlotBits) - 1u);
+            const float4 ta = A.tdata[3 * (size_t)u], e1 = A.tdata[3 * (size_t)u + 1], e2 = A.tdata[3 * (size_t)u + 2];
+            hit = ray_tri_pre(mk(ta.x, ta.y, ta.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), x1, dir, rt, normal, refl,
+                              fl);
+            obj = __float_as_int(ta.w);
+        }
+        const f3 relPos = sub(x1, add(x1, smul(rt, dir)));
+        const float d2 = dot(relPos, relPos);
+        if (hit && d2 <= pc.impact2) {
+            if (d2 > pc.minimal2) {
+                if (pc.reaction) {
+                    f3 resp = sdiv(smul(-1.0f * dot(F, normal), normal), dot(normal, normal));
+                    F = add(F, resp);
+                } else {
+                    const float rad = A.bs[model_index(tt, T, i, fl)];
+                    F = add(F, resilient(pc, relPos, v1, d2, rad, 0.5f, fl));
+                }
+            }
+            const int64_t nv = A.nv;
+            const float speed = len(v1);
+            const f3 dv = smul(1.0f, sub(smul(pc.vcd * speed, refl), v1));
+            const f3 ds = smul(pc.vcfi, v1);
+            v1 = add(v1, dv);
+            const uint32_t i0 = A.tidx[3 * obj], i1 = A.tidx[3 * obj + 1], i2 = A.tidx[3 * obj + 2];
+            const f3 v0 = load3(A.X, nv, i0), va = load3(A.X, nv, i1), vb = load3(A.X, nv, i2);
+            const f3 bc = bary(add(x1, smul(rt, dir)), v0, va, vb);
+            const f3 c0 = smul(bc.x, ds), c1 = smul(bc.y, ds), c2 = smul(bc.z, ds);
+            const uint32_t h = atomicAdd(&A.hctrl[0], 1u);
+            A.hits[3 * (size_t)h + 0] = make_float4(c0.x, c0.y, c0.z, __int_as_float(i));
+            A.hits[3 * (size_t)h + 1] = make_float4(c1.x, c1.y, c1.z, __int_as_float(obj));
+            A.hits[3 * (size_t)h + 2] = make_float4(c2.x, c2.y, c2.z, 0.0f);
+            atomicAdd(&A.vcount[i0], 1u);
+            atomicAdd(&A.vcount[i1], 1u);
+            atomicAdd(&A.vcount[i2], 1u);
+        }
+    }
     if (FUSED) {
         // ---- integrate (Heun / trapezoid): v0 = v/gpuCount; v = v0 + dt*F; x += 0.5*dt*(v + v0)
         if (act) {
             const f3 v0 = sdiv(v1, 1.0f);
             v1 = add(v0, smul(pc.dt, F));
             x1 = add(x1, smul(0.5f * pc.dt, add(v1, v0)));
-            if (pc.use_blood_flow && vein_end_flag(pc, tt.end_spheres[T] != 0, x1)) s_tele[cl] = 1;
         }
+        __syncthreads();  // s_tele initialised
+        if (act && pc.use_blood_flow && vein_end_flag(pc, tt.end_spheres[T] != 0, x1)) s_tele[cl] = 1;
         __syncthreads();
         if (act && pc.use_blood_flow && s_tele[cl]) {
             const int32_t cell = tt.cstart[T] + first + cl;
@@ -457,8 +697,14 @@ __global__ void __launch_bounds__(256) k_vein(VeinArgs A)
     if (MODE & 2) {
         const f3 V = add(load3(A.V, nv, id), smul(A.dt, F));
         store3(A.V, nv, id, V);
-        store3(A.X, nv, id, add(load3(A.X, nv, id), smul(A.dt, V)));
+        const f3 Xn = add(load3(A.X, nv, id), smul(A.dt, V));
+        store3(A.X, nv, id, Xn);
         store3(A.Fv, nv, id, mk(0.f, 0.f, 0.f));
+        const f3 X0 = load3(A.X0, nv, id);
+        float dmax = fmaxf(fabsf(Xn.x - X0.x), fmaxf(fabsf(Xn.y - X0.y), fabsf(Xn.z - X0.z)));
+        if (!(dmax <= 3.0e38f)) dmax = 3.0e38f;  // NaN / inf: disable culling
+        const uint32_t bits = __float_as_uint(dmax);
+        if (bits > __hip_atomic_load(A.vdisp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(A.vdisp, bits);
     } else {
         store3(A.Fv, nv, id, F);
     }
@@ -475,6 +721,57 @@ __global__ void __launch_bounds__(256) k_tri_centers(const float* __restrict__ X
     c[t] = (a.x + b.x + d.x) / 3.0f;
     c[ntri + t] = (a.y + b.y + d.y) / 3.0f;
     c[2 * (int64_t)ntri + t] = (a.z + b.z + d.z) / 3.0f;
+}
+
+// Per sorted triangle slot: (v0 | triangle id), v1 - v0, v2 - v0 from the
+// current vein positions.
+__global__ void __launch_bounds__(256) k_tri_prep(const int32_t* __restrict__ perm, int32_t ntri,
+                                                  const uint32_t* __restrict__ idx, const float* __restrict__ X,
+                                                  int32_t nv, float4* __restrict__ tdata)
+{
+    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= ntri) return;
+    const int32_t tri = perm[u];
+    const f3 v0 = load3(X, nv, idx[3 * tri]), v1 = load3(X, nv, idx[3 * tri + 1]), v2 = load3(X, nv, idx[3 * tri + 2]);
+    const f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+    tdata[3 * (size_t)u] = make_float4(v0.x, v0.y, v0.z, __int_as_float(tri));
+    tdata[3 * (size_t)u + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+    tdata[3 * (size_t)u + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+}
+
+// Box of every occupied triangle cell (stored at its run start).
+__global__ void __launch_bounds__(256) k_tri_aabb(const uint32_t* __restrict__ keys, const int32_t* __restrict__ perm,
+                                                  int32_t ntri, const uint32_t* __restrict__ idx,
+                                                  const float* __restrict__ X, int32_t nv, float4* __restrict__ aabb)
+{
+    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= ntri) return;
+    const uint32_t k = keys[u];
+    if (u > 0 && keys[u - 1] == k) return;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int32_t t = u; t < ntri && keys[t] == k; ++t) {
+        const int32_t tri = perm[t];
+        for (int c = 0; c < 3; ++c) {
+            const f3 v = load3(X, nv, idx[3 * tri + c]);
+            lo[0] = fminf(lo[0], v.x); lo[1] = fminf(lo[1], v.y); lo[2] = fminf(lo[2], v.z);
+            hi[0] = fmaxf(hi[0], v.x); hi[1] = fmaxf(hi[1], v.y); hi[2] = fmaxf(hi[2], v.z);
+        }
+    }
+    aabb[2 * u] = make_float4(lo[0], lo[1], lo[2], 0.f);
+    aabb[2 * u + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+}
+
+// Max per-axis displacement |X - X0| over all vertices (after an external
+// change of the vein positions).
+__global__ void __launch_bounds__(256) k_vdisp(const float* __restrict__ X, const float* __restrict__ X0, int32_t nv,
+                                               uint32_t* __restrict__ vdisp)
+{
+    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nv) return;
+    const f3 a = load3(X, nv, id), b = load3(X0, nv, id);
+    float dmax = fmaxf(fabsf(a.x - b.x), fmaxf(fabsf(a.y - b.y), fabsf(a.z - b.z)));
+    if (!(dmax <= 3.0e38f)) dmax = 3.0e38f;
+    atomicMax(vdisp, __float_as_uint(dmax));
 }
 
 // Contact listing in sorted-slot order (calculateParticleCollisions visit
@@ -508,9 +805,8 @@ __global__ void __launch_bounds__(256) k_contacts(StepArgs A, int pass, int32_t*
             for (int dz = zl; dz <= zh; ++dz) {
                 const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
                 if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
-                const uint32_t s0 = A.p_start[nk];
-                if (s0 >= (uint32_t)np || A.p_keys[s0] != (uint32_t)nk) continue;
-                for (uint32_t u = s0; u < (uint32_t)np && A.p_keys[u] == (uint32_t)nk; ++u) {
+                const uint2 e = A.p_table[nk];
+                for (uint32_t u = e.x; u < min(e.x + e.y, (uint32_t)np); ++u) {
                     const int32_t j = A.p_perm[u];
                     if (j == i) continue;
                     const float r2 = A.bs[model_index(tt, Tl, j, fl)];
@@ -526,12 +822,12 @@ __global__ void __launch_bounds__(256) k_contacts(StepArgs A, int pass, int32_t*
     if (pass == 0) counts[i] = c;
 }
 
-// Gather of table entries (parity views of the cell-start table).
-__global__ void __launch_bounds__(256) k_gather_u32(const uint32_t* __restrict__ table, const uint32_t* __restrict__ idx,
-                                                    int32_t n, uint32_t* __restrict__ out)
+// Gather of cell-table starts (parity views of the grid).
+__global__ void __launch_bounds__(256) k_gather_start(const uint2* __restrict__ table, const uint32_t* __restrict__ idx,
+                                                      int32_t n, uint32_t* __restrict__ out)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = table[idx[i]];
+    if (i < n) out[i] = table[idx[i]].x;
 }
 
 // Plain copy (HBM counter calibration).
@@ -543,11 +839,19 @@ __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, fl
 
 // ------------------------------------------------------------ launchers --
 
-void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
+void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st)
+{
+    if (nchunks > 0) hipLaunchKernelGGL(k_cells_a, dim3(nchunks), dim3(kBlock), 0, st, A);
+}
+void launch_ray(const StepArgs& A, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ray, dim3(1024), dim3(256), 0, st, A);
+}
+void launch_cells_b(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
 {
     if (nchunks <= 0) return;
-    if (fused) hipLaunchKernelGGL(k_cells<true>, dim3(nchunks), dim3(kBlock), 0, st, A);
-    else hipLaunchKernelGGL(k_cells<false>, dim3(nchunks), dim3(kBlock), 0, st, A);
+    if (fused) hipLaunchKernelGGL(k_cells_b<true>, dim3(nchunks), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_cells_b<false>, dim3(nchunks), dim3(kBlock), 0, st, A);
 }
 void launch_integrate(const StepArgs& A, hipStream_t st)
 {
@@ -586,9 +890,25 @@ void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t
     if (A.tt.np > 0)
         hipLaunchKernelGGL(k_contacts, dim3((A.tt.np + 255) / 256), dim3(256), 0, st, A, pass, counts, offs, pairs);
 }
-void launch_gather_u32(const uint32_t* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st)
+void launch_tri_prep(const int32_t* perm, int32_t ntri, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
+                     hipStream_t st)
 {
-    if (n > 0) hipLaunchKernelGGL(k_gather_u32, dim3((n + 255) / 256), dim3(256), 0, st, table, idx, n, out);
+    if (ntri > 0)
+        hipLaunchKernelGGL(k_tri_prep, dim3((ntri + 255) / 256), dim3(256), 0, st, perm, ntri, idx, X, nv, tdata);
+}
+void launch_tri_aabb(const uint32_t* keys, const int32_t* perm, int32_t ntri, const uint32_t* idx, const float* X,
+                     int32_t nv, float4* aabb, hipStream_t st)
+{
+    if (ntri > 0)
+        hipLaunchKernelGGL(k_tri_aabb, dim3((ntri + 255) / 256), dim3(256), 0, st, keys, perm, ntri, idx, X, nv, aabb);
+}
+void launch_vdisp(const float* X, const float* X0, int32_t nv, uint32_t* vdisp, hipStream_t st)
+{
+    if (nv > 0) hipLaunchKernelGGL(k_vdisp, dim3((nv + 255) / 256), dim3(256), 0, st, X, X0, nv, vdisp);
+}
+void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_gather_start, dim3((n + 255) / 256), dim3(256), 0, st, table, idx, n, out);
 }
 void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st)
 {

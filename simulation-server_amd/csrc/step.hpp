@@ -25,6 +25,8 @@ struct PhysConst {
     float cyl_r, spawn_y;
     float thr_upper, thr_lower, thr_right, thr_left, thr_front, thr_back;
     int32_t use_blood_flow, reaction, brake_on, nend;
+    float aabb_margin;      // absolute inflation of the triangle-cell boxes (rounding slack)
+    int32_t aabb_static;    // boxes from creation-time positions (+ tracked displacement)
     uint64_t seed;
     float endc[3 * kMaxEndings];
     float endr[kMaxEndings];
@@ -54,15 +56,13 @@ struct StepArgs {
     const uint32_t* p_keys;
     const int32_t* p_perm;
     const int32_t* p_slot;
-    const uint32_t* p_start;
-    const float* xs;
-    const float* vs;
-    int64_t s_stride;
+    const uint2* p_table;   // {start, count} per cell
+    const float4* xs4;      // sorted (x, y, z, index bits)
+    const float4* vs4;      // sorted (vx, vy, vz, 0)
     // triangle grid + vein
     GridGeom tg;
-    const uint32_t* t_keys;
-    const int32_t* t_perm;
-    const uint32_t* t_start;
+    const uint2* t_table;
+    const float4* tdata;    // per sorted triangle: (v0 | id), e1, e2
     int32_t ntri;
     int32_t nv;
     const float* X;  // SoA stride nv
@@ -73,6 +73,12 @@ struct StepArgs {
     uint32_t* hctrl;  // [0] hit count, [1] vein-scan ticket
     uint64_t* vstatus;
     int32_t vntiles;
+    const float4* t_aabb;     // per triangle run start: [2u] = lo, [2u+1] = hi
+    uint32_t* best;           // per particle: min (cell rank << 27 | slot) of its ray hits
+    uint4* work;              // ray work items: (particle, first slot, cell rank, count)
+    uint32_t* wctrl;          // [0] work items issued
+    uint32_t work_cap;
+    const uint32_t* vdisp;    // max |X - X0| per axis since the boxes were made (float bits)
     uint32_t* flags;
     uint64_t* step;
 };
@@ -92,6 +98,8 @@ struct VeinArgs {
     float4* seg;        // ordered contributions: (c.xyz, key)
     uint32_t* hctrl;
     const uint32_t* tidx;
+    const float* X0;          // positions the triangle-cell boxes were made from
+    uint32_t* vdisp;          // running max displacement (float bits, atomicMax)
     uint32_t* flags;
     uint64_t* step;
 };

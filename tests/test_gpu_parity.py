@@ -150,3 +150,36 @@ def test_c1_full_one_frame(bcs, orc):
     for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
         assert_bits_equal(x, y, "c1 " + n)
     assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "c1 vein")
+
+
+def test_displaced_vein_keeps_parity(bcs, orc):
+    """Vein moved far from its creation position: the triangle-cell boxes
+    (made at creation) are inflated by the tracked displacement, so the
+    culled ray scan still finds exactly the oracle's hits."""
+    sc = _scene(bcs, "c0")
+    sim, o = _pair(bcs, orc, sc)
+    X, V, F = sim.get_vein()
+    X = X.reshape(3, -1).copy()
+    X[0] += np.float32(3.0)
+    X[2] -= np.float32(1.5)
+    X = X.reshape(-1)
+    sim.set_vein(X, V, F)
+    o.set_vein(X, V, F)
+    sim.step(20)
+    o.step(20)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "displaced vein " + n)
+    assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "displaced vein X")
+
+
+def test_dynamic_triangle_grid_parity(bcs, orc):
+    sc = _scene(bcs, "c0")
+    ph = sc.physics
+    ph.dynamic_triangle_grid = 1
+    sim = bcs.Simulation.from_scene(sc, physics=ph)
+    o = orc.Oracle(ph, sc.types, sc.mesh)
+    o.set_particles(sc.positions, sc.velocities, sc.forces)
+    sim.step(20)
+    o.step(20)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "dynamic triangle grid " + n)
