@@ -60,19 +60,20 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt):
                    neighbour 16, first-hit key written 4           68 / particle + 12 / blood cell
       k_cells_b    x, v, F and first-hit key read 40, x, v, F written 36
                                                                    76 / particle
-      k_vein_springs+k_tri_prep  X, V read 24, 9 neighbour ids + lengths 72,
-                   Fv read + written 24 per vertex; perm + indices 16 and the
-                   float4 triangle data written 48 per triangle    120 / vertex + 64 / triangle
-      k_vein       X, V, Fv, X0 read 48, X, V, Fv written 36, offsets 4
-                                                                   88 / vertex"""
+      k_vein_springs+k_tri_box  X, V read 24, 9 neighbour ids + lengths 72,
+                   Fv read + written 24 per vertex; perm + indices 16, the
+                   float4 triangle data written 48 and one box (32) per
+                   occupied triangle cell, counted per triangle 12 120 / vertex + 76 / triangle
+      k_vein       X, V, Fv read 36, X, V, Fv written 36, offsets 4
+                                                                   76 / vertex"""
     table = {
         "k_key": 28 * np_,
         "k_scatter": 12 * np_,
         "k_rank": 76 * np_ + 8 * n_occ,
         "k_cells_a": 68 * np_ + 12 * nc,
         "k_cells_b": 76 * np_,
-        "k_vein_springs+k_tri_prep": 120 * nv + 64 * nt,
-        "k_vein": 88 * nv,
+        "k_vein_springs+k_tri_box": 120 * nv + 76 * nt,
+        "k_vein": 76 * nv,
     }
     return table.get(name)
 
@@ -149,6 +150,7 @@ def main():
     value = np_ * args.steps / wall
 
     flags = sim.error_flags()
+    frame_stats = sim.frame_stats()
     _, perm, occ_keys, _, _ = sim.get_grid(0)
     n_occ = len(occ_keys)
     # per-kernel durations (HIP events on the library stream, un-graphed frames)
@@ -203,6 +205,7 @@ def main():
         "kernel_ms": kern,
         "dominant_kernel": dominant,
         "error_flags": flags,
+        "frame_stats": frame_stats,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)

@@ -44,8 +44,7 @@ extern "C" {
 #define BCS_FLAG_NAN_NORMALIZE      0x2u  /* normalize() produced NaN and returned 0 */
 #define BCS_FLAG_SCAN_TIMEOUT       0x4u  /* a bounded device wait gave up */
 #define BCS_FLAG_RADIUS_CLAMPED     0x8u  /* mixed-type radius index left the model table */
-#define BCS_FLAG_VEIN_SLOT_OVERFLOW 0x10u /* per-vertex hit slots spilled to the overflow list */
-#define BCS_FLAG_OVERFLOW_LOST      0x20u /* overflow list itself was full: a contribution was dropped */
+#define BCS_FLAG_RAY_SERIAL         0x10u /* ray work list full: some rays were scanned serially (same result) */
 
 /* ---- grid selector ----------------------------------------------------- */
 #define BCS_GRID_PARTICLES 0
@@ -269,6 +268,10 @@ int bcs_sim_get_grid(bcs_sim* sim, int32_t which, uint32_t* keys_sorted, int32_t
  * pairs_j may be NULL to only count. */
 int bcs_sim_get_contacts(bcs_sim* sim, int32_t* counts, int32_t* pairs_j, int64_t capacity,
                          int64_t* n_total);
+
+/* Counters of the last completed frame: ray work items (triangle runs of
+ * <= 16 whose cell box the ray can reach) and particle-vein hits. */
+int bcs_sim_frame_stats(bcs_sim* sim, uint32_t* ray_work_items, uint32_t* vein_hits);
 
 /* Device error word (BCS_FLAG_*); clear != 0 resets it. */
 int bcs_sim_error_flags(bcs_sim* sim, int32_t clear, uint32_t* flags);

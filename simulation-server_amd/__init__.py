@@ -212,6 +212,7 @@ def _sig(lib):
     lib.bcs_sim_get_grid.argtypes = [vp, C.c_int32, u32p, i32p, i32p, u32p, i32p, i32p]
     lib.bcs_sim_get_contacts.argtypes = [vp, i32p, i32p, C.c_int64, P(C.c_int64)]
     lib.bcs_sim_error_flags.argtypes = [vp, C.c_int32, u32p]
+    lib.bcs_sim_frame_stats.argtypes = [vp, u32p, u32p]
     lib.bcs_device_count.argtypes = [i32p]
     lib.bcs_calibration_copy.argtypes = [C.c_int64, C.c_int32, f32p]
     lib.bcs_scene_preset.argtypes = [C.c_int32, C.c_double, C.c_uint64, P(vp)]
@@ -494,6 +495,11 @@ class Simulation:
         check(self.lib.bcs_sim_get_contacts(self.h, iptr(counts), iptr(pairs), total.value, C.byref(total)),
               "get_contacts")
         return counts[: self.np], pairs[: total.value]
+
+    def frame_stats(self):
+        w, h = C.c_uint32(0), C.c_uint32(0)
+        check(self.lib.bcs_sim_frame_stats(self.h, C.byref(w), C.byref(h)), "frame_stats")
+        return {"ray_work_items": w.value, "vein_hits": h.value}
 
     def error_flags(self, clear=False) -> int:
         f = C.c_uint32(0)

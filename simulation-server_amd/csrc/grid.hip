@@ -43,6 +43,7 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
     if (gid == 0) {
         ctrl[0] = 0u;  // scan ticket
         ctrl[1] = 0u;  // big-bucket count
+        ctrl[2] = 0u;  // occupied cells (run list)
     }
     if (gid < n_prev) {  // clear the previous build's cell of this object
         const uint32_t old = keys[gid];
@@ -173,6 +174,8 @@ struct RankOut {
     int64_t src_stride;
     float4* xs4;           // (x, y, z, index bits) in sorted order
     float4* vs4;           // (vx, vy, vz, 0) in sorted order
+    uint32_t* runs;        // run starts of occupied cells, in no particular order
+    uint32_t* nruns;
 };
 
 __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32_t idx, uint32_t pos, bool first,
@@ -181,7 +184,10 @@ __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32
     o.keys_sorted[pos] = key;
     o.perm[pos] = idx;
     if (o.slot_of) o.slot_of[idx] = (int32_t)pos;
-    if (first) o.table[key] = make_uint2(pos, count);
+    if (first) {
+        o.table[key] = make_uint2(pos, count);
+        if (o.runs) o.runs[atomicAdd(o.nruns, 1u)] = pos;
+    }
     if (o.xs4) {
         const int64_t s = o.src_stride;
         o.xs4[pos] = make_float4(o.src_pos[idx], o.src_pos[s + idx], o.src_pos[2 * s + idx], __int_as_float(idx));
@@ -299,6 +305,7 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     GRID_HIP(hipMalloc(&G.items_k, n * 4));
     GRID_HIP(hipMalloc(&G.items_i, n * 4));
     GRID_HIP(hipMalloc(&G.keys_sorted, n * 4));
+    GRID_HIP(hipMalloc(&G.runs, n * 4));
     GRID_HIP(hipMalloc(&G.perm, n * 4));
     if (with_slot) GRID_HIP(hipMalloc(&G.slot_of, n * 4));
     if (with_copies) {
@@ -323,7 +330,7 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
 
 void grid_free(DeviceGrid& G)
 {
-    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.perm, G.slot_of, G.xs4, G.vs4,
+    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.runs, G.perm, G.slot_of, G.xs4, G.vs4,
                     G.table, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -355,7 +362,7 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
                        G.items_i, flags);
     te(2);
     RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, vel ? pos : nullptr, vel, stride, vel ? G.xs4 : nullptr,
-              vel ? G.vs4 : nullptr};
+              vel ? G.vs4 : nullptr, vel ? nullptr : G.runs, G.ctrl + 2};
     tb(3);
     hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, o);
     te(3);

@@ -38,7 +38,8 @@ struct DeviceGrid {
     uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds
     uint32_t* bstart = nullptr;       // bucket offsets [nb+2]
     uint64_t* status = nullptr;       // scan tile status
-    uint32_t* ctrl = nullptr;         // [0] scan ticket, [1] big-bucket count
+    uint32_t* ctrl = nullptr;         // [0] scan ticket, [1] big-bucket count, [2] run count
+    uint32_t* runs = nullptr;         // run start of every occupied cell (unordered)
     uint32_t* biglist = nullptr;
     float4* xs4 = nullptr;            // (x, y, z, index bits) in sorted order (optional)
     float4* vs4 = nullptr;            // (vx, vy, vz, 0) in sorted order (optional)

@@ -29,11 +29,8 @@ void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t
 void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st);
 void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
-void launch_tri_prep(const int32_t* perm, int32_t ntri, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
-                     hipStream_t st);
-void launch_tri_aabb(const uint32_t* keys, const int32_t* perm, int32_t ntri, const uint32_t* idx, const float* X,
-                     int32_t nv, float4* aabb, hipStream_t st);
-void launch_vdisp(const float* X, const float* X0, int32_t nv, uint32_t* vdisp, hipStream_t st);
+void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
+                    float4* aabb, hipStream_t st);
 }  // namespace bcs
 
 using namespace bcs;
@@ -66,14 +63,12 @@ struct bcs_sim {
     float4 *hits = nullptr, *seg = nullptr;
     uint64_t* vstatus = nullptr;
     int32_t vntiles = 0;
-    float4* t_aabb = nullptr;  // triangle-cell boxes at run starts
-    float4* tdata = nullptr;   // per sorted triangle: (v0 | id), e1, e2
+    float4* t_aabb = nullptr;  // triangle-cell boxes at run starts (per frame)
+    float4* tdata = nullptr;   // per sorted triangle: (v0 | id), e1, e2 (per frame)
     uint32_t* best = nullptr;  // per particle first-hit key
     uint4* work = nullptr;     // ray work items
     uint32_t* wctrl = nullptr;
     uint32_t work_cap = 0;
-    float* X0 = nullptr;       // vein positions the boxes were made from
-    uint32_t* vdisp = nullptr; // max |X - X0| (float bits)
     // grids
     DeviceGrid pg, tg;
     bool tg_built = false;
@@ -117,7 +112,7 @@ struct bcs_sim {
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.vcount = vcount; A.hits = hits; A.hctrl = hctrl; A.vstatus = vstatus; A.vntiles = vntiles;
-        A.t_aabb = t_aabb; A.vdisp = vdisp;
+        A.t_aabb = t_aabb;
         A.best = best; A.work = work; A.wctrl = wctrl; A.work_cap = work_cap;
         A.flags = flags; A.step = step;
         return A;
@@ -128,7 +123,6 @@ struct bcs_sim {
         Va.nv = nv; Va.X = X; Va.V = V; Va.Fv = Fv; Va.nbr = nbr; Va.nbl = nbl;
         Va.dt = phys.dt; Va.vk = phys.vein_k_sniff; Va.vd = phys.vein_d_fact;
         Va.vcount = vcount; Va.vstart = vstart; Va.hits = hits; Va.seg = seg; Va.hctrl = hctrl; Va.tidx = tidx;
-        Va.X0 = X0; Va.vdisp = vdisp;
         Va.flags = flags; Va.step = step;
         return Va;
     }
@@ -164,17 +158,14 @@ int set_dev(bcs_sim* s)
         if (rc_) return rc_;  \
     } while (0)
 
-// Triangle grid on the triangle centres.  Frozen mode (the reference's
-// single-GPU behaviour): centres and cell boxes come from the creation-time
-// vein, and the boxes are inflated at use by the tracked displacement.
-// Dynamic mode: centres and boxes are recomputed from the current vein.
+// Triangle grid on the triangle centres: frozen after creation (the
+// reference's single-GPU behaviour) unless dynamic_triangle_grid is set.  The
+// per-triangle data and cell boxes are rebuilt every frame (k_tri_box).
 int build_triangle_grid(bcs_sim* s)
 {
     const bool dyn = s->phys.dynamic_triangle_grid != 0;
     if (dyn || !s->tg_built) launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
     RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, nullptr, s->flags, s->stream));
-    if (dyn || !s->tg_built)
-        launch_tri_aabb(s->tg.keys_sorted, s->tg.perm, s->ntri, s->tidx, s->X, s->nv, s->t_aabb, s->stream);
     s->tg_built = true;
     return BCS_OK;
 }
@@ -204,7 +195,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
     const VeinArgs Va = s->vein_args();
     tb(kGridKernels + 0);
     launch_vein_springs(Va, st);
-    launch_tri_prep(s->tg.perm, s->ntri, s->tidx, s->X, s->nv, s->tdata, st);
+    launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, st);
     te(kGridKernels + 0);
     tb(kGridKernels + 1);
     launch_cells_a(A, s->nchunks, st);
@@ -229,7 +220,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
 }
 
 const char* kKernelNames =
-    "k_key;k_scan;k_scatter;k_rank;k_big;k_vein_springs+k_tri_prep;k_cells_a;k_ray;k_cells_b;k_scan(vein);"
+    "k_key;k_scan;k_scatter;k_rank;k_big;k_vein_springs+k_tri_box;k_cells_a;k_ray;k_cells_b;k_scan(vein);"
     "k_vein_scatter;k_vein";
 constexpr int kFusedKernels = kGridKernels + 7;
 
@@ -352,7 +343,6 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     pc.brake_on = phys->enable_big_cells_brake;
     pc.seed = phys->seed;
     pc.nend = mesh->ending_count;
-    pc.aabb_static = phys->dynamic_triangle_grid ? 0 : 1;
     {
         float mabs = 0.f;
         for (int d = 0; d < 3; ++d) mabs = std::max(mabs, std::max(std::fabs(bmin[d]), std::fabs(bmax[d])));
@@ -397,7 +387,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(s->dalloc(&s->flags, 4)); RC(s->dalloc(&s->step, 1));
     RC(s->dalloc(&s->t_aabb, 2 * NT)); RC(s->dalloc(&s->tdata, 3 * NT));
     s->work_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * N, 65536), 0x7FFFFFFFull);
-    RC(s->dalloc(&s->best, N)); RC(s->dalloc(&s->work, s->work_cap)); RC(s->dalloc(&s->wctrl, 4)); RC(s->dalloc(&s->X0, 3 * NV)); RC(s->dalloc(&s->vdisp, 1));
+    RC(s->dalloc(&s->best, N)); RC(s->dalloc(&s->work, s->work_cap)); RC(s->dalloc(&s->wctrl, 4));
     // ---- uploads
     HIPCHK(hipMemsetAsync(s->pos, 0, 12 * std::max<size_t>(N, 1), s->stream));
     HIPCHK(hipMemsetAsync(s->vel, 0, 12 * std::max<size_t>(N, 1), s->stream));
@@ -421,7 +411,6 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
         for (size_t v = 0; v < NV; ++v)
             for (int d = 0; d < 3; ++d) soa[d * NV + v] = mesh->positions[3 * v + d];
         HIPCHK(hipMemcpyAsync(s->X, soa.data(), 12 * NV, hipMemcpyHostToDevice, s->stream));
-        HIPCHK(hipMemcpyAsync(s->X0, soa.data(), 12 * NV, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemsetAsync(s->V, 0, 12 * NV, s->stream));
         HIPCHK(hipMemsetAsync(s->Fv, 0, 12 * NV, s->stream));
         std::vector<int32_t> ids(kVeinSlots * NV);
@@ -437,7 +426,6 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     HIPCHK(hipMemsetAsync(s->flags, 0, 4, s->stream));
     HIPCHK(hipMemsetAsync(s->step, 0, 8, s->stream));
     HIPCHK(hipMemsetAsync(s->wctrl, 0, 16, s->stream));
-    HIPCHK(hipMemsetAsync(s->vdisp, 0, 4, s->stream));
     // ---- grids
     const GridGeom pgeom = make_geom(bmin, bmax, phys->particle_cell);
     const GridGeom tgeom = make_geom(bmin, bmax, phys->triangle_cell);
@@ -538,10 +526,6 @@ extern "C" int bcs_sim_set_vein(bcs_sim* s, const float* p, const float* v, cons
     RC(set_dev(s));
     const size_t n = 3 * (size_t)s->nv;
     RC(copy_in(s, s->X, p, n)); RC(copy_in(s, s->V, v, n)); RC(copy_in(s, s->Fv, f, n));
-    if (p) {  // keep the triangle-cell boxes conservative for the new positions
-        HIPCHK(hipMemsetAsync(s->vdisp, 0, 4, s->stream));
-        launch_vdisp(s->X, s->X0, s->nv, s->vdisp, s->stream);
-    }
     HIPCHK(hipStreamSynchronize(s->stream));
     return BCS_OK;
 }
@@ -585,7 +569,7 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
     const StepArgs A = s->step_args();
     const VeinArgs Va = s->vein_args();
     launch_vein_springs(Va, s->stream);
-    launch_tri_prep(s->tg.perm, s->ntri, s->tidx, s->X, s->nv, s->tdata, s->stream);
+    launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, s->stream);
     launch_cells_a(A, s->nchunks, s->stream);
     launch_ray(A, s->stream);
     launch_cells_b(A, s->nchunks, false, s->stream);
@@ -782,6 +766,18 @@ extern "C" int bcs_sim_get_contacts(bcs_sim* s, int32_t* counts, int32_t* pairs_
     if (doffs) (void)hipFree(doffs);
     if (dpairs) (void)hipFree(dpairs);
     return rc;
+}
+
+extern "C" int bcs_sim_frame_stats(bcs_sim* s, uint32_t* ray_work_items, uint32_t* vein_hits)
+{
+    RC(set_dev(s));
+    uint32_t w[2] = {0, 0}, h[8] = {0};
+    HIPCHK(hipMemcpyAsync(w, s->wctrl, 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(h, s->hctrl, 32, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (ray_work_items) *ray_work_items = w[1];
+    if (vein_hits) *vein_hits = h[4];
+    return BCS_OK;
 }
 
 extern "C" int bcs_sim_error_flags(bcs_sim* s, int32_t clear, uint32_t* flags)

@@ -53,35 +53,13 @@ __device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVe
     return smul(coef, add(add(spring, damp), shear));
 }
 
-// realCollisionDetection, src/simulation/vein_collisions.cu:11-45.
-__device__ __forceinline__ bool ray_tri(f3 v0, f3 v1, f3 v2, f3 orig, f3 dir, float& t_out, f3& normal, f3& refl,
-                                        uint32_t& fl)
-{
-    const float EPS = 0.000001f;
-    f3 edge1 = sub(v1, v0);
-    f3 edge2 = sub(v2, v0);
-    f3 h = cross(dir, edge2);
-    float a = dot(edge1, h);
-    if (a > -EPS && a < EPS) return false;
-    float f = 1.0f / a;
-    f3 s = sub(orig, v0);
-    float u = f * dot(s, h);
-    if (u < 0.0f || u > 1.0f) return false;
-    f3 q = cross(s, edge1);
-    float v = f * dot(dir, q);
-    if (v < 0.0f || u + v > 1.0f) return false;
-    float t = f * dot(edge2, q);
-    if (t > EPS) {
-        t_out = t;
-        normal = normalize(cross(edge2, edge1), fl);
-        refl = sub(dir, smul(2.0f * dot(dir, normal), normal));
-        return true;
-    }
-    return false;
-}
-
-// The same test with edge1 = v1 - v0 and edge2 = v2 - v0 precomputed (the
-// identical float subtractions, done once per frame by k_tri_prep).
+// realCollisionDetection, src/simulation/vein_collisions.cu:11-45, with
+// edge1 = v1 - v0 and edge2 = v2 - v0 precomputed (the identical float
+// subtractions, done once per frame by k_tri_box).  Before the correctly
+// rounded 1/a, two exact rejections of u = RN(RN(1/a) * d), d = dot(s, h):
+//   sign(d) != sign(a), d != 0 and |d| >= |a| 2^-100  =>  u < 0 (no underflow)
+//   same signs and |d| > 1.001 |a|                   =>  u > 1 (rounding < 2^-22)
+// so exactly the candidates the reference rejects at this test are skipped.
 __device__ __forceinline__ bool ray_tri_pre(f3 v0, f3 edge1, f3 edge2, f3 orig, f3 dir, float& t_out, f3& normal,
                                             f3& refl, uint32_t& fl)
 {
@@ -89,9 +67,14 @@ __device__ __forceinline__ bool ray_tri_pre(f3 v0, f3 edge1, f3 edge2, f3 orig, 
     f3 h = cross(dir, edge2);
     float a = dot(edge1, h);
     if (a > -EPS && a < EPS) return false;
-    float f = 1.0f / a;
     f3 s = sub(orig, v0);
-    float u = f * dot(s, h);
+    const float d = dot(s, h);
+    const float aa = fabsf(a), ad = fabsf(d);
+    const bool opposite = (a < 0.0f) != (d < 0.0f);
+    if (d != 0.0f && opposite && ad >= aa * 0x1p-100f) return false;
+    if (!opposite && ad > aa * 1.001f) return false;
+    float f = 1.0f / a;
+    float u = f * d;
     if (u < 0.0f || u > 1.0f) return false;
     f3 q = cross(s, edge1);
     float v = f * dot(dir, q);
@@ -230,10 +213,7 @@ __device__ __forceinline__ f3 inv_dir(f3 dir)
               dir.z != 0.0f ? 1.0f / dir.z : 0.0f);
 }
 
-__device__ __forceinline__ float box_margin(const StepArgs& A)
-{
-    return A.pc.aabb_margin + (A.pc.aabb_static ? __uint_as_float(*A.vdisp) : 0.0f);
-}
+__device__ __forceinline__ float box_margin(const StepArgs& A) { return A.pc.aabb_margin; }
 
 // Serial first-hit scan of one ray (used only when the work list is full).
 __device__ bool ray_first_hit_serial(const StepArgs& A, f3 x1, f3 dir, float& rt, f3& normal, f3& refl, int32_t& obj,
@@ -263,8 +243,11 @@ __device__ bool ray_first_hit_serial(const StepArgs& A, f3 x1, f3 dir, float& rt
     return false;
 }
 
+constexpr int kStage = 4;  // passing triangle cells staged per particle
+
 __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
 {
+    __shared__ uint4 s_stage[kBlock * kStage];  // (start, count, rank) of passing cells
     __shared__ float sx[kBlock], sy[kBlock], sz[kBlock];
     __shared__ float svx[kBlock], svy[kBlock], svz[kBlock];
     __shared__ float sfx[kBlock], sfy[kBlock], sfz[kBlock];
@@ -297,7 +280,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
 
     uint32_t fl = 0;
     f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
-    uint32_t nitems = 0;
+    uint32_t nitems = 0, npass = 0;
     f3 dir = x1, inv = x1;
     RayCells rc{};
     float bm = 0.0f;
@@ -402,8 +385,13 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
                 uint2 ent[9];
                 const uint32_t mask = slab_cells(A, rc, dx, x1, inv, dir, bm, ent);
 #pragma unroll
-                for (int c = 0; c < 9; ++c)
-                    if ((mask >> c) & 1u) nitems += (ent[c].y + kItemTris - 1) / kItemTris;
+                for (int c = 0; c < 9; ++c) {
+                    if (!((mask >> c) & 1u)) continue;
+                    nitems += (ent[c].y + kItemTris - 1) / kItemTris;
+                    if (npass < kStage)
+                        s_stage[l * kStage + npass] = make_uint4(ent[c].x, ent[c].y, (uint32_t)((dx + 1) * 9 + c), 0u);
+                    ++npass;
+                }
             }
         }
         A.frc[i] = F.x; A.frc[np + i] = F.y; A.frc[2 * np + i] = F.z;
@@ -424,7 +412,15 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         const uint32_t off = wbase + incl - nitems;
         const bool fits = (uint64_t)off + nitems <= (uint64_t)A.work_cap;
         A.best[i] = (nitems > 0 && !fits) ? kScanSerial : kNoHit;
-        if (nitems > 0 && fits) {
+        if (nitems > 0 && !fits) fl |= BCS_FLAG_RAY_SERIAL;
+        if (nitems > 0 && fits && npass <= (uint32_t)kStage) {
+            uint32_t w = off;
+            for (uint32_t k = 0; k < npass; ++k) {
+                const uint4 e = s_stage[l * kStage + k];
+                for (uint32_t u = e.x; u < e.x + e.y && w < off + nitems; u += kItemTris)
+                    A.work[w++] = make_uint4((uint32_t)i, u, e.z, min(kItemTris, e.x + e.y - u));
+            }
+        } else if (nitems > 0 && fits) {
             uint32_t w = off;
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx) {
@@ -476,7 +472,10 @@ __global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
     const int l = threadIdx.x;
-    if (blockIdx.x == 0 && l == 0) A.wctrl[0] = 0u;  // work list consumed by k_ray
+    if (blockIdx.x == 0 && l == 0) {  // work list consumed by k_ray
+        A.wctrl[1] = A.wctrl[0];      // kept for frame statistics
+        A.wctrl[0] = 0u;
+    }
     const int T = A.chunk_type[blockIdx.x];
     const int first = A.chunk_first[blockIdx.x];
     const int ncell = A.chunk_n[blockIdx.x];
@@ -668,7 +667,10 @@ __global__ void __launch_bounds__(256) k_vein(VeinArgs A)
 {
     const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (MODE & 1) {
-        if (id == 0) A.hctrl[0] = 0u;  // hits consumed (k_vein_scatter ran before)
+        if (id == 0) {  // hits consumed (k_vein_scatter ran before)
+            A.hctrl[4] = A.hctrl[0];
+            A.hctrl[0] = 0u;
+        }
     }
     if (MODE & 4) {
         if (id == 0) *A.step += 1ull;
@@ -700,11 +702,6 @@ __global__ void __launch_bounds__(256) k_vein(VeinArgs A)
         const f3 Xn = add(load3(A.X, nv, id), smul(A.dt, V));
         store3(A.X, nv, id, Xn);
         store3(A.Fv, nv, id, mk(0.f, 0.f, 0.f));
-        const f3 X0 = load3(A.X0, nv, id);
-        float dmax = fmaxf(fabsf(Xn.x - X0.x), fmaxf(fabsf(Xn.y - X0.y), fabsf(Xn.z - X0.z)));
-        if (!(dmax <= 3.0e38f)) dmax = 3.0e38f;  // NaN / inf: disable culling
-        const uint32_t bits = __float_as_uint(dmax);
-        if (bits > __hip_atomic_load(A.vdisp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(A.vdisp, bits);
     } else {
         store3(A.Fv, nv, id, F);
     }
@@ -723,55 +720,64 @@ __global__ void __launch_bounds__(256) k_tri_centers(const float* __restrict__ X
     c[2 * (int64_t)ntri + t] = (a.z + b.z + d.z) / 3.0f;
 }
 
-// Per sorted triangle slot: (v0 | triangle id), v1 - v0, v2 - v0 from the
-// current vein positions.
-__global__ void __launch_bounds__(256) k_tri_prep(const int32_t* __restrict__ perm, int32_t ntri,
-                                                  const uint32_t* __restrict__ idx, const float* __restrict__ X,
-                                                  int32_t nv, float4* __restrict__ tdata)
+// Per frame, one wavefront per occupied triangle cell: every lane prepares
+// one triangle of the cell's run -- (v0 | triangle id), v1 - v0, v2 - v0 from
+// the current vein positions, stored at its sorted slot -- and the wave
+// reduces the cell's box (exact vertex bounds) stored at the run start.
+// Non-finite coordinates widen the box to everything.
+__global__ void __launch_bounds__(256) k_tri_box(const uint32_t* __restrict__ runs, const uint32_t* __restrict__ ctrl,
+                                                 const uint2* __restrict__ table,
+                                                 const uint32_t* __restrict__ keys_sorted,
+                                                 const int32_t* __restrict__ perm, int32_t ntri,
+                                                 const uint32_t* __restrict__ idx, const float* __restrict__ X,
+                                                 int32_t nv, float4* __restrict__ tdata, float4* __restrict__ aabb)
 {
-    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= ntri) return;
-    const int32_t tri = perm[u];
-    const f3 v0 = load3(X, nv, idx[3 * tri]), v1 = load3(X, nv, idx[3 * tri + 1]), v2 = load3(X, nv, idx[3 * tri + 2]);
-    const f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
-    tdata[3 * (size_t)u] = make_float4(v0.x, v0.y, v0.z, __int_as_float(tri));
-    tdata[3 * (size_t)u + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
-    tdata[3 * (size_t)u + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
-}
-
-// Box of every occupied triangle cell (stored at its run start).
-__global__ void __launch_bounds__(256) k_tri_aabb(const uint32_t* __restrict__ keys, const int32_t* __restrict__ perm,
-                                                  int32_t ntri, const uint32_t* __restrict__ idx,
-                                                  const float* __restrict__ X, int32_t nv, float4* __restrict__ aabb)
-{
-    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= ntri) return;
-    const uint32_t k = keys[u];
-    if (u > 0 && keys[u - 1] == k) return;
-    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
-    for (int32_t t = u; t < ntri && keys[t] == k; ++t) {
-        const int32_t tri = perm[t];
-        for (int c = 0; c < 3; ++c) {
-            const f3 v = load3(X, nv, idx[3 * tri + c]);
-            lo[0] = fminf(lo[0], v.x); lo[1] = fminf(lo[1], v.y); lo[2] = fminf(lo[2], v.z);
-            hi[0] = fmaxf(hi[0], v.x); hi[1] = fmaxf(hi[1], v.y); hi[2] = fmaxf(hi[2], v.z);
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t nr = ctrl[2];
+    for (uint32_t r = wave; r < nr; r += nwaves) {
+        const uint32_t s0 = runs[r];
+        const uint32_t cnt = table[keys_sorted[s0]].y;
+        float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+        for (uint32_t t = lane; t < cnt && s0 + t < (uint32_t)ntri; t += 64) {
+            const uint32_t u = s0 + t;
+            const int32_t tri = perm[u];
+            const f3 v0 = load3(X, nv, idx[3 * tri]), v1 = load3(X, nv, idx[3 * tri + 1]),
+                     v2 = load3(X, nv, idx[3 * tri + 2]);
+            const f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+            tdata[3 * (size_t)u] = make_float4(v0.x, v0.y, v0.z, __int_as_float(tri));
+            tdata[3 * (size_t)u + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+            tdata[3 * (size_t)u + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+            const f3 vs[3] = {v0, v1, v2};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float q[3] = {vs[c].x, vs[c].y, vs[c].z};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (q[d] == q[d] && fabsf(q[d]) < 3.0e38f) {
+                        lo[d] = fminf(lo[d], q[d]);
+                        hi[d] = fmaxf(hi[d], q[d]);
+                    } else {
+                        lo[d] = -3.0e38f;
+                        hi[d] = 3.0e38f;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fminf(lo[d], __shfl_xor(lo[d], off, 64));
+                hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], off, 64));
+            }
+        }
+        if (lane == 0) {
+            aabb[2 * s0] = make_float4(lo[0], lo[1], lo[2], 0.f);
+            aabb[2 * s0 + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
         }
     }
-    aabb[2 * u] = make_float4(lo[0], lo[1], lo[2], 0.f);
-    aabb[2 * u + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
-}
-
-// Max per-axis displacement |X - X0| over all vertices (after an external
-// change of the vein positions).
-__global__ void __launch_bounds__(256) k_vdisp(const float* __restrict__ X, const float* __restrict__ X0, int32_t nv,
-                                               uint32_t* __restrict__ vdisp)
-{
-    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= nv) return;
-    const f3 a = load3(X, nv, id), b = load3(X0, nv, id);
-    float dmax = fmaxf(fabsf(a.x - b.x), fmaxf(fabsf(a.y - b.y), fabsf(a.z - b.z)));
-    if (!(dmax <= 3.0e38f)) dmax = 3.0e38f;
-    atomicMax(vdisp, __float_as_uint(dmax));
 }
 
 // Contact listing in sorted-slot order (calculateParticleCollisions visit
@@ -890,21 +896,12 @@ void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t
     if (A.tt.np > 0)
         hipLaunchKernelGGL(k_contacts, dim3((A.tt.np + 255) / 256), dim3(256), 0, st, A, pass, counts, offs, pairs);
 }
-void launch_tri_prep(const int32_t* perm, int32_t ntri, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
-                     hipStream_t st)
+void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
+                    float4* aabb, hipStream_t st)
 {
-    if (ntri > 0)
-        hipLaunchKernelGGL(k_tri_prep, dim3((ntri + 255) / 256), dim3(256), 0, st, perm, ntri, idx, X, nv, tdata);
-}
-void launch_tri_aabb(const uint32_t* keys, const int32_t* perm, int32_t ntri, const uint32_t* idx, const float* X,
-                     int32_t nv, float4* aabb, hipStream_t st)
-{
-    if (ntri > 0)
-        hipLaunchKernelGGL(k_tri_aabb, dim3((ntri + 255) / 256), dim3(256), 0, st, keys, perm, ntri, idx, X, nv, aabb);
-}
-void launch_vdisp(const float* X, const float* X0, int32_t nv, uint32_t* vdisp, hipStream_t st)
-{
-    if (nv > 0) hipLaunchKernelGGL(k_vdisp, dim3((nv + 255) / 256), dim3(256), 0, st, X, X0, nv, vdisp);
+    if (tg.n > 0)
+        hipLaunchKernelGGL(k_tri_box, dim3(1024), dim3(256), 0, st, tg.runs, tg.ctrl, tg.table, tg.keys_sorted,
+                           tg.perm, tg.n, idx, X, nv, tdata, aabb);
 }
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st)
 {

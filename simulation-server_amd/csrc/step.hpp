@@ -26,7 +26,7 @@ struct PhysConst {
     float thr_upper, thr_lower, thr_right, thr_left, thr_front, thr_back;
     int32_t use_blood_flow, reaction, brake_on, nend;
     float aabb_margin;      // absolute inflation of the triangle-cell boxes (rounding slack)
-    int32_t aabb_static;    // boxes from creation-time positions (+ tracked displacement)
+    int32_t pad0;
     uint64_t seed;
     float endc[3 * kMaxEndings];
     float endr[kMaxEndings];
@@ -70,15 +70,14 @@ struct StepArgs {
     // vein hits
     uint32_t* vcount;
     float4* hits;     // 3 float4 per hit
-    uint32_t* hctrl;  // [0] hit count, [1] vein-scan ticket
+    uint32_t* hctrl;  // [0] hit count, [1..2] vein-scan control, [4] hits of the last frame
     uint64_t* vstatus;
     int32_t vntiles;
     const float4* t_aabb;     // per triangle run start: [2u] = lo, [2u+1] = hi
     uint32_t* best;           // per particle: min (cell rank << 27 | slot) of its ray hits
     uint4* work;              // ray work items: (particle, first slot, cell rank, count)
-    uint32_t* wctrl;          // [0] work items issued
+    uint32_t* wctrl;          // [0] work items issued, [1] items of the last frame
     uint32_t work_cap;
-    const uint32_t* vdisp;    // max |X - X0| per axis since the boxes were made (float bits)
     uint32_t* flags;
     uint64_t* step;
 };
@@ -98,8 +97,6 @@ struct VeinArgs {
     float4* seg;        // ordered contributions: (c.xyz, key)
     uint32_t* hctrl;
     const uint32_t* tidx;
-    const float* X0;          // positions the triangle-cell boxes were made from
-    uint32_t* vdisp;          // running max displacement (float bits, atomicMax)
     uint32_t* flags;
     uint64_t* step;
 };

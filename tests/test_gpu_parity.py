@@ -153,9 +153,10 @@ def test_c1_full_one_frame(bcs, orc):
 
 
 def test_displaced_vein_keeps_parity(bcs, orc):
-    """Vein moved far from its creation position: the triangle-cell boxes
-    (made at creation) are inflated by the tracked displacement, so the
-    culled ray scan still finds exactly the oracle's hits."""
+    """Vein moved away from its creation position (the frozen triangle grid
+    keeps its creation-time cells): the per-frame triangle-cell boxes follow
+    the moved triangles, so the culled ray scan still finds exactly the
+    oracle's hits."""
     sc = _scene(bcs, "c0")
     sim, o = _pair(bcs, orc, sc)
     X, V, F = sim.get_vein()
