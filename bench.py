@@ -123,6 +123,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="enqueue the fused frame kernels directly instead of replaying the HIP graph")
+    ap.add_argument("--calibrate-bytes", type=int, default=0,
+                    help="also run the plain copy kernel (k_copy) over this many bytes, for HBM counter calibration")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -135,6 +139,8 @@ def main():
         sys.exit(1)
     scene = bcs.Scene(args.config, 1.0, seed=args.seed)
     sim = bcs.Simulation.from_scene(scene, device=0)
+    if args.no_graph:
+        sim.use_graph(False)
     info = sim.info
     np_, nc, nv, nt = info.particle_count, info.blood_cell_count, info.vertex_count, info.triangle_count
 
@@ -199,6 +205,7 @@ def main():
             "particle_grid": list(info.particle_cells),
             "triangle_grid": list(info.triangle_cells),
             "dt": scene.physics.dt,
+            "frame_launch": "direct" if args.no_graph else "hip-graph",
         },
         "roofline": roof,
         "step_roofline": step_roof,
@@ -207,6 +214,10 @@ def main():
         "error_flags": flags,
         "frame_stats": frame_stats,
     }
+    if args.calibrate_bytes:
+        cms = bcs.calibration_copy(args.calibrate_bytes, 3)
+        out["calibration_copy"] = {"bytes_read": args.calibrate_bytes, "bytes_written": args.calibrate_bytes,
+                                   "ms": cms, "GBps": 2 * args.calibrate_bytes / (cms * 1e-3) / 1e9}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(out))

@@ -239,6 +239,12 @@ int bcs_sim_propagate_all(bcs_sim* sim);
  * graph.  Asynchronous: returns after enqueueing. */
 int bcs_sim_step(bcs_sim* sim, int32_t n_steps);
 
+/* Replay frames from the HIP graph (enabled != 0, the default) or enqueue the
+ * same fused kernels directly per frame (enabled == 0, e.g. for profilers that
+ * do not follow graph launches).  Results are identical either way.  No
+ * counterpart in the reference (its frames are direct launches). */
+int bcs_sim_use_graph(bcs_sim* sim, int32_t enabled);
+
 int bcs_sim_synchronize(bcs_sim* sim);
 
 /* Time n fused steps with HIP events on the simulation's stream (ms total). */

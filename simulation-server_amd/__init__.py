@@ -206,6 +206,7 @@ def _sig(lib):
     lib.bcs_sim_calculate_next_frame.argtypes = [vp]
     lib.bcs_sim_propagate_all.argtypes = [vp]
     lib.bcs_sim_step.argtypes = [vp, C.c_int32]
+    lib.bcs_sim_use_graph.argtypes = [vp, C.c_int32]
     lib.bcs_sim_synchronize.argtypes = [vp]
     lib.bcs_sim_time_steps.argtypes = [vp, C.c_int32, f32p]
     lib.bcs_sim_profile_kernels.argtypes = [vp, C.c_int32, f32p, C.c_int32, i32p, C.c_char_p, C.c_int32]
@@ -457,6 +458,9 @@ class Simulation:
         check(self.lib.bcs_sim_step(self.h, int(n)), "step")
         if sync:
             self.synchronize()
+
+    def use_graph(self, enabled: bool = True):
+        check(self.lib.bcs_sim_use_graph(self.h, 1 if enabled else 0), "use_graph")
 
     def synchronize(self):
         check(self.lib.bcs_sim_synchronize(self.h), "synchronize")

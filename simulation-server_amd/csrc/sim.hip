@@ -31,6 +31,9 @@ void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
 void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
                     float4* aabb, hipStream_t st);
+int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
+                 uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
+                 uint32_t* flags, hipStream_t st);
 }  // namespace bcs
 
 using namespace bcs;
@@ -55,6 +58,12 @@ struct bcs_sim {
     float *pos = nullptr, *vel = nullptr, *frc = nullptr, *centers = nullptr;
     float *bs = nullptr, *r0 = nullptr, *model = nullptr, *sp_L = nullptr;
     int32_t *sp_off = nullptr, *sp_b = nullptr, *chunk_type = nullptr, *chunk_first = nullptr, *chunk_n = nullptr;
+    // cell processing order (spatial; see k_order_key)
+    int32_t* corder = nullptr;
+    uint32_t *okey = nullptr, *ocount = nullptr, *ostart = nullptr, *octrl = nullptr;
+    uint64_t* ostatus = nullptr;
+    int32_t otiles = 0;
+    int32_t frames_since_order = 0;
     // vein
     float *X = nullptr, *V = nullptr, *Fv = nullptr, *nbl = nullptr, *tcent = nullptr;
     int32_t* nbr = nullptr;
@@ -76,6 +85,7 @@ struct bcs_sim {
     uint32_t* flags = nullptr;
     uint64_t* step = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    bool use_graph = true;
     hipGraph_t graph = nullptr;
     std::vector<float> host_bs;
     std::vector<float> host_smallest;
@@ -107,6 +117,7 @@ struct bcs_sim {
         A.pos = pos; A.vel = vel; A.frc = frc; A.centers = centers;
         A.bs = bs; A.r0 = r0; A.model = model; A.sp_off = sp_off; A.sp_b = sp_b; A.sp_L = sp_L;
         A.chunk_type = chunk_type; A.chunk_first = chunk_first; A.chunk_n = chunk_n;
+        A.corder = corder; A.nchunks = nchunks;
         A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table;
         A.xs4 = pg.xs4; A.vs4 = pg.vs4;
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
@@ -376,6 +387,13 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(s->dalloc(&s->bs, nm)); RC(s->dalloc(&s->r0, nm)); RC(s->dalloc(&s->model, 3 * nm));
     RC(s->dalloc(&s->sp_off, nm + 1)); RC(s->dalloc(&s->sp_b, hb.size())); RC(s->dalloc(&s->sp_L, hL.size()));
     RC(s->dalloc(&s->chunk_type, ct.size())); RC(s->dalloc(&s->chunk_first, cf.size())); RC(s->dalloc(&s->chunk_n, cn.size()));
+    {
+        const int64_t nbins = (int64_t)n_types << kOrderBits;
+        s->otiles = scan_tiles(nbins);
+        RC(s->dalloc(&s->corder, nc)); RC(s->dalloc(&s->okey, nc));
+        RC(s->dalloc(&s->ocount, nbins)); RC(s->dalloc(&s->ostart, nbins + 1));
+        RC(s->dalloc(&s->ostatus, s->otiles)); RC(s->dalloc(&s->octrl, 4));
+    }
     RC(s->dalloc(&s->X, 3 * NV)); RC(s->dalloc(&s->V, 3 * NV)); RC(s->dalloc(&s->Fv, 3 * NV));
     RC(s->dalloc(&s->nbr, kVeinSlots * NV)); RC(s->dalloc(&s->nbl, kVeinSlots * NV));
     RC(s->dalloc(&s->tidx, 3 * NT)); RC(s->dalloc(&s->tcent, 3 * NT));
@@ -400,6 +418,13 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     if (!hb.empty()) {
         HIPCHK(hipMemcpyAsync(s->sp_b, hb.data(), 4 * hb.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync(s->sp_L, hL.data(), 4 * hL.size(), hipMemcpyHostToDevice, s->stream));
+    }
+    {
+        std::vector<int32_t> ident(std::max<int64_t>(nc, 1));
+        for (int64_t c = 0; c < nc; ++c) ident[c] = (int32_t)c;
+        HIPCHK(hipMemcpyAsync(s->corder, ident.data(), 4 * (size_t)nc, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemsetAsync(s->ocount, 0, 4 * ((size_t)n_types << kOrderBits), s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));  // ident dies here
     }
     if (!ct.empty()) {
         HIPCHK(hipMemcpyAsync(s->chunk_type, ct.data(), 4 * ct.size(), hipMemcpyHostToDevice, s->stream));
@@ -616,8 +641,27 @@ extern "C" int bcs_sim_step(bcs_sim* s, int32_t n_steps)
     RC(set_dev(s));
     if (n_steps < 0) return fail(BCS_ERR_INVALID_ARGUMENT, "negative step count");
     if (n_steps == 0) return BCS_OK;
-    RC(ensure_graph(s));
-    for (int32_t k = 0; k < n_steps; ++k) HIPCHK(hipGraphLaunch(s->graph_exec, s->stream));
+    if (s->use_graph) RC(ensure_graph(s));
+    for (int32_t k = 0; k < n_steps; ++k) {
+        // Refresh the spatial processing order of the cells from the last
+        // frame's centres (performance only: results do not depend on it).
+        if (s->frames_since_order >= kReorderPeriod) {
+            RC(launch_order(s->tt, s->pg.g, s->centers, s->okey, s->ocount, s->ostart, s->ostatus, s->otiles,
+                            s->octrl, s->corder, s->flags, s->stream));
+            s->frames_since_order = 0;
+        }
+        if (s->use_graph) HIPCHK(hipGraphLaunch(s->graph_exec, s->stream));
+        else RC(enqueue_fused_frame(s));
+        ++s->frames_since_order;
+    }
+    HIPCHK(hipGetLastError());
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_use_graph(bcs_sim* s, int32_t enabled)
+{
+    if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
+    s->use_graph = enabled != 0;
     return BCS_OK;
 }
 

@@ -42,6 +42,26 @@ __device__ __forceinline__ int type_of_index(const TypeTable& tt, int32_t s)
 }
 
 // addResilientForceOnCollision, src/simulation/physics.cuh:133-145 (increment).
+// Chunk of this block.  Workgroups are dispatched round-robin over the 8
+// XCDs, so block b runs on XCD b % 8; each XCD takes one contiguous eighth of
+// the chunk list (and, with corder spatially sorted, one compact region of the
+// vein, whose grid-table lines then stay in that XCD's L2).  The grid is
+// 8 * ceil(nchunks / 8) blocks; blocks past the end return -1.
+__device__ __forceinline__ int chunk_of_block(int32_t nchunks)
+{
+    const int per = (nchunks + 7) >> 3;
+    const int c = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    return c < nchunks ? c : -1;
+}
+
+// Particle of vertex a of the cl-th cell of chunk (T, first).
+__device__ __forceinline__ int32_t chunk_particle(const StepArgs& A, int T, int first, int cl, int a, int p,
+                                                  int32_t& cell)
+{
+    cell = A.corder[A.tt.cstart[T] + first + cl];
+    return A.tt.pstart[T] + (cell - A.tt.cstart[T]) * p + a;
+}
+
 __device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVel, float d2, float radius, float coef,
                                         uint32_t& fl)
 {
@@ -261,16 +281,19 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         for (int64_t t = gid; t < A.vntiles; t += (int64_t)gridDim.x * blockDim.x) A.vstatus[t] = 0ull;
         if (gid == 0) A.hctrl[1] = 0u;
     }
-    const int T = A.chunk_type[blockIdx.x];
-    const int first = A.chunk_first[blockIdx.x];
-    const int ncell = A.chunk_n[blockIdx.x];
+    const int chunk = chunk_of_block(A.nchunks);
+    if (chunk < 0) return;
+    const int T = A.chunk_type[chunk];
+    const int first = A.chunk_first[chunk];
+    const int ncell = A.chunk_n[chunk];
     const int p = tt.ppc[T];
     const int nloc = ncell * p;
     const bool act = l < nloc;
     const int a = act ? l % p : 0;
     const int cl = act ? l / p : 0;
     const int64_t np = tt.np;
-    const int32_t i = tt.pstart[T] + first * p + l;
+    int32_t cell = 0;
+    const int32_t i = act ? chunk_particle(A, T, first, cl, a, p, cell) : 0;
     if (act) {
         sx[l] = A.pos[i]; sy[l] = A.pos[np + i]; sz[l] = A.pos[2 * np + i];
         svx[l] = A.vel[i]; svy[l] = A.vel[np + i]; svz[l] = A.vel[2 * np + i];
@@ -293,7 +316,6 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         f3 cen = mk(0.f, 0.f, 0.f);
         for (int b = 0; b < p; ++b) cen = add(cen, mk(sx[base + b], sy[base + b], sz[base + b]));
         cen = sdiv(cen, (float)p);
-        const int32_t cell = tt.cstart[T] + first + cl;
         if (a == 0) {
             A.centers[cell] = cen.x;
             A.centers[tt.nc + cell] = cen.y;
@@ -476,15 +498,18 @@ __global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
         A.wctrl[1] = A.wctrl[0];      // kept for frame statistics
         A.wctrl[0] = 0u;
     }
-    const int T = A.chunk_type[blockIdx.x];
-    const int first = A.chunk_first[blockIdx.x];
-    const int ncell = A.chunk_n[blockIdx.x];
+    const int chunk = chunk_of_block(A.nchunks);
+    if (chunk < 0) return;
+    const int T = A.chunk_type[chunk];
+    const int first = A.chunk_first[chunk];
+    const int ncell = A.chunk_n[chunk];
     const int p = tt.ppc[T];
     const bool act = l < ncell * p;
     const int a = act ? l % p : 0;
     const int cl = act ? l / p : 0;
     const int64_t np = tt.np;
-    const int32_t i = tt.pstart[T] + first * p + l;
+    int32_t cell = 0;
+    const int32_t i = act ? chunk_particle(A, T, first, cl, a, p, cell) : 0;
     if (l < ncell) s_tele[l] = 0;
     uint32_t fl = 0;
     f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
@@ -548,7 +573,6 @@ __global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
         if (act && pc.use_blood_flow && vein_end_flag(pc, tt.end_spheres[T] != 0, x1)) s_tele[cl] = 1;
         __syncthreads();
         if (act && pc.use_blood_flow && s_tele[cl]) {
-            const int32_t cell = tt.cstart[T] + first + cl;
             const uint64_t stp = *A.step;
             const float u1 = uniform01(pc.seed, stp, (uint64_t)cell, 0u);
             const float u2 = uniform01(pc.seed, stp, (uint64_t)cell, 1u);
@@ -586,12 +610,15 @@ __global__ void __launch_bounds__(kBlock) k_vein_end(StepArgs A)
     __shared__ int s_tele[kBlock];
     const TypeTable& tt = A.tt;
     const int l = threadIdx.x;
-    const int T = A.chunk_type[blockIdx.x], first = A.chunk_first[blockIdx.x], ncell = A.chunk_n[blockIdx.x];
+    const int chunk = chunk_of_block(A.nchunks);
+    if (chunk < 0) return;
+    const int T = A.chunk_type[chunk], first = A.chunk_first[chunk], ncell = A.chunk_n[chunk];
     const int p = tt.ppc[T];
     const bool act = l < ncell * p;
     const int a = act ? l % p : 0, cl = act ? l / p : 0;
     const int64_t np = tt.np;
-    const int32_t i = tt.pstart[T] + first * p + l;
+    int32_t cell = 0;
+    const int32_t i = act ? chunk_particle(A, T, first, cl, a, p, cell) : 0;
     if (l < ncell) s_tele[l] = 0;
     __syncthreads();
     f3 x = mk(0.f, 0.f, 0.f);
@@ -601,7 +628,6 @@ __global__ void __launch_bounds__(kBlock) k_vein_end(StepArgs A)
     }
     __syncthreads();
     if (act && s_tele[cl]) {
-        const int32_t cell = tt.cstart[T] + first + cl;
         const uint64_t stp = *A.step;
         const float u1 = uniform01(A.pc.seed, stp, (uint64_t)cell, 0u);
         const float u2 = uniform01(A.pc.seed, stp, (uint64_t)cell, 1u);
@@ -780,6 +806,66 @@ __global__ void __launch_bounds__(256) k_tri_box(const uint32_t* __restrict__ ru
     }
 }
 
+// ---- processing order of the cells: by type, then by the Morton code of the
+// cell centre (previous frame) on a 1024^3 lattice over the particle-grid box,
+// cut to its top kOrderBits bits.  A counting sort; the order within a bin is
+// arbitrary (it changes which block handles a cell, never a result).
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t lattice10(float c, float lo, float ext)
+{
+    float t = (c - lo) / ext * 1024.0f;
+    if (!(t >= 0.0f)) t = 0.0f;  // NaN too
+    if (t > 1023.0f) t = 1023.0f;
+    return (uint32_t)t;
+}
+
+__global__ void __launch_bounds__(256) k_order_key(TypeTable tt, GridGeom g, const float* __restrict__ centers,
+                                                   uint32_t* __restrict__ okey, uint32_t* __restrict__ ocount,
+                                                   uint64_t* __restrict__ ostatus, int32_t otiles,
+                                                   uint32_t* __restrict__ octrl)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int32_t t = c; t < otiles; t += gridDim.x * blockDim.x) ostatus[t] = 0ull;
+    if (c == 0) octrl[0] = 0u;
+    if (c >= tt.nc) return;
+    int T = 0;
+    while (T + 1 < tt.nt && c >= tt.cstart[T + 1]) ++T;
+    const int64_t nc = tt.nc;
+    const uint32_t qx = lattice10(centers[c], g.minv[0], g.extent[0]);
+    const uint32_t qy = lattice10(centers[nc + c], g.minv[1], g.extent[1]);
+    const uint32_t qz = lattice10(centers[2 * nc + c], g.minv[2], g.extent[2]);
+    const uint32_t m = (spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2)) >> (30 - kOrderBits);
+    const uint32_t key = ((uint32_t)T << kOrderBits) | m;
+    okey[c] = key;
+    atomicAdd(&ocount[key], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_order_scatter(int32_t nc, const uint32_t* __restrict__ okey,
+                                                       uint32_t* __restrict__ ocount,
+                                                       const uint32_t* __restrict__ ostart, int32_t* __restrict__ corder,
+                                                       uint32_t* __restrict__ flags)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const uint32_t key = okey[c];
+    const uint32_t old = atomicSub(&ocount[key], 1u);  // counts return to zero
+    const uint32_t pos = ostart[key] + old - 1u;
+    if (old == 0u || pos >= (uint32_t)nc) {
+        atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);
+        return;
+    }
+    corder[pos] = c;
+}
+
 // Contact listing in sorted-slot order (calculateParticleCollisions visit
 // order; the force is not applied).  pass 0 counts, pass 1 writes pairs at
 // offs[i].
@@ -845,9 +931,25 @@ __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, fl
 
 // ------------------------------------------------------------ launchers --
 
+int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
+                 uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
+                 uint32_t* flags, hipStream_t st)
+{
+    if (tt.nc <= 0) return BCS_OK;
+    const int nb = (tt.nc + 255) / 256;
+    hipLaunchKernelGGL(k_order_key, dim3(std::max(nb, (otiles + 255) / 256)), dim3(256), 0, st, tt, g, centers,
+                       okey, ocount, ostatus, otiles, octrl);
+    const int rc = scan_exclusive(ocount, ostart, (int64_t)tt.nt << kOrderBits, ostatus, otiles, octrl, flags, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, st, tt.nc, okey, ocount, ostart, corder, flags);
+    return BCS_OK;
+}
+
+static dim3 chunk_grid(int nchunks) { return dim3(8 * ((nchunks + 7) / 8)); }
+
 void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st)
 {
-    if (nchunks > 0) hipLaunchKernelGGL(k_cells_a, dim3(nchunks), dim3(kBlock), 0, st, A);
+    if (nchunks > 0) hipLaunchKernelGGL(k_cells_a, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
 }
 void launch_ray(const StepArgs& A, hipStream_t st)
 {
@@ -856,8 +958,8 @@ void launch_ray(const StepArgs& A, hipStream_t st)
 void launch_cells_b(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
 {
     if (nchunks <= 0) return;
-    if (fused) hipLaunchKernelGGL(k_cells_b<true>, dim3(nchunks), dim3(kBlock), 0, st, A);
-    else hipLaunchKernelGGL(k_cells_b<false>, dim3(nchunks), dim3(kBlock), 0, st, A);
+    if (fused) hipLaunchKernelGGL(k_cells_b<true>, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_cells_b<false>, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
 }
 void launch_integrate(const StepArgs& A, hipStream_t st)
 {
@@ -865,7 +967,7 @@ void launch_integrate(const StepArgs& A, hipStream_t st)
 }
 void launch_vein_end(const StepArgs& A, int nchunks, hipStream_t st)
 {
-    if (nchunks > 0) hipLaunchKernelGGL(k_vein_end, dim3(nchunks), dim3(kBlock), 0, st, A);
+    if (nchunks > 0) hipLaunchKernelGGL(k_vein_end, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
 }
 void launch_bump(uint64_t* step, hipStream_t st) { hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(1), 0, st, step); }
 void launch_vein_springs(const VeinArgs& V, hipStream_t st)

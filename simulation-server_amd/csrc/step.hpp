@@ -7,6 +7,8 @@ namespace bcs {
 
 constexpr int kMaxEndings = 16;
 constexpr int kBlock = 256;  // threads per cell-chunk block
+constexpr int kOrderBits = 15;      // Morton bits of the cell processing order
+constexpr int kReorderPeriod = 16;  // frames between updates of the order
 
 // Folded type tables (src/meta_factory/blood_cell_factory.hpp:172-282).
 struct TypeTable {
@@ -47,10 +49,12 @@ struct StepArgs {
     const int32_t* sp_off;  // CSR per model vertex, ascending neighbour
     const int32_t* sp_b;
     const float* sp_L;
-    // work chunks (whole cells of one type)
+    // work chunks (whole cells of one type), taken in the order of corder
     const int32_t* chunk_type;
     const int32_t* chunk_first;
     const int32_t* chunk_n;
+    const int32_t* corder;  // processing order of the cells (a permutation within each type)
+    int32_t nchunks;
     // particle grid
     GridGeom pg;
     const uint32_t* p_keys;

@@ -117,6 +117,19 @@ def test_fused_100_steps_c0(bcs, orc):
     assert sim.get_info().step_index == 100
 
 
+def test_fused_40_steps_mixed_types(bcs, orc):
+    """Two cell types (discs and icosahedra): 40 fused frames, across two
+    refreshes of the spatial processing order of the cells."""
+    sc = _scene(bcs, "mixed")
+    sim, o = _pair(bcs, orc, sc)
+    sim.step(40)
+    o.step(40)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "mixed 40 frames " + n)
+    assert_bits_equal(sim.get_centers(), o.get_centers(), "mixed 40 frames centres")
+    assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "mixed 40 frames vein")
+
+
 def test_fused_equals_phase_api(bcs):
     sc = _scene(bcs, "mixed")
     a = bcs.Simulation.from_scene(sc)
@@ -128,6 +141,20 @@ def test_fused_equals_phase_api(bcs):
         b.propagate_all()
     for x, y, n in zip(a.get_particles(), b.get_particles(), ("pos", "vel", "force")):
         assert_bits_equal(x, y, "fused vs phase " + n)
+
+
+def test_graph_replay_equals_direct_launch(bcs):
+    sc = _scene(bcs, "c1_small")
+    a = bcs.Simulation.from_scene(sc)
+    b = bcs.Simulation.from_scene(sc)
+    b.use_graph(False)
+    for _ in range(3):  # replays interleaved with host synchronisation
+        a.step(4)
+        b.step(4)
+    for x, y, n in zip(a.get_particles(), b.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "graph vs direct " + n)
+    assert_bits_equal(a.get_vein()[0], b.get_vein()[0], "graph vs direct vein")
+    assert a.get_info().step_index == b.get_info().step_index == 12
 
 
 def test_determinism_two_runs(bcs):
