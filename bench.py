@@ -42,22 +42,30 @@ WORKLOADS = {
 
 
 def step_bytes(np_, n_occ, nc, nv, nt):
-    """Algorithmic HBM bytes of one plain frame:
+    """Algorithmic HBM bytes of one plain frame: positions, velocities and
+    forces read and written once (72 / particle), the sorted key, index and
+    grid keys (16 / particle), cell table entries (16 / occupied cell), cell
+    centres (24 / blood cell), vein state, springs and triangle data
+    (144 / vertex + 16 / triangle):
     88 N_p + 16 N_occ + 24 N_c + 144 N_v + 16 N_t."""
     return 88 * np_ + 16 * n_occ + 24 * nc + 144 * nv + 16 * nt
 
 
-def kernel_bytes(name, np_, n_occ, nc, nv, nt):
+def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
     """Algorithmic HBM bytes of one launch of each frame kernel (every datum
     counted once, as a perfect cache would move it):
-      k_key        x read 12, previous key read 4, its table entry cleared 8,
-                   key written 4                                   28 / particle
+      k_key        x read 12, previous key read 4, its occupancy byte cleared 1,
+                   key written 4                                   21 / particle
       k_scatter    key read 4, (key, index) written 8              12 / particle
-      k_rank       (key, index) read 8, sorted key, index, inverse written 12,
-                   x, v gathered 24, float4 sorted copies written 32
-                   + {start, count} per occupied cell 8            76 / particle + 8 / cell
-      k_cells_a    x, v, F read 36, F written 12, sorted copy read once as a
-                   neighbour 16, first-hit key written 4           68 / particle + 12 / blood cell
+      k_rank       (key, index) read 8, sorted key, index, inverse written 12
+                   + {start, count} and occupancy byte per occupied cell 9
+                                                                   20 / particle + 9 / cell
+      k_cells_a    x, v, F read 36, own key 4, F and first-hit key written 16;
+                   centre written 12 per blood cell; table entry + occupancy
+                   byte read 9 per occupied cell; ray work items written 16 each
+                                                                   56 / particle + 12 / blood cell
+                                                                   + 9 / cell + 16 / item
+      k_ray        work item 16 + x, v 24 per item (triangle data counted in k_tri_box)
       k_cells_b    x, v, F and first-hit key read 40, x, v, F written 36
                                                                    76 / particle
       k_vein_springs+k_tri_box  X, V read 24, 9 neighbour ids + lengths 72,
@@ -67,10 +75,11 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt):
       k_vein       X, V, Fv read 36, X, V, Fv written 36, offsets 4
                                                                    76 / vertex"""
     table = {
-        "k_key": 28 * np_,
+        "k_key": 21 * np_,
         "k_scatter": 12 * np_,
-        "k_rank": 76 * np_ + 8 * n_occ,
-        "k_cells_a": 68 * np_ + 12 * nc,
+        "k_rank": 20 * np_ + 9 * n_occ,
+        "k_cells_a": 56 * np_ + 12 * nc + 9 * n_occ + 16 * n_items,
+        "k_ray": 40 * n_items,
         "k_cells_b": 76 * np_,
         "k_vein_springs+k_tri_box": 120 * nv + 76 * nt,
         "k_vein": 76 * nv,
@@ -163,7 +172,7 @@ def main():
     kern = sim.profile_kernels(args.profile_steps)
     dominant = max(kern, key=kern.get)
     dom_ms = kern[dominant]
-    dom_bytes = kernel_bytes(dominant, np_, n_occ, nc, nv, nt)
+    dom_bytes = kernel_bytes(dominant, np_, n_occ, nc, nv, nt, frame_stats.get("ray_work_items", 0))
     achieved = dom_bytes / (dom_ms * 1e-3) if dom_bytes else None
     roof = {
         "kernel": dominant,

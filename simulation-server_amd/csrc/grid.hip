@@ -13,9 +13,12 @@
 //              exactly thrust's stable order (ties by object index); run starts
 //              write the dense cell-start table
 //   k_big      the same for buckets larger than the k_rank threshold (LDS tiles)
-// Cell table: dense {start, count} per cell id (8 bytes, one load per lookup).
-// Only occupied cells are written; k_key first clears the entries of the
-// previous build's keys, so every entry is either current or {0, 0}.
+// Cell table: dense {start, count} per cell id (8 bytes, one load per lookup),
+// written for occupied cells only, plus an occupancy byte map (saturating
+// object count per cell, 0 = empty).  k_key first zeroes the map bytes of the
+// previous build's keys, so the map is always current.  The table entries of
+// cells that emptied are cleared only when clear_table is set (the triangle
+// grid); readers of the particle grid consult the map first.
 #include "grid.hpp"
 
 #include <algorithm>
@@ -33,7 +36,8 @@ constexpr uint64_t kSpinLimit = 1ull << 22;
 
 __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict__ pos, int64_t stride, int32_t n,
                                              int32_t n_prev, int shift, uint32_t* __restrict__ keys,
-                                             uint2* __restrict__ table, uint32_t* __restrict__ bcount,
+                                             uint2* __restrict__ table, uint8_t* __restrict__ occ, int clear_table,
+                                             uint32_t* __restrict__ bcount,
                                              uint64_t* __restrict__ status, int32_t ntiles,
                                              uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags)
 {
@@ -47,7 +51,10 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
     }
     if (gid < n_prev) {  // clear the previous build's cell of this object
         const uint32_t old = keys[gid];
-        if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
+        if ((uint64_t)old < g.cell_count) {
+            occ[old] = 0;
+            if (clear_table) table[old] = make_uint2(0u, 0u);
+        }
     }
     uint32_t fl = 0;
     if (gid < n) {
@@ -169,12 +176,8 @@ struct RankOut {
     int32_t* perm;
     int32_t* slot_of;      // nullable
     uint2* table;          // {start, count} per cell
-    const float* src_pos;  // sorted copies (nullable)
-    const float* src_vel;
-    int64_t src_stride;
-    float4* xs4;           // (x, y, z, index bits) in sorted order
-    float4* vs4;           // (vx, vy, vz, 0) in sorted order
-    uint32_t* runs;        // run starts of occupied cells, in no particular order
+    uint8_t* occ;          // saturating count per cell
+    uint32_t* runs;        // run starts of occupied cells, in no particular order (nullable)
     uint32_t* nruns;
 };
 
@@ -186,12 +189,8 @@ __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32
     if (o.slot_of) o.slot_of[idx] = (int32_t)pos;
     if (first) {
         o.table[key] = make_uint2(pos, count);
+        o.occ[key] = (uint8_t)min(count, 255u);
         if (o.runs) o.runs[atomicAdd(o.nruns, 1u)] = pos;
-    }
-    if (o.xs4) {
-        const int64_t s = o.src_stride;
-        o.xs4[pos] = make_float4(o.src_pos[idx], o.src_pos[s + idx], o.src_pos[2 * s + idx], __int_as_float(idx));
-        o.vs4[pos] = make_float4(o.src_vel[idx], o.src_vel[s + idx], o.src_vel[2 * s + idx], 0.0f);
     }
 }
 
@@ -289,7 +288,7 @@ static int choose_shift(uint64_t cell_count, int64_t cap)
         if (e_ != hipSuccess) return fail(BCS_ERR_HIP, std::string("grid: ") + hipGetErrorString(e_)); \
     } while (0)
 
-int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_copies, hipStream_t st)
+int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_runs, hipStream_t st)
 {
     G = DeviceGrid{};
     G.g = geom;
@@ -305,18 +304,19 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     GRID_HIP(hipMalloc(&G.items_k, n * 4));
     GRID_HIP(hipMalloc(&G.items_i, n * 4));
     GRID_HIP(hipMalloc(&G.keys_sorted, n * 4));
-    GRID_HIP(hipMalloc(&G.runs, n * 4));
+    if (with_runs) GRID_HIP(hipMalloc(&G.runs, n * 4));
     GRID_HIP(hipMalloc(&G.perm, n * 4));
     if (with_slot) GRID_HIP(hipMalloc(&G.slot_of, n * 4));
-    if (with_copies) {
-        GRID_HIP(hipMalloc(&G.xs4, n * 16));
-        GRID_HIP(hipMalloc(&G.vs4, n * 16));
-    }
     GRID_HIP(hipMemsetAsync(G.keys, 0xFF, n * 4, st));
     hipError_t e = hipMalloc(&G.table, geom.cell_count * 8);
     if (e != hipSuccess)
         return fail(BCS_ERR_OUT_OF_MEMORY, "grid: cell table of " + std::to_string(geom.cell_count * 8) + " bytes");
     GRID_HIP(hipMemsetAsync(G.table, 0, geom.cell_count * 8, st));
+    const size_t occ_bytes = 2 * kOccPad + ((geom.cell_count + 3) & ~3ull);
+    e = hipMalloc(&G.occ_alloc, occ_bytes);
+    if (e != hipSuccess) return fail(BCS_ERR_OUT_OF_MEMORY, "grid: occupancy map of " + std::to_string(occ_bytes) + " bytes");
+    GRID_HIP(hipMemsetAsync(G.occ_alloc, 0, occ_bytes, st));
+    G.occ = G.occ_alloc + kOccPad;
     GRID_HIP(hipMalloc(&G.bcount, (G.nb + 1) * 4));
     GRID_HIP(hipMemsetAsync(G.bcount, 0, (G.nb + 1) * 4, st));
     GRID_HIP(hipMalloc(&G.bstart, (G.nb + 2) * 4));
@@ -330,15 +330,15 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
 
 void grid_free(DeviceGrid& G)
 {
-    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.runs, G.perm, G.slot_of, G.xs4, G.vs4,
-                    G.table, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
+    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.runs, G.perm, G.slot_of,
+                    G.table, G.occ_alloc, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     G = DeviceGrid{};
 }
 
-int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
-               hipStream_t st, const KernelTimer* timer)
+int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint32_t* flags, hipStream_t st,
+               const KernelTimer* timer)
 {
     if (n < 0 || n > G.cap) return fail(BCS_ERR_INVALID_ARGUMENT, "grid: object count exceeds capacity");
     // Every slot below the capacity clears its previous cell (slots never
@@ -351,7 +351,8 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     auto te = [&](int id) { if (timer) timer->end(id, st); };
     tb(0);
     hipLaunchKernelGGL(k_key, dim3(std::max(blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride, n,
-                       n_prev, G.shift, G.keys, G.table, G.bcount, G.status, G.ntiles, G.ctrl, flags);
+                       n_prev, G.shift, G.keys, G.table, G.occ, G.clear_table ? 1 : 0, G.bcount, G.status, G.ntiles,
+                       G.ctrl, flags);
     te(0);
     tb(1);
     hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,
@@ -361,8 +362,7 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.bcount, G.items_k,
                        G.items_i, flags);
     te(2);
-    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, vel ? pos : nullptr, vel, stride, vel ? G.xs4 : nullptr,
-              vel ? G.vs4 : nullptr, vel ? nullptr : G.runs, G.ctrl + 2};
+    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.occ, G.runs, G.ctrl + 2};
     tb(3);
     hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, o);
     te(3);

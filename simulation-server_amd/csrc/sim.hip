@@ -118,8 +118,7 @@ struct bcs_sim {
         A.bs = bs; A.r0 = r0; A.model = model; A.sp_off = sp_off; A.sp_b = sp_b; A.sp_L = sp_L;
         A.chunk_type = chunk_type; A.chunk_first = chunk_first; A.chunk_n = chunk_n;
         A.corder = corder; A.nchunks = nchunks;
-        A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table;
-        A.xs4 = pg.xs4; A.vs4 = pg.vs4;
+        A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table; A.p_occ = pg.occ; A.p_key_of = pg.keys;
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.vcount = vcount; A.hits = hits; A.hctrl = hctrl; A.vstatus = vstatus; A.vntiles = vntiles;
@@ -176,14 +175,14 @@ int build_triangle_grid(bcs_sim* s)
 {
     const bool dyn = s->phys.dynamic_triangle_grid != 0;
     if (dyn || !s->tg_built) launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
-    RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, nullptr, s->flags, s->stream));
+    RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, s->flags, s->stream));
     s->tg_built = true;
     return BCS_OK;
 }
 
 int enqueue_particle_grid(bcs_sim* s, const KernelTimer* timer = nullptr)
 {
-    return grid_build(s->pg, s->pos, s->np, s->np, s->vel, s->flags, s->stream, timer);
+    return grid_build(s->pg, s->pos, s->np, s->np, s->flags, s->stream, timer);
 }
 
 int enqueue_vein_hits(bcs_sim* s, const VeinArgs& Va)
@@ -454,8 +453,9 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     // ---- grids
     const GridGeom pgeom = make_geom(bmin, bmax, phys->particle_cell);
     const GridGeom tgeom = make_geom(bmin, bmax, phys->triangle_cell);
-    RC(grid_alloc(s->pg, pgeom, s->np, true, true, s->stream));
-    RC(grid_alloc(s->tg, tgeom, s->ntri, false, false, s->stream));
+    RC(grid_alloc(s->pg, pgeom, s->np, true, false, s->stream));
+    s->pg.clear_table = false;  // readers consult the occupancy map
+    RC(grid_alloc(s->tg, tgeom, s->ntri, false, true, s->stream));
     RC(build_triangle_grid(s.get()));
     HIPCHK(hipStreamSynchronize(s->stream));
     *out = s.release();

@@ -227,6 +227,45 @@ __device__ __forceinline__ uint32_t slab_cells(const StepArgs& A, const RayCells
     return mask;
 }
 
+// Neighbour-cell ranges (clipped to the grid) of a particle.
+struct NbrRange {
+    int xl, xh, yl, yh, zl, zh;
+};
+
+// Occupied cells of the 27-neighbourhood of cell `cellId` as a mask over the
+// visit rank r = (dx+1)*9 + (dy+1)*3 + (dz+1) (the reference's x, y, z loop
+// order), restricted to the clipped ranges and to keys in [0, cellCount).
+// Each of the nine (dy, dz) rows is one pair of word loads from the
+// occupancy map (its three x-cells lie in two consecutive words).  The own
+// cell is dropped when it holds exactly one object and that object is the
+// particle itself (own_is_i: the grid build put it in this cell).
+__device__ __forceinline__ uint32_t neighbour_mask(const uint8_t* __restrict__ occ, const GridGeom& g, uint32_t cellId,
+                                                   const NbrRange& nr, int64_t cX, int64_t cXY, bool own_is_i)
+{
+    const uint32_t* occ32 = reinterpret_cast<const uint32_t*>(occ);
+    const int64_t wlo = -(kOccPad >> 2), whi = (int64_t)((g.cell_count + (uint64_t)kOccPad) >> 2) - 2;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        const int dy = c / 3 - 1, dz = c % 3 - 1;
+        if (dy < nr.yl || dy > nr.yh || dz < nr.zl || dz > nr.zh) continue;
+        const int64_t kb = (int64_t)cellId + dz * cXY + dy * cX;
+        const int64_t w = (kb - 1) >> 2;
+        if (w < wlo || w > whi) continue;  // every cell of the row is outside [0, cellCount)
+        const uint64_t v = (uint64_t)occ32[w] | ((uint64_t)occ32[w + 1] << 32);
+        const int sh = (int)((kb - 1) - 4 * w);  // 0..3
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int64_t nk = kb + dx;
+            if (dx < nr.xl || dx > nr.xh || nk < 0 || (uint64_t)nk >= g.cell_count) continue;
+            const uint32_t cnt = (uint32_t)(v >> (8 * (sh + 1 + dx))) & 0xFFu;
+            const bool skip_one = own_is_i && dx == 0 && dy == 0 && dz == 0;
+            if (cnt > (skip_one ? 1u : 0u)) mask |= 1u << ((dx + 1) * 9 + c);
+        }
+    }
+    return mask;
+}
+
 __device__ __forceinline__ f3 inv_dir(f3 dir)
 {
     return mk(dir.x != 0.0f ? 1.0f / dir.x : 0.0f, dir.y != 0.0f ? 1.0f / dir.y : 0.0f,
@@ -344,9 +383,9 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         newF = add(newF, env);
         F = sdiv(add(F0, newF), 2.0f);
 
-        // ---- particle-particle contacts over the 27 neighbour cells.  The
-        // cell table is read one x-slab (9 cells) at a time with independent
-        // loads; cells are then visited in the reference order (x, y, z).
+        // ---- particle-particle contacts over the 27 neighbour cells, visited
+        // in the reference order (x, then y, then z); only occupied cells
+        // (from the occupancy map) are looked up in the cell table.
         {
             const GridGeom& g = A.pg;
             int Tl = T;
@@ -357,39 +396,30 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             const int32_t xId = sat_i32((x1.x - g.minv[0]) / g.cell[0]);
             const int32_t yId = sat_i32((x1.y - g.minv[1]) / g.cell[1]);
             const int32_t zId = sat_i32((x1.z - g.minv[2]) / g.cell[2]);
-            int xl, xh, yl, yh, zl, zh;
-            axis_range_i(xId, g.cnt[0], xl, xh);
-            axis_range_i(yId, g.cnt[1], yl, yh);
-            axis_range_i(zId, g.cnt[2], zl, zh);
+            NbrRange nr;
+            axis_range_i(xId, g.cnt[0], nr.xl, nr.xh);
+            axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
+            axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
             const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (dx < xl || dx > xh) continue;
-                uint2 ent[9];
-#pragma unroll
-                for (int c = 0; c < 9; ++c) {
-                    const int dy = c / 3 - 1, dz = c % 3 - 1;
-                    const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
-                    const bool in = dy >= yl && dy <= yh && dz >= zl && dz <= zh && nk >= 0 &&
-                                    (uint64_t)nk < g.cell_count;
-                    ent[c] = in ? A.p_table[nk] : make_uint2(0u, 0u);
-                }
-#pragma unroll
-                for (int c = 0; c < 9; ++c) {
-                    const uint32_t u1 = min(ent[c].x + ent[c].y, (uint32_t)np);
-                    for (uint32_t u = ent[c].x; u < u1; ++u) {
-                        const float4 q = A.xs4[u];
-                        const int32_t j = __float_as_int(q.w);
-                        if (j == i) continue;
-                        const float r2 = A.bs[model_index(tt, Tl, j, fl)];
-                        const f3 rel = sub(x1, mk(q.x, q.y, q.z));
-                        const float d2 = dot(rel, rel);
-                        const float minD = r1 + r2;
-                        if (d2 <= minD * minD && d2 >= 0.0001f) {
-                            const float4 w = A.vs4[u];
-                            const f3 rv = sub(v1, mk(w.x, w.y, w.z));
-                            F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
-                        }
+            uint32_t mask = neighbour_mask(A.p_occ, g, cellId, nr, cX, cXY, A.p_key_of[i] == cellId);
+            while (mask) {
+                const int r = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                const int dx = r / 9 - 1, c = r % 9;
+                const int dy = c / 3 - 1, dz = c % 3 - 1;
+                const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
+                const uint2 ent = A.p_table[nk];
+                const uint32_t u1 = min(ent.x + ent.y, (uint32_t)np);
+                for (uint32_t u = ent.x; u < u1; ++u) {
+                    const int32_t j = A.p_perm[u];
+                    if (j == i) continue;
+                    const float r2 = A.bs[model_index(tt, Tl, j, fl)];
+                    const f3 rel = sub(x1, load3(A.pos, np, j));
+                    const float d2 = dot(rel, rel);
+                    const float minD = r1 + r2;
+                    if (d2 <= minD * minD && d2 >= 0.0001f) {
+                        const f3 rv = sub(v1, load3(A.vel, np, j));
+                        F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
                     }
                 }
             }
@@ -897,6 +927,7 @@ __global__ void __launch_bounds__(256) k_contacts(StepArgs A, int pass, int32_t*
             for (int dz = zl; dz <= zh; ++dz) {
                 const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
                 if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
+                if (A.p_occ[nk] == 0) continue;
                 const uint2 e = A.p_table[nk];
                 for (uint32_t u = e.x; u < min(e.x + e.y, (uint32_t)np); ++u) {
                     const int32_t j = A.p_perm[u];

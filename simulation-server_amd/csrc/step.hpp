@@ -60,9 +60,9 @@ struct StepArgs {
     const uint32_t* p_keys;
     const int32_t* p_perm;
     const int32_t* p_slot;
-    const uint2* p_table;   // {start, count} per cell
-    const float4* xs4;      // sorted (x, y, z, index bits)
-    const float4* vs4;      // sorted (vx, vy, vz, 0)
+    const uint2* p_table;   // {start, count} per cell (current where p_occ != 0)
+    const uint8_t* p_occ;   // occupancy map, kOccPad readable bytes on both sides
+    const uint32_t* p_key_of;  // per particle: its cell key in the last build
     // triangle grid + vein
     GridGeom tg;
     const uint2* t_table;

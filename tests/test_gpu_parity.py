@@ -69,6 +69,14 @@ def test_contact_sets_match_oracle(bcs, orc, kind):
     assert_bits_equal(gc, oc, "contact counts")
     assert_bits_equal(gp, opairs, "contact pairs (visit order)")
     assert gp.size > 0
+    # the forces of the same crowded state (contacts added in visit order)
+    o.calculate_grid(1)
+    sim.calculate_next_frame()
+    o.calculate_next_frame()
+    gpos, gv, gf = sim.get_particles()
+    opos, ov, of = o.get_particles()
+    assert_bits_equal(gf, of, "crowded forces")
+    assert_bits_equal(gv, ov, "crowded velocities")
 
 
 @pytest.mark.parametrize("kind", ["c0", "mixed"])
