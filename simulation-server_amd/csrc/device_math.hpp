@@ -77,6 +77,28 @@ __device__ __forceinline__ float uniform01(uint64_t seed, uint64_t step, uint64_
     return (float)((h >> 40) + 1ull) * 0x1p-24f;
 }
 
+// Unsigned 32-bit division by a run-time constant d (Granlund & Montgomery,
+// PLDI 1994: round-up multiplier with its 33rd bit folded into an add):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(m, n),
+// n / d = (t + ((n - t) >> 1)) >> (l - 1); s == 255 marks d == 1.
+struct Div32 {
+    uint32_t m, s;
+};
+__host__ __device__ inline Div32 make_div32(uint32_t d)
+{
+    if (d <= 1u) return Div32{0u, 255u};
+    uint32_t l = 0;
+    while (l < 32u && (1ull << l) < d) ++l;
+    const uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1ull;
+    return Div32{(uint32_t)m, l - 1u};
+}
+__device__ __forceinline__ uint32_t udiv(uint32_t n, Div32 d)
+{
+    if (d.s == 255u) return n;
+    const uint32_t t = __umulhi(d.m, n);
+    return (t + ((n - t) >> 1)) >> d.s;
+}
+
 // Uniform grid geometry (src/grids/uniform_grid.cuh:15-65, bounds from
 // src/meta_factory/vein_factory.hpp:46-86).
 struct GridGeom {
@@ -87,7 +109,17 @@ struct GridGeom {
     int32_t cnt[3];
     uint32_t pad;
     uint64_t cell_count;
+    Div32 div_x, div_y;  // division by cnt[0] and cnt[1] (key -> cell coordinates)
 };
+
+// Cell coordinates of a key in [0, cellCount) (key = x + cnt0 * (y + cnt1 * z)).
+__device__ __forceinline__ void grid_coords(const GridGeom& g, uint32_t key, uint32_t& x, uint32_t& y, uint32_t& z)
+{
+    const uint32_t q = udiv(key, g.div_x);
+    x = key - q * (uint32_t)g.cnt[0];
+    z = udiv(q, g.div_y);
+    y = q - z * (uint32_t)g.cnt[1];
+}
 
 // calculateIdForCell, src/grids/uniform_grid.cu:24-36 (max/min macros of
 // :20-21, upper clamp in world units).  Keys outside [0, cellCount) are
@@ -111,6 +143,38 @@ __device__ __forceinline__ uint32_t cell_key(const GridGeom& g, float x, float y
         key = (int64_t)g.cell_count - 1;
     }
     return (uint32_t)key;
+}
+
+// Occupancy map of a grid in bricks of 8 x 8 x 8 cells: 2 bits per cell
+// (0 empty, 1 one object, 2 more), one 16-bit word per 8-cell x-row of a
+// brick and 64 words -- one 128-byte line -- per brick, so the 3 x 3 x 3
+// neighbourhood of a cell lies in one to eight lines (two on average).
+struct BrickMap {
+    uint16_t* words;
+    int32_t cnt[3];  // grid cells per axis
+    uint32_t nbx, nby;
+    Div32 div_x, div_y;
+};
+constexpr uint32_t kNoWord = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t brick_word(const BrickMap& b, uint32_t x, uint32_t y, uint32_t z)
+{
+    const uint32_t br = (x >> 3) + b.nbx * ((y >> 3) + b.nby * (z >> 3));
+    return br * 64u + (z & 7u) * 8u + (y & 7u);
+}
+// Cell coordinates of a key in [0, cellCount).
+__device__ __forceinline__ void key_coords(const BrickMap& b, uint32_t key, uint32_t& x, uint32_t& y, uint32_t& z)
+{
+    const uint32_t q = udiv(key, b.div_x);
+    x = key - q * (uint32_t)b.cnt[0];
+    z = udiv(q, b.div_y);
+    y = q - z * (uint32_t)b.cnt[1];
+}
+__device__ __forceinline__ uint32_t cell_code(const BrickMap& b, uint32_t key)
+{
+    uint32_t x, y, z;
+    key_coords(b, key, x, y, z);
+    return ((uint32_t)b.words[brick_word(b, x, y, z)] >> (2u * (x & 7u))) & 3u;
 }
 
 // Face-clipped neighbour offsets of one axis

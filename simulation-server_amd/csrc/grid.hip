@@ -13,12 +13,13 @@
 //              exactly thrust's stable order (ties by object index); run starts
 //              write the dense cell-start table
 //   k_big      the same for buckets larger than the k_rank threshold (LDS tiles)
+//   k_occ      (optional) the 2-bit brick occupancy map from the sorted keys:
+//              the first slot of every 8-cell x-group writes the group's word
 // Cell table: dense {start, count} per cell id (8 bytes, one load per lookup),
-// written for occupied cells only, plus an occupancy byte map (saturating
-// object count per cell, 0 = empty).  k_key first zeroes the map bytes of the
-// previous build's keys, so the map is always current.  The table entries of
-// cells that emptied are cleared only when clear_table is set (the triangle
-// grid); readers of the particle grid consult the map first.
+// written for occupied cells only.  With clear_table (the triangle grid) k_key
+// first zeroes the entries of the previous build's keys, so every entry is
+// current or {0, 0}; without it (the particle grid) k_key zeroes the map
+// words the previous build wrote and readers consult the map first.
 #include "grid.hpp"
 
 #include <algorithm>
@@ -36,8 +37,8 @@ constexpr uint64_t kSpinLimit = 1ull << 22;
 
 __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict__ pos, int64_t stride, int32_t n,
                                              int32_t n_prev, int shift, uint32_t* __restrict__ keys,
-                                             uint2* __restrict__ table, uint8_t* __restrict__ occ, int clear_table,
-                                             uint32_t* __restrict__ bcount,
+                                             uint2* __restrict__ table, int clear_table, BrickMap bm,
+                                             const uint32_t* __restrict__ gpos, uint32_t* __restrict__ bcount,
                                              uint64_t* __restrict__ status, int32_t ntiles,
                                              uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags)
 {
@@ -50,10 +51,13 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
         ctrl[2] = 0u;  // occupied cells (run list)
     }
     if (gid < n_prev) {  // clear the previous build's cell of this object
-        const uint32_t old = keys[gid];
-        if ((uint64_t)old < g.cell_count) {
-            occ[old] = 0;
-            if (clear_table) table[old] = make_uint2(0u, 0u);
+        if (clear_table) {
+            const uint32_t old = keys[gid];
+            if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
+        }
+        if (gpos) {  // map words written by the previous build (indexed by its sorted slots)
+            const uint32_t w = gpos[gid];
+            if (w != kNoWord) bm.words[w] = 0;
         }
     }
     uint32_t fl = 0;
@@ -176,7 +180,6 @@ struct RankOut {
     int32_t* perm;
     int32_t* slot_of;      // nullable
     uint2* table;          // {start, count} per cell
-    uint8_t* occ;          // saturating count per cell
     uint32_t* runs;        // run starts of occupied cells, in no particular order (nullable)
     uint32_t* nruns;
 };
@@ -189,7 +192,6 @@ __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32
     if (o.slot_of) o.slot_of[idx] = (int32_t)pos;
     if (first) {
         o.table[key] = make_uint2(pos, count);
-        o.occ[key] = (uint8_t)min(count, 255u);
         if (o.runs) o.runs[atomicAdd(o.nruns, 1u)] = pos;
     }
 }
@@ -271,6 +273,41 @@ __global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_
     }
 }
 
+// Brick occupancy map: the first sorted slot of each 8-cell x-group (cells
+// x&~7 .. x|7 of one row) builds the group's 2-bit codes from the sorted keys
+// and writes its word; every slot records the word it wrote (or kNoWord) so
+// that the next build can clear exactly these words.
+__global__ void __launch_bounds__(256) k_occ(const uint32_t* __restrict__ keys_sorted, int32_t n, int32_t cap,
+                                             const uint2* __restrict__ table, BrickMap bm,
+                                             uint32_t* __restrict__ gpos)
+{
+    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= cap) return;
+    uint32_t w = kNoWord;
+    if (u < n) {
+        const uint32_t key = keys_sorted[u];
+        uint32_t x, y, z;
+        key_coords(bm, key, x, y, z);
+        const uint32_t gs = key - (x & 7u);
+        if (u == 0 || keys_sorted[u - 1] < gs) {
+            const uint32_t ge = gs + min(8u, (uint32_t)bm.cnt[0] - (x & ~7u));
+            uint32_t code = 0;
+            int32_t v = u;
+            while (v < n) {
+                const uint32_t k2 = keys_sorted[v];
+                if (k2 >= ge) break;
+                // run length from the next sorted key; the table only for runs of 2+
+                const bool multi = v + 1 < n && keys_sorted[v + 1] == k2;
+                code |= (multi ? 2u : 1u) << (2u * (k2 - gs));
+                v += multi ? (int32_t)max(table[k2].y, 2u) : 1;
+            }
+            w = brick_word(bm, x, y, z);
+            bm.words[w] = (uint16_t)code;
+        }
+    }
+    gpos[u] = w;
+}
+
 // ------------------------------------------------------------------ host --
 
 static int choose_shift(uint64_t cell_count, int64_t cap)
@@ -288,7 +325,8 @@ static int choose_shift(uint64_t cell_count, int64_t cap)
         if (e_ != hipSuccess) return fail(BCS_ERR_HIP, std::string("grid: ") + hipGetErrorString(e_)); \
     } while (0)
 
-int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_runs, hipStream_t st)
+int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_runs, bool with_map,
+               hipStream_t st)
 {
     G = DeviceGrid{};
     G.g = geom;
@@ -312,11 +350,23 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     if (e != hipSuccess)
         return fail(BCS_ERR_OUT_OF_MEMORY, "grid: cell table of " + std::to_string(geom.cell_count * 8) + " bytes");
     GRID_HIP(hipMemsetAsync(G.table, 0, geom.cell_count * 8, st));
-    const size_t occ_bytes = 2 * kOccPad + ((geom.cell_count + 3) & ~3ull);
-    e = hipMalloc(&G.occ_alloc, occ_bytes);
-    if (e != hipSuccess) return fail(BCS_ERR_OUT_OF_MEMORY, "grid: occupancy map of " + std::to_string(occ_bytes) + " bytes");
-    GRID_HIP(hipMemsetAsync(G.occ_alloc, 0, occ_bytes, st));
-    G.occ = G.occ_alloc + kOccPad;
+    if (with_map) {
+        const uint64_t nbx = ((uint64_t)geom.cnt[0] + 7) / 8, nby = ((uint64_t)geom.cnt[1] + 7) / 8,
+                       nbz = ((uint64_t)geom.cnt[2] + 7) / 8;
+        const uint64_t words = nbx * nby * nbz * 64;
+        if (words >= kNoWord) return fail(BCS_ERR_GRID_TOO_LARGE, "grid: occupancy map exceeds 2^32 words");
+        e = hipMalloc(&G.bm.words, words * 2);
+        if (e != hipSuccess)
+            return fail(BCS_ERR_OUT_OF_MEMORY, "grid: occupancy map of " + std::to_string(words * 2) + " bytes");
+        GRID_HIP(hipMemsetAsync(G.bm.words, 0, words * 2, st));
+        for (int d = 0; d < 3; ++d) G.bm.cnt[d] = geom.cnt[d];
+        G.bm.div_x = geom.div_x;
+        G.bm.div_y = geom.div_y;
+        G.bm.nbx = (uint32_t)nbx;
+        G.bm.nby = (uint32_t)nby;
+        GRID_HIP(hipMalloc(&G.gpos, n * 4));
+        GRID_HIP(hipMemsetAsync(G.gpos, 0xFF, n * 4, st));
+    }
     GRID_HIP(hipMalloc(&G.bcount, (G.nb + 1) * 4));
     GRID_HIP(hipMemsetAsync(G.bcount, 0, (G.nb + 1) * 4, st));
     GRID_HIP(hipMalloc(&G.bstart, (G.nb + 2) * 4));
@@ -331,7 +381,7 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
 void grid_free(DeviceGrid& G)
 {
     void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.runs, G.perm, G.slot_of,
-                    G.table, G.occ_alloc, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
+                    G.table, G.bm.words, G.gpos, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     G = DeviceGrid{};
@@ -351,8 +401,8 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint3
     auto te = [&](int id) { if (timer) timer->end(id, st); };
     tb(0);
     hipLaunchKernelGGL(k_key, dim3(std::max(blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride, n,
-                       n_prev, G.shift, G.keys, G.table, G.occ, G.clear_table ? 1 : 0, G.bcount, G.status, G.ntiles,
-                       G.ctrl, flags);
+                       n_prev, G.shift, G.keys, G.table, G.clear_table ? 1 : 0, G.bm, G.gpos, G.bcount, G.status,
+                       G.ntiles, G.ctrl, flags);
     te(0);
     tb(1);
     hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,
@@ -362,7 +412,7 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint3
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.bcount, G.items_k,
                        G.items_i, flags);
     te(2);
-    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.occ, G.runs, G.ctrl + 2};
+    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2};
     tb(3);
     hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, o);
     te(3);
@@ -370,6 +420,11 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint3
     hipLaunchKernelGGL(k_big, dim3(64), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, G.biglist, G.ctrl,
                        o);
     te(4);
+    tb(5);
+    if (G.gpos)
+        hipLaunchKernelGGL(k_occ, dim3((std::max(G.cap, 1) + 255) / 256), dim3(256), 0, st, G.keys_sorted, n, G.cap,
+                           G.table, G.bm, G.gpos);
+    te(5);
     GRID_HIP(hipGetLastError());
     return BCS_OK;
 }

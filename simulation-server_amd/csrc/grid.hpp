@@ -20,10 +20,6 @@ struct KernelTimer {
     void end(int id, hipStream_t st) const { (void)hipEventRecord((*ev_end)[offset + id], st); }
 };
 
-// Bytes of padding on each side of the occupancy map, so that word loads of a
-// neighbourhood row never leave the allocation.
-constexpr int64_t kOccPad = 64;
-
 struct DeviceGrid {
     GridGeom g{};
     int32_t cap = 0;            // object capacity
@@ -38,10 +34,10 @@ struct DeviceGrid {
     uint32_t* keys_sorted = nullptr;  // gridCellIds after the sort
     int32_t* perm = nullptr;          // particleIds after the sort
     int32_t* slot_of = nullptr;       // inverse of perm (optional)
-    uint2* table = nullptr;           // dense {start, count} per cell id (current where occ != 0)
-    uint8_t* occ = nullptr;           // occupancy map: saturating count per cell id (0 = empty)
-    uint8_t* occ_alloc = nullptr;     // occ - kOccPad
-    bool clear_table = true;          // also zero the entries of cells that emptied
+    uint2* table = nullptr;           // dense {start, count} per cell id (current where occupied)
+    bool clear_table = true;          // zero the entries of cells that emptied (else: consult bm)
+    BrickMap bm{};                    // occupancy map (optional; words == nullptr when absent)
+    uint32_t* gpos = nullptr;         // per sorted slot: map word written by the last build, or kNoWord
     uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds
     uint32_t* bstart = nullptr;       // bucket offsets [nb+2]
     uint64_t* status = nullptr;       // scan tile status
@@ -52,12 +48,13 @@ struct DeviceGrid {
 
 // Initialisation is enqueued on `st` (the owner's stream, which does not
 // synchronise with the legacy NULL stream).
-int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_runs, hipStream_t st);
+int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_runs, bool with_map,
+               hipStream_t st);
 void grid_free(DeviceGrid& G);
 // pos: SoA base pointer (x, y, z at +0, +stride, +2*stride).
 int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint32_t* flags, hipStream_t st,
                const KernelTimer* timer = nullptr);
-constexpr int kGridKernels = 5;
+constexpr int kGridKernels = 6;
 
 int scan_exclusive(const uint32_t* count, uint32_t* out, int64_t m, uint64_t* status, int32_t ntiles, uint32_t* ctrl,
                    uint32_t* flags, hipStream_t st);

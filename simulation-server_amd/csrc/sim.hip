@@ -31,6 +31,7 @@ void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
 void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
                     float4* aabb, hipStream_t st);
+void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st);
 int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
                  uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
                  uint32_t* flags, hipStream_t st);
@@ -72,12 +73,14 @@ struct bcs_sim {
     float4 *hits = nullptr, *seg = nullptr;
     uint64_t* vstatus = nullptr;
     int32_t vntiles = 0;
-    float4* t_aabb = nullptr;  // triangle-cell boxes at run starts (per frame)
+    float4* t_aabb = nullptr;  // triangle-cell boxes by cell key (per frame)
+    uint32_t* t_nmask = nullptr;  // per triangle cell: occupied neighbourhood mask
     float4* tdata = nullptr;   // per sorted triangle: (v0 | id), e1, e2 (per frame)
     uint32_t* best = nullptr;  // per particle first-hit key
     uint4* work = nullptr;     // ray work items
-    uint32_t* wctrl = nullptr;
-    uint32_t work_cap = 0;
+    uint32_t* wcount = nullptr;  // per k_cells_a block: work items
+    uint32_t region_cap = 0;     // work items per block region
+    int nblocks = 0;             // k_cells_a grid (8 * ceil(nchunks / 8))
     // grids
     DeviceGrid pg, tg;
     bool tg_built = false;
@@ -118,12 +121,12 @@ struct bcs_sim {
         A.bs = bs; A.r0 = r0; A.model = model; A.sp_off = sp_off; A.sp_b = sp_b; A.sp_L = sp_L;
         A.chunk_type = chunk_type; A.chunk_first = chunk_first; A.chunk_n = chunk_n;
         A.corder = corder; A.nchunks = nchunks;
-        A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table; A.p_occ = pg.occ; A.p_key_of = pg.keys;
+        A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table; A.p_map = pg.bm; A.p_key_of = pg.keys;
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.vcount = vcount; A.hits = hits; A.hctrl = hctrl; A.vstatus = vstatus; A.vntiles = vntiles;
-        A.t_aabb = t_aabb;
-        A.best = best; A.work = work; A.wctrl = wctrl; A.work_cap = work_cap;
+        A.t_aabb = t_aabb; A.t_nmask = t_nmask;
+        A.best = best; A.work = work; A.wcount = wcount; A.region_cap = region_cap;
         A.flags = flags; A.step = step;
         return A;
     }
@@ -152,6 +155,8 @@ GridGeom make_geom(const float bmin[3], const float bmax[3], const float cell[3]
         g.cnt[d] = (q >= 2147483647.0f) ? 2147483647 : (q > 0 ? (int32_t)q : 0);
     }
     g.cell_count = (uint64_t)g.cnt[0] * (uint64_t)g.cnt[1] * (uint64_t)g.cnt[2];
+    g.div_x = make_div32((uint32_t)std::max(g.cnt[0], 1));
+    g.div_y = make_div32((uint32_t)std::max(g.cnt[1], 1));
     return g;
 }
 
@@ -176,6 +181,7 @@ int build_triangle_grid(bcs_sim* s)
     const bool dyn = s->phys.dynamic_triangle_grid != 0;
     if (dyn || !s->tg_built) launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
     RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, s->flags, s->stream));
+    launch_tri_nmask(s->tg, s->t_nmask, s->stream);
     s->tg_built = true;
     return BCS_OK;
 }
@@ -230,7 +236,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
 }
 
 const char* kKernelNames =
-    "k_key;k_scan;k_scatter;k_rank;k_big;k_vein_springs+k_tri_box;k_cells_a;k_ray;k_cells_b;k_scan(vein);"
+    "k_key;k_scan;k_scatter;k_rank;k_big;k_occ;k_vein_springs+k_tri_box;k_cells_a;k_ray;k_cells_b;k_scan(vein);"
     "k_vein_scatter;k_vein";
 constexpr int kFusedKernels = kGridKernels + 7;
 
@@ -402,9 +408,17 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     s->vntiles = scan_tiles(s->nv);
     RC(s->dalloc(&s->vstatus, (size_t)s->vntiles));
     RC(s->dalloc(&s->flags, 4)); RC(s->dalloc(&s->step, 1));
-    RC(s->dalloc(&s->t_aabb, 2 * NT)); RC(s->dalloc(&s->tdata, 3 * NT));
-    s->work_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * N, 65536), 0x7FFFFFFFull);
-    RC(s->dalloc(&s->best, N)); RC(s->dalloc(&s->work, s->work_cap)); RC(s->dalloc(&s->wctrl, 4));
+    RC(s->dalloc(&s->tdata, 3 * NT));
+    // ray work list: 8 items per particle, split evenly over the k_cells_a blocks
+    s->nblocks = 8 * ((s->nchunks + 7) / 8);
+    {
+        const uint64_t total = std::max<uint64_t>(8 * N, 65536);
+        const uint64_t per = std::max<uint64_t>(total / std::max(s->nblocks, 1), 1024);
+        s->region_cap = (uint32_t)std::min<uint64_t>(per, 0x7FFFFFFFull);
+    }
+    RC(s->dalloc(&s->best, N));
+    RC(s->dalloc(&s->work, (size_t)s->region_cap * std::max(s->nblocks, 1)));
+    RC(s->dalloc(&s->wcount, std::max(s->nblocks, 1)));
     // ---- uploads
     HIPCHK(hipMemsetAsync(s->pos, 0, 12 * std::max<size_t>(N, 1), s->stream));
     HIPCHK(hipMemsetAsync(s->vel, 0, 12 * std::max<size_t>(N, 1), s->stream));
@@ -449,13 +463,15 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     HIPCHK(hipMemsetAsync(s->hctrl, 0, 64, s->stream));
     HIPCHK(hipMemsetAsync(s->flags, 0, 4, s->stream));
     HIPCHK(hipMemsetAsync(s->step, 0, 8, s->stream));
-    HIPCHK(hipMemsetAsync(s->wctrl, 0, 16, s->stream));
+    HIPCHK(hipMemsetAsync(s->wcount, 0, 4 * (size_t)std::max(s->nblocks, 1), s->stream));
     // ---- grids
     const GridGeom pgeom = make_geom(bmin, bmax, phys->particle_cell);
     const GridGeom tgeom = make_geom(bmin, bmax, phys->triangle_cell);
-    RC(grid_alloc(s->pg, pgeom, s->np, true, false, s->stream));
+    RC(grid_alloc(s->pg, pgeom, s->np, true, false, true, s->stream));
     s->pg.clear_table = false;  // readers consult the occupancy map
-    RC(grid_alloc(s->tg, tgeom, s->ntri, false, true, s->stream));
+    RC(grid_alloc(s->tg, tgeom, s->ntri, false, true, false, s->stream));
+    RC(s->dalloc(&s->t_nmask, (size_t)tgeom.cell_count));
+    RC(s->dalloc(&s->t_aabb, 2 * (size_t)tgeom.cell_count));
     RC(build_triangle_grid(s.get()));
     HIPCHK(hipStreamSynchronize(s->stream));
     *out = s.release();
@@ -815,11 +831,14 @@ extern "C" int bcs_sim_get_contacts(bcs_sim* s, int32_t* counts, int32_t* pairs_
 extern "C" int bcs_sim_frame_stats(bcs_sim* s, uint32_t* ray_work_items, uint32_t* vein_hits)
 {
     RC(set_dev(s));
-    uint32_t w[2] = {0, 0}, h[8] = {0};
-    HIPCHK(hipMemcpyAsync(w, s->wctrl, 8, hipMemcpyDeviceToHost, s->stream));
+    std::vector<uint32_t> w((size_t)std::max(s->nblocks, 1), 0u);
+    uint32_t h[8] = {0};
+    HIPCHK(hipMemcpyAsync(w.data(), s->wcount, 4 * w.size(), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(h, s->hctrl, 32, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    if (ray_work_items) *ray_work_items = w[1];
+    uint64_t items = 0;
+    for (uint32_t v : w) items += v;
+    if (ray_work_items) *ray_work_items = (uint32_t)std::min<uint64_t>(items, 0xFFFFFFFFull);
     if (vein_hits) *vein_hits = h[4];
     return BCS_OK;
 }

@@ -177,11 +177,13 @@ constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kScanSerial = 0xFFFFFFFEu;  // work list full: k_cells_b scans serially
 constexpr uint32_t kSlotBits = 27;             // best = rank << 27 | sorted triangle slot
 constexpr uint32_t kItemTris = 16;
+constexpr uint32_t kRaySplit = 4;  // k_ray blocks per work region
 
 // Triangle-grid neighbourhood of a particle (face clipped, unsigned ids as
 // in src/simulation/vein_collisions.cu:80-232).
 struct RayCells {
     uint32_t cellId;
+    uint32_t xId, yId, zId;  // unclamped (saturated) cell coordinates
     int xl, xh, yl, yh, zl, zh;
     int64_t cX, cXY;
 };
@@ -191,15 +193,38 @@ __device__ __forceinline__ RayCells ray_cells(const GridGeom& g, f3 x1)
     RayCells rc;
     uint32_t dummy = 0;
     rc.cellId = cell_key(g, x1.x, x1.y, x1.z, dummy);
-    const uint32_t xId = sat_u32((x1.x - g.minv[0]) / g.cell[0]);
-    const uint32_t yId = sat_u32((x1.y - g.minv[1]) / g.cell[1]);
-    const uint32_t zId = sat_u32((x1.z - g.minv[2]) / g.cell[2]);
-    axis_range_u(xId, g.cnt[0], rc.xl, rc.xh);
-    axis_range_u(yId, g.cnt[1], rc.yl, rc.yh);
-    axis_range_u(zId, g.cnt[2], rc.zl, rc.zh);
+    rc.xId = sat_u32((x1.x - g.minv[0]) / g.cell[0]);
+    rc.yId = sat_u32((x1.y - g.minv[1]) / g.cell[1]);
+    rc.zId = sat_u32((x1.z - g.minv[2]) / g.cell[2]);
+    axis_range_u(rc.xId, g.cnt[0], rc.xl, rc.xh);
+    axis_range_u(rc.yId, g.cnt[1], rc.yl, rc.yh);
+    axis_range_u(rc.zId, g.cnt[2], rc.zl, rc.zh);
     rc.cX = g.cnt[0];
     rc.cXY = (int64_t)g.cnt[0] * g.cnt[1];
     return rc;
+}
+
+// Occupied triangle cells of the neighbourhood as a mask over the visit rank
+// r = (dx+1)*9 + (dy+1)*3 + (dz+1).  When the particle's cell coordinates are
+// those of its (clamped) key and every axis has at least two cells, the
+// neighbourhood is the cell's own and the mask precomputed per cell
+// (k_tri_nmask) is returned; otherwise it is formed key by key.
+__device__ __forceinline__ uint32_t tri_occupied(const StepArgs& A, const RayCells& rc)
+{
+    const GridGeom& g = A.tg;
+    uint32_t xc, yc, zc;
+    grid_coords(g, rc.cellId, xc, yc, zc);
+    if (xc == rc.xId && yc == rc.yId && zc == rc.zId && g.cnt[0] >= 2 && g.cnt[1] >= 2 && g.cnt[2] >= 2)
+        return A.t_nmask[rc.cellId];
+    uint32_t m = 0;
+    for (int dx = rc.xl; dx <= rc.xh; ++dx)
+        for (int dy = rc.yl; dy <= rc.yh; ++dy)
+            for (int dz = rc.zl; dz <= rc.zh; ++dz) {
+                const int64_t nk = (int64_t)rc.cellId + dz * rc.cXY + dy * rc.cX + dx;
+                if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
+                if (A.t_table[nk].y != 0u) m |= 1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
+            }
+    return m;
 }
 
 // Table entries of the 9 cells of x-slab dx (independent loads) and the mask
@@ -220,7 +245,8 @@ __device__ __forceinline__ uint32_t slab_cells(const StepArgs& A, const RayCells
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
         if (ent[c].y != 0u && ent[c].x < (uint32_t)A.ntri) {
-            const float4 lo = A.t_aabb[2 * ent[c].x], hi = A.t_aabb[2 * ent[c].x + 1];
+            const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
+            const float4 lo = A.t_aabb[2 * nk], hi = A.t_aabb[2 * nk + 1];
             if (ray_box(x1, inv, dir, lo, hi, bm)) mask |= 1u << c;
         }
     }
@@ -234,34 +260,49 @@ struct NbrRange {
 
 // Occupied cells of the 27-neighbourhood of cell `cellId` as a mask over the
 // visit rank r = (dx+1)*9 + (dy+1)*3 + (dz+1) (the reference's x, y, z loop
-// order), restricted to the clipped ranges and to keys in [0, cellCount).
-// Each of the nine (dy, dz) rows is one pair of word loads from the
-// occupancy map (its three x-cells lie in two consecutive words).  The own
-// cell is dropped when it holds exactly one object and that object is the
-// particle itself (own_is_i: the grid build put it in this cell).
-__device__ __forceinline__ uint32_t neighbour_mask(const uint8_t* __restrict__ occ, const GridGeom& g, uint32_t cellId,
+// order), restricted to the clipped ranges and to keys in [0, cellCount).  The
+// reference forms neighbour keys as cellId + dz*cX*cY + dy*cX + dx; when every
+// allowed offset stays inside the grid on each axis (the usual case) that key
+// is the cell (x+dx, y+dy, z+dz) and the brick map is read by coordinates,
+// else every key is decoded on its own.  The own cell is dropped when it holds
+// exactly one object and that object is the particle itself (own_is_i).
+__device__ __forceinline__ uint32_t neighbour_mask(const BrickMap& bm, const GridGeom& g, uint32_t cellId,
                                                    const NbrRange& nr, int64_t cX, int64_t cXY, bool own_is_i)
 {
-    const uint32_t* occ32 = reinterpret_cast<const uint32_t*>(occ);
-    const int64_t wlo = -(kOccPad >> 2), whi = (int64_t)((g.cell_count + (uint64_t)kOccPad) >> 2) - 2;
+    uint32_t x, y, z;
+    key_coords(bm, cellId, x, y, z);
     uint32_t mask = 0;
+    const bool inside = (int64_t)x + nr.xl >= 0 && (int64_t)x + nr.xh < bm.cnt[0] && (int64_t)y + nr.yl >= 0 &&
+                        (int64_t)y + nr.yh < bm.cnt[1] && (int64_t)z + nr.zl >= 0 && (int64_t)z + nr.zh < bm.cnt[2];
+    if (inside) {
+        const uint32_t xa = x + nr.xl, xb = x + nr.xh;
 #pragma unroll
-    for (int c = 0; c < 9; ++c) {
-        const int dy = c / 3 - 1, dz = c % 3 - 1;
-        if (dy < nr.yl || dy > nr.yh || dz < nr.zl || dz > nr.zh) continue;
-        const int64_t kb = (int64_t)cellId + dz * cXY + dy * cX;
-        const int64_t w = (kb - 1) >> 2;
-        if (w < wlo || w > whi) continue;  // every cell of the row is outside [0, cellCount)
-        const uint64_t v = (uint64_t)occ32[w] | ((uint64_t)occ32[w + 1] << 32);
-        const int sh = (int)((kb - 1) - 4 * w);  // 0..3
+        for (int c = 0; c < 9; ++c) {
+            const int dy = c / 3 - 1, dz = c % 3 - 1;
+            if (dy < nr.yl || dy > nr.yh || dz < nr.zl || dz > nr.zh) continue;
+            const uint32_t yy = y + dy, zz = z + dz;
+            const uint32_t wa = bm.words[brick_word(bm, xa, yy, zz)];
+            const uint32_t wb = ((xa ^ xb) >> 3) ? (uint32_t)bm.words[brick_word(bm, xb, yy, zz)] : wa;
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int64_t nk = kb + dx;
-            if (dx < nr.xl || dx > nr.xh || nk < 0 || (uint64_t)nk >= g.cell_count) continue;
-            const uint32_t cnt = (uint32_t)(v >> (8 * (sh + 1 + dx))) & 0xFFu;
-            const bool skip_one = own_is_i && dx == 0 && dy == 0 && dz == 0;
-            if (cnt > (skip_one ? 1u : 0u)) mask |= 1u << ((dx + 1) * 9 + c);
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx < nr.xl || dx > nr.xh) continue;
+                const uint32_t xx = x + dx;
+                const uint32_t w = ((xx ^ xa) >> 3) ? wb : wa;
+                const uint32_t code = (w >> (2u * (xx & 7u))) & 3u;
+                const bool skip_one = own_is_i && dx == 0 && dy == 0 && dz == 0;
+                if (code > (skip_one ? 1u : 0u)) mask |= 1u << ((dx + 1) * 9 + c);
+            }
         }
+    } else {
+        for (int dx = nr.xl; dx <= nr.xh; ++dx)
+            for (int dy = nr.yl; dy <= nr.yh; ++dy)
+                for (int dz = nr.zl; dz <= nr.zh; ++dz) {
+                    const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
+                    if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
+                    const uint32_t code = cell_code(bm, (uint32_t)nk);
+                    const bool skip_one = own_is_i && dx == 0 && dy == 0 && dz == 0;
+                    if (code > (skip_one ? 1u : 0u)) mask |= 1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
+                }
     }
     return mask;
 }
@@ -302,14 +343,14 @@ __device__ bool ray_first_hit_serial(const StepArgs& A, f3 x1, f3 dir, float& rt
     return false;
 }
 
-constexpr int kStage = 4;  // passing triangle cells staged per particle
 
 __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
 {
-    __shared__ uint4 s_stage[kBlock * kStage];  // (start, count, rank) of passing cells
     __shared__ float sx[kBlock], sy[kBlock], sz[kBlock];
     __shared__ float svx[kBlock], svy[kBlock], svz[kBlock];
     __shared__ float sfx[kBlock], sfy[kBlock], sfz[kBlock];
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    __shared__ uint32_t s_end;
 
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
@@ -321,7 +362,10 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         if (gid == 0) A.hctrl[1] = 0u;
     }
     const int chunk = chunk_of_block(A.nchunks);
-    if (chunk < 0) return;
+    if (chunk < 0) {
+        if (l == 0) A.wcount[blockIdx.x] = 0u;
+        return;
+    }
     const int T = A.chunk_type[chunk];
     const int first = A.chunk_first[chunk];
     const int ncell = A.chunk_n[chunk];
@@ -342,10 +386,9 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
 
     uint32_t fl = 0;
     f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
-    uint32_t nitems = 0, npass = 0;
-    f3 dir = x1, inv = x1;
+    uint32_t nitems = 0, pmask = 0;
+    f3 dir = x1;
     RayCells rc{};
-    float bm = 0.0f;
     if (act) {
         x1 = mk(sx[l], sy[l], sz[l]);
         v1 = mk(svx[l], svy[l], svz[l]);
@@ -370,9 +413,21 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             f3 dP = sub(x1, mk(sx[nb], sy[nb], sz[nb]));
             f3 dv = sub(v1, mk(svx[nb], svy[nb], svz[nb]));
             f3 f2 = mk(sfx[nb], sfy[nb], sfz[nb]);
-            f3 shift = normalize(smul(-1.0f, dP), fl);
+            // normalize(-dP) == -normalize(dP) and len(dP) is the same
+            // sqrtf(dot(dP, dP)) bit for bit (negation commutes with every
+            // rounding here), so one square root and three divisions serve all
+            // three; a NaN quotient is replaced by (0,0,0) for both, as in
+            // normalize().
+            const float sl = sqrtf(dot(dP, dP));
+            f3 nrm = sdiv(dP, sl);
+            f3 shift = smul(-1.0f, nrm);
+            if (__builtin_isnan(nrm.x) || __builtin_isnan(nrm.y) || __builtin_isnan(nrm.z)) {
+                fl |= BCS_FLAG_NAN_NORMALIZE;
+                nrm = mk(0.f, 0.f, 0.f);
+                shift = nrm;
+            }
             f3 dvv = add(dv, smul(pc.dt, sub(F0, f2)));
-            float mag = (len(dP) - L) * pc.k + dot(normalize(dP, fl), dvv) * pc.d;
+            float mag = (sl - L) * pc.k + dot(nrm, dvv) * pc.d;
             newF = add(newF, smul(mag, shift));
         }
         const float ratio = len(sub(x1, cen)) / A.r0[ma];
@@ -401,7 +456,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
             axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
             const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-            uint32_t mask = neighbour_mask(A.p_occ, g, cellId, nr, cX, cXY, A.p_key_of[i] == cellId);
+            uint32_t mask = neighbour_mask(A.p_map, g, cellId, nr, cX, cXY, A.p_key_of[i] == cellId);
             while (mask) {
                 const int r = __builtin_ctz(mask);
                 mask &= mask - 1u;
@@ -425,66 +480,93 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             }
         }
 
-        // ---- vein ray candidates: count work items (cells the ray can reach,
-        // split into runs of <= 16 sorted triangles)
+        // ---- vein ray candidates: occupied triangle cells whose box the ray
+        // can reach (pmask), split into work items of <= 16 sorted triangles
         dir = normalize(v1, fl);
         if (dir.x != 0.0f || dir.y != 0.0f || dir.z != 0.0f) {
             rc = ray_cells(A.tg, x1);
-            inv = inv_dir(dir);
-            bm = box_margin(A);
+            const uint32_t om = tri_occupied(A, rc);
+            if (om) {
+                const f3 inv = inv_dir(dir);
+                const float bm = box_margin(A);
+                // boxes of one x-slab's occupied cells loaded together, then tested
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                uint2 ent[9];
-                const uint32_t mask = slab_cells(A, rc, dx, x1, inv, dir, bm, ent);
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint32_t sm = (om >> ((dx + 1) * 9)) & 0x1FFu;
+                    if (!sm) continue;
+                    float4 lo[9], hi[9];
 #pragma unroll
-                for (int c = 0; c < 9; ++c) {
-                    if (!((mask >> c) & 1u)) continue;
-                    nitems += (ent[c].y + kItemTris - 1) / kItemTris;
-                    if (npass < kStage)
-                        s_stage[l * kStage + npass] = make_uint4(ent[c].x, ent[c].y, (uint32_t)((dx + 1) * 9 + c), 0u);
-                    ++npass;
+                    for (int c = 0; c < 9; ++c) {
+                        if ((sm >> c) & 1u) {
+                            const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
+                            lo[c] = A.t_aabb[2 * nk];
+                            hi[c] = A.t_aabb[2 * nk + 1];
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 9; ++c)
+                        if (((sm >> c) & 1u) && ray_box(x1, inv, dir, lo[c], hi[c], bm))
+                            pmask |= 1u << ((dx + 1) * 9 + c);
+                }
+                // work items of the passing cells (independent table loads)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint32_t sm = (pmask >> ((dx + 1) * 9)) & 0x1FFu;
+                    if (!sm) continue;
+                    uint2 ent[9];
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) {
+                        if ((sm >> c) & 1u) {
+                            const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
+                            ent[c] = A.t_table[nk];
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 9; ++c)
+                        if ((sm >> c) & 1u) nitems += (min(ent[c].y, (uint32_t)A.ntri) + kItemTris - 1) / kItemTris;
                 }
             }
         }
         A.frc[i] = F.x; A.frc[np + i] = F.y; A.frc[2 * np + i] = F.z;
     }
-    // ---- reserve work-list space: one atomic per wave
-    const int lane = l & 63;
+    // ---- work-list space: every block owns region blockIdx.x of region_cap
+    // items; offsets from a block-wide exclusive scan (no global atomics)
+    const int lane = l & 63, wv = l >> 6;
     uint32_t incl = nitems;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t o = __shfl_up(incl, d, 64);
         if (lane >= d) incl += o;
     }
-    const uint32_t wave_total = __shfl(incl, 63, 64);
-    uint32_t wbase = 0;
-    if (lane == 63 && wave_total > 0) wbase = atomicAdd(&A.wctrl[0], wave_total);
-    wbase = __shfl(wbase, 63, 64);
+    if (lane == 63) s_wsum[wv] = incl;
+    if (l == 0) s_end = 0u;
+    __syncthreads();
+    uint32_t wave_off = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w)
+        if (w < wv) wave_off += s_wsum[w];
+    const uint32_t off = wave_off + incl - nitems;  // within the region
+    // particles whose items fit form a prefix of the scan order; the region's
+    // count ends with the last of them (no unwritten slots below it)
+    const bool fits = (uint64_t)off + nitems <= (uint64_t)A.region_cap;
+    if (act && nitems > 0 && fits) atomicMax(&s_end, off + nitems);
+    __syncthreads();
+    if (l == 0) A.wcount[blockIdx.x] = s_end;
     if (act) {
-        const uint32_t off = wbase + incl - nitems;
-        const bool fits = (uint64_t)off + nitems <= (uint64_t)A.work_cap;
         A.best[i] = (nitems > 0 && !fits) ? kScanSerial : kNoHit;
         if (nitems > 0 && !fits) fl |= BCS_FLAG_RAY_SERIAL;
-        if (nitems > 0 && fits && npass <= (uint32_t)kStage) {
+        if (nitems > 0 && fits) {
+            uint4* out = A.work + (size_t)blockIdx.x * A.region_cap;
             uint32_t w = off;
-            for (uint32_t k = 0; k < npass; ++k) {
-                const uint4 e = s_stage[l * kStage + k];
-                for (uint32_t u = e.x; u < e.x + e.y && w < off + nitems; u += kItemTris)
-                    A.work[w++] = make_uint4((uint32_t)i, u, e.z, min(kItemTris, e.x + e.y - u));
-            }
-        } else if (nitems > 0 && fits) {
-            uint32_t w = off;
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                uint2 ent[9];
-                const uint32_t mask = slab_cells(A, rc, dx, x1, inv, dir, bm, ent);
-#pragma unroll
-                for (int c = 0; c < 9; ++c) {
-                    if (!((mask >> c) & 1u)) continue;
-                    const uint32_t rank = (uint32_t)((dx + 1) * 9 + c);
-                    for (uint32_t u = ent[c].x; u < ent[c].x + ent[c].y && w < off + nitems; u += kItemTris)
-                        A.work[w++] = make_uint4((uint32_t)i, u, rank, min(kItemTris, ent[c].x + ent[c].y - u));
-                }
+            uint32_t m = pmask;
+            while (m) {
+                const int r = __builtin_ctz(m);
+                m &= m - 1u;
+                const int dx = r / 9 - 1, c = r % 9;
+                const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
+                const uint2 ent = A.t_table[nk];
+                for (uint32_t u = ent.x; u < ent.x + ent.y && w < off + nitems; u += kItemTris)
+                    out[w++] = make_uint4((uint32_t)i, u, (uint32_t)r, min(kItemTris, ent.x + ent.y - u));
             }
         }
     }
@@ -492,26 +574,45 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
 }
 
 // One thread per work item: first hit among its <= 16 triangles, folded into
-// the particle's minimum (cell rank, sorted slot).
+// the particle's minimum (cell rank, sorted slot).  One block per work region.
 __global__ void __launch_bounds__(256) k_ray(StepArgs A)
 {
-    const uint32_t n = min(A.wctrl[0], A.work_cap);
+    // kRaySplit blocks share the work region of k_cells_a block r; block
+    // b = s * nregions + r runs on the XCD (b % 8 == r % 8) whose L2 holds it
+    const uint32_t nreg = gridDim.x / kRaySplit;
+    const uint32_t r = blockIdx.x % nreg, sub = blockIdx.x / nreg;
+    const uint32_t n = min(A.wcount[r], A.region_cap);
+    const uint4* items = A.work + (size_t)r * A.region_cap;
     const int64_t np = A.tt.np;
     uint32_t fl = 0;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
-        const uint4 it = A.work[w];
+    for (uint32_t w = sub * blockDim.x + threadIdx.x; w < n; w += kRaySplit * blockDim.x) {
+        const uint4 it = items[w];
         const int32_t i = (int32_t)it.x;
         const f3 x1 = load3(A.pos, np, i);
         const f3 dir = normalize(load3(A.vel, np, i), fl);
         const uint32_t u1 = min(it.y + it.w, (uint32_t)A.ntri);
-        for (uint32_t u = it.y; u < u1; ++u) {
-            const float4 a = A.tdata[3 * (size_t)u], e1 = A.tdata[3 * (size_t)u + 1], e2 = A.tdata[3 * (size_t)u + 2];
-            float rt;
-            f3 nrm, refl;
-            if (ray_tri_pre(mk(a.x, a.y, a.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), x1, dir, rt, nrm, refl,
-                            fl)) {
-                atomicMin(&A.best[i], (it.z << kSlotBits) | u);
-                break;
+        // triangles in sorted order, their data loaded four at a time
+        bool found = false;
+        for (uint32_t u = it.y; u < u1 && !found; u += 4) {
+            float4 a[4], e1[4], e2[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (u + k < u1) {
+                    a[k] = A.tdata[3 * (size_t)(u + k)];
+                    e1[k] = A.tdata[3 * (size_t)(u + k) + 1];
+                    e2[k] = A.tdata[3 * (size_t)(u + k) + 2];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (found || u + k >= u1) continue;
+                float rt;
+                f3 nrm, refl;
+                if (ray_tri_pre(mk(a[k].x, a[k].y, a[k].z), mk(e1[k].x, e1[k].y, e1[k].z),
+                                mk(e2[k].x, e2[k].y, e2[k].z), x1, dir, rt, nrm, refl, fl)) {
+                    atomicMin(&A.best[i], (it.z << kSlotBits) | (u + k));
+                    found = true;
+                }
             }
         }
     }
@@ -524,10 +625,6 @@ __global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
     const int l = threadIdx.x;
-    if (blockIdx.x == 0 && l == 0) {  // work list consumed by k_ray
-        A.wctrl[1] = A.wctrl[0];      // kept for frame statistics
-        A.wctrl[0] = 0u;
-    }
     const int chunk = chunk_of_block(A.nchunks);
     if (chunk < 0) return;
     const int T = A.chunk_type[chunk];
@@ -779,7 +876,7 @@ __global__ void __launch_bounds__(256) k_tri_centers(const float* __restrict__ X
 // Per frame, one wavefront per occupied triangle cell: every lane prepares
 // one triangle of the cell's run -- (v0 | triangle id), v1 - v0, v2 - v0 from
 // the current vein positions, stored at its sorted slot -- and the wave
-// reduces the cell's box (exact vertex bounds) stored at the run start.
+// reduces the cell's box (exact vertex bounds), stored under the cell key.
 // Non-finite coordinates widen the box to everything.
 __global__ void __launch_bounds__(256) k_tri_box(const uint32_t* __restrict__ runs, const uint32_t* __restrict__ ctrl,
                                                  const uint2* __restrict__ table,
@@ -794,7 +891,8 @@ __global__ void __launch_bounds__(256) k_tri_box(const uint32_t* __restrict__ ru
     const uint32_t nr = ctrl[2];
     for (uint32_t r = wave; r < nr; r += nwaves) {
         const uint32_t s0 = runs[r];
-        const uint32_t cnt = table[keys_sorted[s0]].y;
+        const uint32_t key = keys_sorted[s0];
+        const uint32_t cnt = table[key].y;
         float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
         for (uint32_t t = lane; t < cnt && s0 + t < (uint32_t)ntri; t += 64) {
             const uint32_t u = s0 + t;
@@ -830,8 +928,8 @@ __global__ void __launch_bounds__(256) k_tri_box(const uint32_t* __restrict__ ru
             }
         }
         if (lane == 0) {
-            aabb[2 * s0] = make_float4(lo[0], lo[1], lo[2], 0.f);
-            aabb[2 * s0 + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+            aabb[2 * (size_t)key] = make_float4(lo[0], lo[1], lo[2], 0.f);
+            aabb[2 * (size_t)key + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
         }
     }
 }
@@ -896,6 +994,32 @@ __global__ void __launch_bounds__(256) k_order_scatter(int32_t nc, const uint32_
     corder[pos] = c;
 }
 
+// Per triangle-grid cell: mask of the occupied cells of its neighbourhood
+// (ranges of axis_range_u on the cell's own coordinates), over the visit rank.
+__global__ void __launch_bounds__(256) k_tri_nmask(GridGeom g, const uint2* __restrict__ table,
+                                                   uint32_t* __restrict__ nmask)
+{
+    const uint64_t cell = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= g.cell_count) return;
+    const uint32_t k = (uint32_t)cell;
+    uint32_t x, y, z;
+    grid_coords(g, k, x, y, z);
+    int xl, xh, yl, yh, zl, zh;
+    axis_range_u(x, g.cnt[0], xl, xh);
+    axis_range_u(y, g.cnt[1], yl, yh);
+    axis_range_u(z, g.cnt[2], zl, zh);
+    const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
+    uint32_t m = 0;
+    for (int dx = xl; dx <= xh; ++dx)
+        for (int dy = yl; dy <= yh; ++dy)
+            for (int dz = zl; dz <= zh; ++dz) {
+                const int64_t nk = (int64_t)k + dz * cXY + dy * cX + dx;
+                if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
+                if (table[nk].y != 0u) m |= 1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
+            }
+    nmask[k] = m;
+}
+
 // Contact listing in sorted-slot order (calculateParticleCollisions visit
 // order; the force is not applied).  pass 0 counts, pass 1 writes pairs at
 // offs[i].
@@ -927,7 +1051,7 @@ __global__ void __launch_bounds__(256) k_contacts(StepArgs A, int pass, int32_t*
             for (int dz = zl; dz <= zh; ++dz) {
                 const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
                 if (nk < 0 || (uint64_t)nk >= g.cell_count) continue;
-                if (A.p_occ[nk] == 0) continue;
+                if (cell_code(A.p_map, (uint32_t)nk) == 0u) continue;
                 const uint2 e = A.p_table[nk];
                 for (uint32_t u = e.x; u < min(e.x + e.y, (uint32_t)np); ++u) {
                     const int32_t j = A.p_perm[u];
@@ -976,6 +1100,12 @@ int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, u
     return BCS_OK;
 }
 
+void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tri_nmask, dim3((unsigned)((tg.g.cell_count + 255) / 256)), dim3(256), 0, st, tg.g, tg.table,
+                       nmask);
+}
+
 static dim3 chunk_grid(int nchunks) { return dim3(8 * ((nchunks + 7) / 8)); }
 
 void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st)
@@ -984,7 +1114,7 @@ void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st)
 }
 void launch_ray(const StepArgs& A, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ray, dim3(1024), dim3(256), 0, st, A);
+    if (A.nchunks > 0) hipLaunchKernelGGL(k_ray, dim3(chunk_grid(A.nchunks).x * kRaySplit), dim3(256), 0, st, A);
 }
 void launch_cells_b(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
 {

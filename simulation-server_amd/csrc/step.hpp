@@ -60,8 +60,8 @@ struct StepArgs {
     const uint32_t* p_keys;
     const int32_t* p_perm;
     const int32_t* p_slot;
-    const uint2* p_table;   // {start, count} per cell (current where p_occ != 0)
-    const uint8_t* p_occ;   // occupancy map, kOccPad readable bytes on both sides
+    const uint2* p_table;   // {start, count} per cell (current where occupied in p_map)
+    BrickMap p_map;         // 2-bit brick occupancy map of the particle grid
     const uint32_t* p_key_of;  // per particle: its cell key in the last build
     // triangle grid + vein
     GridGeom tg;
@@ -77,11 +77,13 @@ struct StepArgs {
     uint32_t* hctrl;  // [0] hit count, [1..2] vein-scan control, [4] hits of the last frame
     uint64_t* vstatus;
     int32_t vntiles;
-    const float4* t_aabb;     // per triangle run start: [2u] = lo, [2u+1] = hi
+    const float4* t_aabb;     // per occupied triangle cell key k: [2k] = lo, [2k+1] = hi
+    const uint32_t* t_nmask;  // per triangle cell: occupied cells of its neighbourhood
     uint32_t* best;           // per particle: min (cell rank << 27 | slot) of its ray hits
-    uint4* work;              // ray work items: (particle, first slot, cell rank, count)
-    uint32_t* wctrl;          // [0] work items issued, [1] items of the last frame
-    uint32_t work_cap;
+    uint4* work;              // ray work items: (particle, first slot, cell rank, count), one
+                              // region of region_cap items per k_cells_a block
+    uint32_t* wcount;         // items per region (last frame)
+    uint32_t region_cap;
     uint32_t* flags;
     uint64_t* step;
 };
