@@ -18,6 +18,13 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "lib" / "libbcs.so"
 
 BCS_OK = 0
+BCS_ERR_INVALID_ARGUMENT = 1
+BCS_ERR_HIP = 2
+BCS_ERR_OUT_OF_MEMORY = 3
+BCS_ERR_GRID_TOO_LARGE = 4
+BCS_ERR_NO_DEVICE = 5
+BCS_ERR_STATE = 6
+BCS_ERR_UNSUPPORTED = 7
 GRID_PARTICLES = 0
 GRID_TRIANGLES = 1
 
@@ -25,6 +32,7 @@ FLAG_OUT_OF_GRID = 0x1
 FLAG_NAN_NORMALIZE = 0x2
 FLAG_SCAN_TIMEOUT = 0x4
 FLAG_RADIUS_CLAMPED = 0x8
+FLAG_RAY_SERIAL = 0x10
 
 VEIN_CYLINDER, VEIN_Y = 0, 1
 CELL_ICOSAHEDRON, CELL_DISC42 = 0, 1

@@ -125,6 +125,22 @@ def test_fused_100_steps_c0(bcs, orc):
     assert sim.get_info().step_index == 100
 
 
+def test_fused_1000_steps_c0_bit_exact(bcs, orc):
+    """The whole config-0 run (1,000 frames, teleports included) stays bit
+    for bit on the oracle's trajectory."""
+    sc = _scene(bcs, "c0")
+    sim, o = _pair(bcs, orc, sc)
+    for chunk in range(4):
+        sim.step(250)
+        o.step(250)
+        gp, gv, gf = sim.get_particles()
+        op, ov, of = o.get_particles()
+        assert_bits_equal(gp, op, f"positions after {250 * (chunk + 1)} frames")
+        assert_bits_equal(gv, ov, f"velocities after {250 * (chunk + 1)} frames")
+    assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "vein after 1000 frames")
+    assert sim.get_info().step_index == 1000
+
+
 def test_fused_40_steps_mixed_types(bcs, orc):
     """Two cell types (discs and icosahedra): 40 fused frames, across two
     refreshes of the spatial processing order of the cells."""
