@@ -12,7 +12,9 @@
 //   k_rank     in-bucket rank by counting over (key, index): the output order is
 //              exactly thrust's stable order (ties by object index); run starts
 //              write the dense cell-start table
-//   k_big      the same for buckets larger than the k_rank threshold (LDS tiles)
+//   k_big      buckets larger than the k_rank threshold: bitonic sort of the
+//              (key, index) pairs in LDS (up to kSortLds), else ranking by
+//              counting over LDS tiles
 //   k_occ      (optional) the 2-bit brick occupancy map from the sorted keys:
 //              the first slot of every 8-cell x-group writes the group's word
 // Cell table: dense {start, count} per cell id (8 bytes, one load per lookup),
@@ -30,7 +32,8 @@ namespace bcs {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
-constexpr uint32_t kRankThreshold = 256;
+constexpr uint32_t kRankThreshold = 64;
+constexpr uint32_t kSortLds = 4096;  // bucket size sorted in LDS by k_big
 constexpr uint64_t kStatusAgg = 1ull << 62;
 constexpr uint64_t kStatusInc = 2ull << 62;
 constexpr uint64_t kSpinLimit = 1ull << 22;
@@ -231,15 +234,54 @@ __global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_
                                              const uint32_t* __restrict__ biglist, const uint32_t* __restrict__ ctrl,
                                              RankOut o)
 {
+    __shared__ uint64_t sv[kSortLds];
     __shared__ uint32_t sk[kBigChunk];
     __shared__ int32_t si[kBigChunk];
     const uint32_t nbig = ctrl[1];
+    const uint32_t tid = threadIdx.x;
     for (uint32_t e = blockIdx.x; e < nbig; e += gridDim.x) {
         const uint32_t b = biglist[e];
         const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
         if (s1 < s0 || s1 > (uint32_t)n) continue;
+        const uint32_t m = s1 - s0;
+        if (m <= kSortLds) {
+            // (key << 32 | index) orders exactly like the stable sort by key
+            uint32_t P = 1;
+            while (P < m) P <<= 1;
+            for (uint32_t t = tid; t < P; t += blockDim.x)
+                sv[t] = t < m ? ((uint64_t)items_k[s0 + t] << 32) | (uint32_t)items_i[s0 + t] : ~0ull;
+            __syncthreads();
+            for (uint32_t k = 2; k <= P; k <<= 1) {
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t t = tid; t < P; t += blockDim.x) {
+                        const uint32_t u = t ^ j;
+                        if (u > t) {
+                            const uint64_t a = sv[t], c = sv[u];
+                            if ((a > c) == ((t & k) == 0)) {
+                                sv[t] = c;
+                                sv[u] = a;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (uint32_t t = tid; t < m; t += blockDim.x) {
+                const uint64_t v = sv[t];
+                const uint32_t key = (uint32_t)(v >> 32);
+                const bool first = t == 0 || (uint32_t)(sv[t - 1] >> 32) != key;
+                uint32_t count = 0;
+                if (first) {
+                    count = 1;
+                    while (t + count < m && (uint32_t)(sv[t + count] >> 32) == key) ++count;
+                }
+                rank_write(o, key, (int32_t)(uint32_t)v, s0 + t, first, count);
+            }
+            __syncthreads();  // sv is reused by the next bucket
+            continue;
+        }
         for (uint32_t gbase = s0; gbase < s1; gbase += blockDim.x) {
-            const uint32_t q = gbase + threadIdx.x;
+            const uint32_t q = gbase + tid;
             const bool own = q < s1;
             const uint32_t key = own ? items_k[q] : 0u;
             const int32_t idx = own ? items_i[q] : 0;
@@ -248,7 +290,7 @@ __global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_
             for (uint32_t cbase = s0; cbase < s1; cbase += kBigChunk) {
                 const uint32_t cn = min((uint32_t)kBigChunk, s1 - cbase);
                 __syncthreads();
-                for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
+                for (uint32_t t = tid; t < cn; t += blockDim.x) {
                     sk[t] = items_k[cbase + t];
                     si[t] = items_i[cbase + t];
                 }
@@ -270,6 +312,7 @@ __global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_
             }
             if (own) rank_write(o, key, idx, s0 + rank, first, same);
         }
+        __syncthreads();
     }
 }
 
@@ -417,8 +460,8 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint3
     hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, o);
     te(3);
     tb(4);
-    hipLaunchKernelGGL(k_big, dim3(64), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, G.biglist, G.ctrl,
-                       o);
+    hipLaunchKernelGGL(k_big, dim3(std::max(1u, std::min(G.big_cap, 4096u))), dim3(256), 0, st, G.items_k, G.items_i, n,
+                       G.shift, G.bstart, G.biglist, G.ctrl, o);
     te(4);
     tb(5);
     if (G.gpos)

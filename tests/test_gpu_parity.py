@@ -44,6 +44,27 @@ def test_particle_grid_matches_oracle(bcs, orc, kind):
         assert_bits_equal(a, b, name)
 
 
+@pytest.mark.parametrize("squeeze", [0.25, 0.03, 0.002])
+def test_particle_grid_crowded_matches_oracle(bcs, orc, squeeze):
+    """Particles pulled together so that sort buckets hold hundreds to all
+    6,000 objects (LDS bitonic path and the tiled fallback)."""
+    sc = _scene(bcs, "c0")
+    p = sc.positions.reshape(3, -1).copy()
+    c = p.mean(axis=1, keepdims=True)
+    p = (c + (p - c) * np.float32(squeeze)).astype(np.float32)
+    sc.positions = p.reshape(-1)
+    sim, o = _pair(bcs, orc, sc)
+    sim.calculate_grid(0)
+    o.calculate_grid(0)
+    g, r = sim.get_grid(0), o.get_grid(0)
+    for name, a, b in zip(("keys_sorted", "perm", "occ_keys", "occ_start", "occ_end"), g, r):
+        assert_bits_equal(a, b, f"crowded x{squeeze} {name}")
+    o.calculate_grid(1)
+    sim.calculate_next_frame()
+    o.calculate_next_frame()
+    assert_bits_equal(sim.get_particles()[2], o.get_particles()[2], f"crowded x{squeeze} forces")
+
+
 @pytest.mark.parametrize("kind", ["c0", "c1_small"])
 def test_triangle_grid_matches_oracle(bcs, orc, kind):
     sc = _scene(bcs, kind)
