@@ -115,13 +115,14 @@ def cpu_baseline(scene, target_s=12.0, max_steps=400):
 
 
 def pmc_traffic(kernel, config_id):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
-    p = ROOT / "profiles" / f"{kernel}_config{config_id}_pmc.json"
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-kernel calibration)."""
+    p = ROOT / "profiles" / f"frame_kernels_config{config_id}_pmc.json"
     if not p.exists():
         return None
     try:
         d = json.loads(p.read_text())
-        return float(d["hbm_bytes_per_launch"])
+        return float(d["kernels"][kernel]["hbm_bytes_per_launch"])
     except Exception:
         return None
 
