@@ -23,7 +23,6 @@ void launch_integrate(const StepArgs& A, hipStream_t st);
 void launch_vein_end(const StepArgs& A, int nchunks, hipStream_t st);
 void launch_bump(uint64_t* step, hipStream_t st);
 void launch_vein_springs(const VeinArgs& V, hipStream_t st);
-void launch_vein_scatter(const VeinArgs& V, hipStream_t st);
 void launch_vein(const VeinArgs& V, int mode, hipStream_t st);
 void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t ntri, float* c, hipStream_t st);
 void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st);
@@ -48,6 +47,8 @@ using namespace bcs;
 struct bcs_sim {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;        // second branch of the frame (vein springs, triangle data)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bcs_physics phys{};
     int32_t nt = 0, np = 0, nc = 0, nm = 0, nv = 0, ntri = 0;
     float bmin[3]{}, bmax[3]{};
@@ -69,10 +70,9 @@ struct bcs_sim {
     float *X = nullptr, *V = nullptr, *Fv = nullptr, *nbl = nullptr, *tcent = nullptr;
     int32_t* nbr = nullptr;
     uint32_t* tidx = nullptr;
-    uint32_t *vcount = nullptr, *vstart = nullptr, *hctrl = nullptr;
-    float4 *hits = nullptr, *seg = nullptr;
-    uint64_t* vstatus = nullptr;
-    int32_t vntiles = 0;
+    uint32_t* hctrl = nullptr;
+    float4* hits = nullptr;                     // 3 float4 per hit
+    int32_t *vhead = nullptr, *vnext = nullptr;  // per-vertex lists of hit entries
     float4* t_aabb = nullptr;  // triangle-cell boxes by cell key (per frame)
     uint32_t* t_nmask = nullptr;  // per triangle cell: occupied neighbourhood mask
     float4* tdata = nullptr;   // per sorted triangle: (v0 | id), e1, e2 (per frame)
@@ -100,6 +100,9 @@ struct bcs_sim {
         grid_free(pg);
         grid_free(tg);
         for (void* p : allocs) (void)hipFree(p);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -124,7 +127,7 @@ struct bcs_sim {
         A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table; A.p_map = pg.bm; A.p_key_of = pg.keys;
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
         A.nv = nv; A.X = X; A.tidx = tidx;
-        A.vcount = vcount; A.hits = hits; A.hctrl = hctrl; A.vstatus = vstatus; A.vntiles = vntiles;
+        A.hits = hits; A.vhead = vhead; A.vnext = vnext; A.hctrl = hctrl;
         A.t_aabb = t_aabb; A.t_nmask = t_nmask;
         A.best = best; A.work = work; A.wcount = wcount; A.region_cap = region_cap;
         A.flags = flags; A.step = step;
@@ -135,7 +138,7 @@ struct bcs_sim {
         VeinArgs Va{};
         Va.nv = nv; Va.X = X; Va.V = V; Va.Fv = Fv; Va.nbr = nbr; Va.nbl = nbl;
         Va.dt = phys.dt; Va.vk = phys.vein_k_sniff; Va.vd = phys.vein_d_fact;
-        Va.vcount = vcount; Va.vstart = vstart; Va.hits = hits; Va.seg = seg; Va.hctrl = hctrl; Va.tidx = tidx;
+        Va.hits = hits; Va.vhead = vhead; Va.vnext = vnext; Va.hctrl = hctrl; Va.tidx = tidx;
         Va.flags = flags; Va.step = step;
         return Va;
     }
@@ -191,12 +194,6 @@ int enqueue_particle_grid(bcs_sim* s, const KernelTimer* timer = nullptr)
     return grid_build(s->pg, s->pos, s->np, s->np, s->flags, s->stream, timer);
 }
 
-int enqueue_vein_hits(bcs_sim* s, const VeinArgs& Va)
-{
-    RC(scan_exclusive(s->vcount, s->vstart, s->nv, s->vstatus, s->vntiles, s->hctrl + 1, s->flags, s->stream));
-    launch_vein_scatter(Va, s->stream);
-    return BCS_OK;
-}
 
 // One fused frame: particle grid, vein springs, k_cells (springs, contacts,
 // vein ray, integrate, vein end), vein-hit segments, vein sum + integrate.
@@ -206,13 +203,25 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
     auto tb = [&](int id) { if (timer) timer->begin(id, st); };
     auto te = [&](int id) { if (timer) timer->end(id, st); };
     if (s->phys.dynamic_triangle_grid) RC(build_triangle_grid(s));
-    RC(enqueue_particle_grid(s, timer));
     const StepArgs A = s->step_args();
     const VeinArgs Va = s->vein_args();
-    tb(kGridKernels + 0);
-    launch_vein_springs(Va, st);
-    launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, st);
-    te(kGridKernels + 0);
+    if (!timer) {
+        // Vein springs and the triangle data depend only on the vein state: a
+        // second branch runs them while the particle grid is built.
+        HIPCHK(hipEventRecord(s->ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+        launch_vein_springs(Va, s->side);
+        launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, s->side);
+        HIPCHK(hipEventRecord(s->ev_join, s->side));
+        RC(enqueue_particle_grid(s));
+        HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
+    } else {
+        RC(enqueue_particle_grid(s, timer));
+        tb(kGridKernels + 0);
+        launch_vein_springs(Va, st);
+        launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, st);
+        te(kGridKernels + 0);
+    }
     tb(kGridKernels + 1);
     launch_cells_a(A, s->nchunks, st);
     te(kGridKernels + 1);
@@ -223,22 +232,15 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
     launch_cells_b(A, s->nchunks, true, st);
     te(kGridKernels + 3);
     tb(kGridKernels + 4);
-    RC(scan_exclusive(s->vcount, s->vstart, s->nv, s->vstatus, s->vntiles, s->hctrl + 1, s->flags, st));
-    te(kGridKernels + 4);
-    tb(kGridKernels + 5);
-    launch_vein_scatter(Va, st);
-    te(kGridKernels + 5);
-    tb(kGridKernels + 6);
     launch_vein(Va, 7, st);
-    te(kGridKernels + 6);
+    te(kGridKernels + 4);
     HIPCHK(hipGetLastError());
     return BCS_OK;
 }
 
 const char* kKernelNames =
-    "k_key;k_scan;k_scatter;k_rank;k_big;k_occ;k_vein_springs+k_tri_box;k_cells_a;k_ray;k_cells_b;k_scan(vein);"
-    "k_vein_scatter;k_vein";
-constexpr int kFusedKernels = kGridKernels + 7;
+    "k_key;k_scan;k_scatter;k_rank;k_big;k_occ;k_vein_springs+k_tri_box;k_cells_a;k_ray;k_cells_b;k_vein";
+constexpr int kFusedKernels = kGridKernels + 5;
 
 }  // namespace
 
@@ -268,6 +270,9 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     s->device = device;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
     s->phys = *phys;
     s->nt = n_types;
     // ---- type tables (blood_cell_factory.hpp:172-282)
@@ -402,11 +407,8 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(s->dalloc(&s->X, 3 * NV)); RC(s->dalloc(&s->V, 3 * NV)); RC(s->dalloc(&s->Fv, 3 * NV));
     RC(s->dalloc(&s->nbr, kVeinSlots * NV)); RC(s->dalloc(&s->nbl, kVeinSlots * NV));
     RC(s->dalloc(&s->tidx, 3 * NT)); RC(s->dalloc(&s->tcent, 3 * NT));
-    RC(s->dalloc(&s->vcount, NV + 1)); RC(s->dalloc(&s->vstart, NV + 2));
-    RC(s->dalloc(&s->hits, 3 * N)); RC(s->dalloc(&s->seg, 3 * N));
+    RC(s->dalloc(&s->hits, 3 * N)); RC(s->dalloc(&s->vnext, 3 * N)); RC(s->dalloc(&s->vhead, NV));
     RC(s->dalloc(&s->hctrl, 16));
-    s->vntiles = scan_tiles(s->nv);
-    RC(s->dalloc(&s->vstatus, (size_t)s->vntiles));
     RC(s->dalloc(&s->flags, 4)); RC(s->dalloc(&s->step, 1));
     RC(s->dalloc(&s->tdata, 3 * NT));
     // ray work list: 8 items per particle, split evenly over the k_cells_a blocks
@@ -459,7 +461,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
         HIPCHK(hipMemcpyAsync(s->tidx, mesh->indices, 12 * NT, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));  // host staging vectors die at the end of this block
     }
-    HIPCHK(hipMemsetAsync(s->vcount, 0, 4 * (NV + 1), s->stream));
+    HIPCHK(hipMemsetAsync(s->vhead, 0xFF, 4 * std::max<size_t>(NV, 1), s->stream));
     HIPCHK(hipMemsetAsync(s->hctrl, 0, 64, s->stream));
     HIPCHK(hipMemsetAsync(s->flags, 0, 4, s->stream));
     HIPCHK(hipMemsetAsync(s->step, 0, 8, s->stream));
@@ -614,7 +616,6 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
     launch_cells_a(A, s->nchunks, s->stream);
     launch_ray(A, s->stream);
     launch_cells_b(A, s->nchunks, false, s->stream);
-    RC(enqueue_vein_hits(s, Va));
     launch_vein(Va, 1, s->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));

@@ -355,12 +355,6 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
     const int l = threadIdx.x;
-    // reset the vein-hit scan control for this frame (its k_scan runs later)
-    {
-        const int64_t gid = (int64_t)blockIdx.x * blockDim.x + l;
-        for (int64_t t = gid; t < A.vntiles; t += (int64_t)gridDim.x * blockDim.x) A.vstatus[t] = 0ull;
-        if (gid == 0) A.hctrl[1] = 0u;
-    }
     const int chunk = chunk_of_block(A.nchunks);
     if (chunk < 0) {
         if (l == 0) A.wcount[blockIdx.x] = 0u;
@@ -684,9 +678,13 @@ __global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
             A.hits[3 * (size_t)h + 0] = make_float4(c0.x, c0.y, c0.z, __int_as_float(i));
             A.hits[3 * (size_t)h + 1] = make_float4(c1.x, c1.y, c1.z, __int_as_float(obj));
             A.hits[3 * (size_t)h + 2] = make_float4(c2.x, c2.y, c2.z, 0.0f);
-            atomicAdd(&A.vcount[i0], 1u);
-            atomicAdd(&A.vcount[i1], 1u);
-            atomicAdd(&A.vcount[i2], 1u);
+            // push entry 3h + k (corner k) on the list of its vertex
+            const uint32_t vs[3] = {i0, i1, i2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int32_t e = (int32_t)(3 * h + k);
+                A.vnext[e] = atomicExch(&A.vhead[vs[k]], e);
+            }
         }
     }
     if (FUSED) {
@@ -792,26 +790,6 @@ __global__ void __launch_bounds__(256) k_vein_springs(VeinArgs A)
     if (fl) atomicOr(A.flags, fl);
 }
 
-// Distribute recorded hits into per-vertex segments (order within a segment
-// is fixed later by the particle index).
-__global__ void __launch_bounds__(256) k_vein_scatter(VeinArgs A)
-{
-    const uint32_t nh = A.hctrl[0];
-    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
-        const float4 r0 = A.hits[3 * (size_t)h], r1 = A.hits[3 * (size_t)h + 1], r2 = A.hits[3 * (size_t)h + 2];
-        const int32_t i = __float_as_int(r0.w);
-        const int32_t tri = __float_as_int(r1.w);
-        const float4 rr[3] = {r0, r1, r2};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const uint32_t v = A.tidx[3 * tri + k];
-            const uint32_t old = atomicSub(&A.vcount[v], 1u);
-            const uint32_t pos = A.vstart[v] + old - 1u;
-            A.seg[pos] = make_float4(rr[k].x, rr[k].y, rr[k].z, __int_as_float(i * 4 + k));
-        }
-    }
-}
-
 // Per vertex: add the recorded contributions in (particle, corner) order, then
 // (mode & 2) integrate: V += dt*Fv; X += dt*V; Fv = 0
 // (propagateForcesIntoPositionsKernel, src/objects/vein_triangles.cu:88-117).
@@ -820,7 +798,7 @@ __global__ void __launch_bounds__(256) k_vein(VeinArgs A)
 {
     const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (MODE & 1) {
-        if (id == 0) {  // hits consumed (k_vein_scatter ran before)
+        if (id == 0) {  // hits consumed (the hit buffer is only read through the lists)
             A.hctrl[4] = A.hctrl[0];
             A.hctrl[0] = 0u;
         }
@@ -832,21 +810,30 @@ __global__ void __launch_bounds__(256) k_vein(VeinArgs A)
     const int64_t nv = A.nv;
     f3 F = load3(A.Fv, nv, id);
     if (MODE & 1) {
-        const uint32_t s0 = A.vstart[id], s1 = A.vstart[id + 1];
-        int32_t last = -1;
-        for (uint32_t r = s0; r < s1; ++r) {
-            int32_t best = 0x7FFFFFFF;
-            float4 bv = make_float4(0.f, 0.f, 0.f, 0.f);
-            for (uint32_t u = s0; u < s1; ++u) {
-                const float4 e = A.seg[u];
-                const int32_t key = __float_as_int(e.w);
-                if (key > last && key < best) {
-                    best = key;
-                    bv = e;
+        const int32_t head = A.vhead[id];
+        if (head >= 0) {
+            // the vertex's entries in ascending (particle, corner) key order,
+            // picked by selection from its list (lists are a few entries long)
+            int32_t n = 0;
+            for (int32_t e = head; e >= 0 && n < (1 << 20); e = A.vnext[e]) ++n;
+            int64_t last = -1;
+            for (int32_t r = 0; r < n; ++r) {
+                int64_t best = 0x7FFFFFFFFFFFFFFFll;
+                int32_t be = -1;
+                int32_t c = 0;
+                for (int32_t e = head; e >= 0 && c < n; e = A.vnext[e], ++c) {
+                    const int64_t key = (int64_t)__float_as_int(A.hits[3 * (size_t)(e / 3)].w) * 4 + (e % 3);
+                    if (key > last && key < best) {
+                        best = key;
+                        be = e;
+                    }
                 }
+                if (be < 0) break;
+                const float4 cv = A.hits[be];
+                F = add(F, mk(cv.x, cv.y, cv.z));
+                last = best;
             }
-            F = add(F, mk(bv.x, bv.y, bv.z));
-            last = best;
+            A.vhead[id] = -1;
         }
     }
     if (MODE & 2) {
@@ -1134,10 +1121,6 @@ void launch_bump(uint64_t* step, hipStream_t st) { hipLaunchKernelGGL(k_bump_ste
 void launch_vein_springs(const VeinArgs& V, hipStream_t st)
 {
     if (V.nv > 0) hipLaunchKernelGGL(k_vein_springs, dim3((V.nv + 255) / 256), dim3(256), 0, st, V);
-}
-void launch_vein_scatter(const VeinArgs& V, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_vein_scatter, dim3(1024), dim3(256), 0, st, V);
 }
 void launch_vein(const VeinArgs& V, int mode, hipStream_t st)
 {

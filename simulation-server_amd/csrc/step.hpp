@@ -72,11 +72,10 @@ struct StepArgs {
     const float* X;  // SoA stride nv
     const uint32_t* tidx;
     // vein hits
-    uint32_t* vcount;
-    float4* hits;     // 3 float4 per hit
-    uint32_t* hctrl;  // [0] hit count, [1..2] vein-scan control, [4] hits of the last frame
-    uint64_t* vstatus;
-    int32_t vntiles;
+    float4* hits;     // 3 float4 per hit: (c0, particle), (c1, triangle), (c2, -)
+    int32_t* vhead;   // per vein vertex: first hit entry (3h + corner) of its list, -1 = none
+    int32_t* vnext;   // per hit entry: next entry of the same vertex, -1 = end
+    uint32_t* hctrl;  // [0] hit count, [4] hits of the last frame
     const float4* t_aabb;     // per occupied triangle cell key k: [2k] = lo, [2k+1] = hi
     const uint32_t* t_nmask;  // per triangle cell: occupied cells of its neighbourhood
     uint32_t* best;           // per particle: min (cell rank << 27 | slot) of its ray hits
@@ -97,10 +96,9 @@ struct VeinArgs {
     const int32_t* nbr;  // 9*nv slot-major
     const float* nbl;
     float dt, vk, vd;
-    uint32_t* vcount;
-    uint32_t* vstart;   // nv+1
     const float4* hits;
-    float4* seg;        // ordered contributions: (c.xyz, key)
+    int32_t* vhead;
+    const int32_t* vnext;
     uint32_t* hctrl;
     const uint32_t* tidx;
     uint32_t* flags;
