@@ -43,7 +43,9 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
                                              uint2* __restrict__ table, int clear_table, BrickMap bm,
                                              const uint32_t* __restrict__ gpos, uint32_t* __restrict__ bcount,
                                              uint64_t* __restrict__ status, int32_t ntiles,
-                                             uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags)
+                                             uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags,
+                                             const float* __restrict__ vel, float4* __restrict__ px4,
+                                             float4* __restrict__ vx4)
 {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -65,9 +67,14 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
     }
     uint32_t fl = 0;
     if (gid < n) {
-        uint32_t k = cell_key(g, pos[gid], pos[stride + gid], pos[2 * stride + gid], fl);
+        const float x = pos[gid], y = pos[stride + gid], z = pos[2 * stride + gid];
+        uint32_t k = cell_key(g, x, y, z, fl);
         keys[gid] = k;
         atomicAdd(&bcount[k >> shift], 1u);
+        if (px4) {  // packed copies for neighbour reads of this frame
+            px4[gid] = make_float4(x, y, z, 0.0f);
+            vx4[gid] = make_float4(vel[gid], vel[stride + gid], vel[2 * stride + gid], 0.0f);
+        }
     }
     if (fl) atomicOr(flags, fl);
 }
@@ -369,7 +376,7 @@ static int choose_shift(uint64_t cell_count, int64_t cap)
     } while (0)
 
 int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_runs, bool with_map,
-               hipStream_t st)
+               bool with_copies, hipStream_t st)
 {
     G = DeviceGrid{};
     G.g = geom;
@@ -386,6 +393,10 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     GRID_HIP(hipMalloc(&G.items_i, n * 4));
     GRID_HIP(hipMalloc(&G.keys_sorted, n * 4));
     if (with_runs) GRID_HIP(hipMalloc(&G.runs, n * 4));
+    if (with_copies) {
+        GRID_HIP(hipMalloc(&G.px4, n * 16));
+        GRID_HIP(hipMalloc(&G.vx4, n * 16));
+    }
     GRID_HIP(hipMalloc(&G.perm, n * 4));
     if (with_slot) GRID_HIP(hipMalloc(&G.slot_of, n * 4));
     GRID_HIP(hipMemsetAsync(G.keys, 0xFF, n * 4, st));
@@ -423,15 +434,15 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
 
 void grid_free(DeviceGrid& G)
 {
-    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.runs, G.perm, G.slot_of,
+    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.runs, G.perm, G.slot_of, G.px4, G.vx4,
                     G.table, G.bm.words, G.gpos, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     G = DeviceGrid{};
 }
 
-int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint32_t* flags, hipStream_t st,
-               const KernelTimer* timer)
+int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
+               hipStream_t st, const KernelTimer* timer)
 {
     if (n < 0 || n > G.cap) return fail(BCS_ERR_INVALID_ARGUMENT, "grid: object count exceeds capacity");
     // Every slot below the capacity clears its previous cell (slots never
@@ -445,7 +456,7 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint3
     tb(0);
     hipLaunchKernelGGL(k_key, dim3(std::max(blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride, n,
                        n_prev, G.shift, G.keys, G.table, G.clear_table ? 1 : 0, G.bm, G.gpos, G.bcount, G.status,
-                       G.ntiles, G.ctrl, flags);
+                       G.ntiles, G.ctrl, flags, vel, vel ? G.px4 : nullptr, vel ? G.vx4 : nullptr);
     te(0);
     tb(1);
     hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,

@@ -38,6 +38,8 @@ struct DeviceGrid {
     bool clear_table = true;          // zero the entries of cells that emptied (else: consult bm)
     BrickMap bm{};                    // occupancy map (optional; words == nullptr when absent)
     uint32_t* gpos = nullptr;         // per sorted slot: map word written by the last build, or kNoWord
+    float4* px4 = nullptr;            // packed copies in object order (optional): (x, y, z, 0)
+    float4* vx4 = nullptr;            //                                           (vx, vy, vz, 0)
     uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds
     uint32_t* bstart = nullptr;       // bucket offsets [nb+2]
     uint64_t* status = nullptr;       // scan tile status
@@ -49,11 +51,12 @@ struct DeviceGrid {
 // Initialisation is enqueued on `st` (the owner's stream, which does not
 // synchronise with the legacy NULL stream).
 int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_runs, bool with_map,
-               hipStream_t st);
+               bool with_copies, hipStream_t st);
 void grid_free(DeviceGrid& G);
-// pos: SoA base pointer (x, y, z at +0, +stride, +2*stride).
-int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, uint32_t* flags, hipStream_t st,
-               const KernelTimer* timer = nullptr);
+// pos: SoA base pointer (x, y, z at +0, +stride, +2*stride).  vel (same
+// layout, nullable) with a grid allocated with copies also writes px4 / vx4.
+int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
+               hipStream_t st, const KernelTimer* timer = nullptr);
 constexpr int kGridKernels = 6;
 
 int scan_exclusive(const uint32_t* count, uint32_t* out, int64_t m, uint64_t* status, int32_t ntiles, uint32_t* ctrl,

@@ -31,6 +31,7 @@ void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uin
 void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
                     float4* aabb, hipStream_t st);
 void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st);
+void launch_refresh_copies(const float* pos, const float* vel, int32_t n, float4* px4, float4* vx4, hipStream_t st);
 int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
                  uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
                  uint32_t* flags, hipStream_t st);
@@ -84,6 +85,7 @@ struct bcs_sim {
     // grids
     DeviceGrid pg, tg;
     bool tg_built = false;
+    bool copies_current = false;  // packed position/velocity copies match pos/vel
     // misc
     uint32_t* flags = nullptr;
     uint64_t* step = nullptr;
@@ -125,6 +127,7 @@ struct bcs_sim {
         A.chunk_type = chunk_type; A.chunk_first = chunk_first; A.chunk_n = chunk_n;
         A.corder = corder; A.nchunks = nchunks;
         A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table; A.p_map = pg.bm; A.p_key_of = pg.keys;
+        A.px4 = pg.px4; A.vx4 = pg.vx4;
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.hits = hits; A.vhead = vhead; A.vnext = vnext; A.hctrl = hctrl;
@@ -183,7 +186,7 @@ int build_triangle_grid(bcs_sim* s)
 {
     const bool dyn = s->phys.dynamic_triangle_grid != 0;
     if (dyn || !s->tg_built) launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
-    RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, s->flags, s->stream));
+    RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, nullptr, s->flags, s->stream));
     launch_tri_nmask(s->tg, s->t_nmask, s->stream);
     s->tg_built = true;
     return BCS_OK;
@@ -191,7 +194,9 @@ int build_triangle_grid(bcs_sim* s)
 
 int enqueue_particle_grid(bcs_sim* s, const KernelTimer* timer = nullptr)
 {
-    return grid_build(s->pg, s->pos, s->np, s->np, s->flags, s->stream, timer);
+    RC(grid_build(s->pg, s->pos, s->np, s->np, s->vel, s->flags, s->stream, timer));
+    s->copies_current = true;
+    return BCS_OK;
 }
 
 
@@ -469,9 +474,9 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     // ---- grids
     const GridGeom pgeom = make_geom(bmin, bmax, phys->particle_cell);
     const GridGeom tgeom = make_geom(bmin, bmax, phys->triangle_cell);
-    RC(grid_alloc(s->pg, pgeom, s->np, true, false, true, s->stream));
+    RC(grid_alloc(s->pg, pgeom, s->np, true, false, true, true, s->stream));
     s->pg.clear_table = false;  // readers consult the occupancy map
-    RC(grid_alloc(s->tg, tgeom, s->ntri, false, true, false, s->stream));
+    RC(grid_alloc(s->tg, tgeom, s->ntri, false, true, false, false, s->stream));
     RC(s->dalloc(&s->t_nmask, (size_t)tgeom.cell_count));
     RC(s->dalloc(&s->t_aabb, 2 * (size_t)tgeom.cell_count));
     RC(build_triangle_grid(s.get()));
@@ -553,6 +558,7 @@ extern "C" int bcs_sim_set_particles(bcs_sim* s, const float* p, const float* v,
     RC(set_dev(s));
     const size_t n = 3 * (size_t)s->np;
     RC(copy_in(s, s->pos, p, n)); RC(copy_in(s, s->vel, v, n)); RC(copy_in(s, s->frc, f, n));
+    if (p || v) s->copies_current = false;
     HIPCHK(hipStreamSynchronize(s->stream));
     return BCS_OK;
 }
@@ -609,6 +615,12 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
 {
     RC(set_dev(s));
     if (s->pg.n != s->np) return fail(BCS_ERR_STATE, "particle grid not built (call bcs_sim_calculate_grid)");
+    // particles moved since the grid was built: the reference then scans the
+    // old cells with the current positions, so refresh the packed copies
+    if (!s->copies_current) {
+        launch_refresh_copies(s->pos, s->vel, s->np, s->pg.px4, s->pg.vx4, s->stream);
+        s->copies_current = true;
+    }
     const StepArgs A = s->step_args();
     const VeinArgs Va = s->vein_args();
     launch_vein_springs(Va, s->stream);
@@ -625,6 +637,7 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
 extern "C" int bcs_sim_propagate_all(bcs_sim* s)
 {
     RC(set_dev(s));
+    s->copies_current = false;
     const StepArgs A = s->step_args();
     const VeinArgs Va = s->vein_args();
     launch_integrate(A, s->stream);
@@ -671,6 +684,7 @@ extern "C" int bcs_sim_step(bcs_sim* s, int32_t n_steps)
         else RC(enqueue_fused_frame(s));
         ++s->frames_since_order;
     }
+    s->copies_current = false;  // the frames moved the particles after their grid builds
     HIPCHK(hipGetLastError());
     return BCS_OK;
 }
@@ -738,6 +752,7 @@ extern "C" int bcs_sim_profile_kernels(bcs_sim* s, int32_t n_steps, float* ms_ou
         (void)hipEventDestroy(ee[k]);
     }
     *n_kernels = kFusedKernels;
+    s->copies_current = false;  // the profiled frames moved the particles
     if (names_out && names_capacity > 0) {
         std::strncpy(names_out, kKernelNames, (size_t)names_capacity - 1);
         names_out[names_capacity - 1] = 0;

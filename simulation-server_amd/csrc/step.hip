@@ -434,14 +434,16 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
 
         // ---- particle-particle contacts over the 27 neighbour cells, visited
         // in the reference order (x, then y, then z); only occupied cells
-        // (from the occupancy map) are looked up in the cell table.
+        // (from the occupancy map) are looked up in the cell table
+        // (calculateParticleCollisions, src/simulation/particle_collisions.cuh:104-269).
         {
             const GridGeom& g = A.pg;
             int Tl = T;
             if (tt.nt > 1) Tl = type_of_index(tt, A.p_slot[i]);
             const float r1 = A.bs[model_index(tt, Tl, i, fl)];
-            uint32_t dummy = 0;
-            const uint32_t cellId = cell_key(g, x1.x, x1.y, x1.z, dummy);
+            // the cell the grid build gave i (gridCellIds of its slot); the face
+            // clipping below uses the current position, as the reference does
+            const uint32_t cellId = A.p_key_of[i];
             const int32_t xId = sat_i32((x1.x - g.minv[0]) / g.cell[0]);
             const int32_t yId = sat_i32((x1.y - g.minv[1]) / g.cell[1]);
             const int32_t zId = sat_i32((x1.z - g.minv[2]) / g.cell[2]);
@@ -450,7 +452,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
             axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
             const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-            uint32_t mask = neighbour_mask(A.p_map, g, cellId, nr, cX, cXY, A.p_key_of[i] == cellId);
+            uint32_t mask = neighbour_mask(A.p_map, g, cellId, nr, cX, cXY, true);
             while (mask) {
                 const int r = __builtin_ctz(mask);
                 mask &= mask - 1u;
@@ -463,11 +465,13 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
                     const int32_t j = A.p_perm[u];
                     if (j == i) continue;
                     const float r2 = A.bs[model_index(tt, Tl, j, fl)];
-                    const f3 rel = sub(x1, load3(A.pos, np, j));
+                    const float4 q = A.px4[j];
+                    const f3 rel = sub(x1, mk(q.x, q.y, q.z));
                     const float d2 = dot(rel, rel);
                     const float minD = r1 + r2;
                     if (d2 <= minD * minD && d2 >= 0.0001f) {
-                        const f3 rv = sub(v1, load3(A.vel, np, j));
+                        const float4 w = A.vx4[j];
+                        const f3 rv = sub(v1, mk(w.x, w.y, w.z));
                         F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
                     }
                 }
@@ -1056,6 +1060,18 @@ __global__ void __launch_bounds__(256) k_contacts(StepArgs A, int pass, int32_t*
     if (pass == 0) counts[i] = c;
 }
 
+// Packed copies from the current state (when particles moved after the last
+// grid build).
+__global__ void __launch_bounds__(256) k_refresh_copies(const float* __restrict__ pos, const float* __restrict__ vel,
+                                                        int32_t n, float4* __restrict__ px4, float4* __restrict__ vx4)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 x = load3(pos, n, i), v = load3(vel, n, i);
+    px4[i] = make_float4(x.x, x.y, x.z, 0.0f);
+    vx4[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
 // Gather of cell-table starts (parity views of the grid).
 __global__ void __launch_bounds__(256) k_gather_start(const uint2* __restrict__ table, const uint32_t* __restrict__ idx,
                                                       int32_t n, uint32_t* __restrict__ out)
@@ -1085,6 +1101,11 @@ int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, u
     if (rc) return rc;
     hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, st, tt.nc, okey, ocount, ostart, corder, flags);
     return BCS_OK;
+}
+
+void launch_refresh_copies(const float* pos, const float* vel, int32_t n, float4* px4, float4* vx4, hipStream_t st)
+{
+    if (n > 0 && px4) hipLaunchKernelGGL(k_refresh_copies, dim3((n + 255) / 256), dim3(256), 0, st, pos, vel, n, px4, vx4);
 }
 
 void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st)

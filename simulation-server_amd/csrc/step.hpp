@@ -62,7 +62,9 @@ struct StepArgs {
     const int32_t* p_slot;
     const uint2* p_table;   // {start, count} per cell (current where occupied in p_map)
     BrickMap p_map;         // 2-bit brick occupancy map of the particle grid
-    const uint32_t* p_key_of;  // per particle: its cell key in the last build
+    const uint32_t* p_key_of;  // per particle: its cell key in the last build (gridCellIds by particle)
+    const float4* px4;      // per particle (x, y, z, 0) as of the last grid build
+    const float4* vx4;      // per particle (vx, vy, vz, 0)
     // triangle grid + vein
     GridGeom tg;
     const uint2* t_table;

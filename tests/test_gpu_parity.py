@@ -129,6 +129,59 @@ def test_phases_match_oracle(bcs, orc, kind):
         assert_bits_equal(V, oV, f"frame {frame} vein velocities")
 
 
+def test_stale_grid_uses_current_positions(bcs, orc):
+    """calculateNextFrame after propagateAll without a new grid: the old cells
+    are scanned with the moved particles, as in the reference."""
+    sc = _scene(bcs, "mixed")
+    p = sc.positions.reshape(3, -1).copy()
+    p[1] = p[1] * np.float32(0.2) - np.float32(30)
+    sc.positions = p.reshape(-1)
+    sim, o = _pair(bcs, orc, sc)
+    sim.calculate_grid(0)
+    o.calculate_grid(0)
+    o.calculate_grid(1)
+    for frame in range(3):
+        sim.calculate_next_frame()
+        o.calculate_next_frame()
+        sim.propagate_all()
+        o.propagate_all()
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "stale grid " + n)
+
+
+@pytest.mark.parametrize("history", ["fused", "phases"])
+def test_stale_grid_after_frames(bcs, orc, history):
+    """Two frames (fused or by phases), a phase frame, then calculateNextFrame
+    on the now stale grid: the search is centred on each particle's stored
+    cell (gridCellIds) and clipped by its current position, as in the
+    reference."""
+    sc = _scene(bcs, "mixed")
+    sim, o = _pair(bcs, orc, sc)
+    if history == "fused":
+        sim.step(2)
+        o.step(2)
+    else:
+        for _ in range(2):
+            sim.calculate_grid(0)
+            o.calculate_grid(0)
+            o.calculate_grid(1)
+            sim.calculate_next_frame()
+            o.calculate_next_frame()
+            sim.propagate_all()
+            o.propagate_all()
+    sim.calculate_grid(0)
+    o.calculate_grid(0)
+    o.calculate_grid(1)
+    sim.calculate_next_frame()
+    o.calculate_next_frame()
+    sim.propagate_all()
+    o.propagate_all()
+    sim.calculate_next_frame()
+    o.calculate_next_frame()
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, f"stale grid after {history} frames: {n}")
+
+
 def test_fused_100_steps_c0(bcs, orc):
     """Config-0 workload, 100 fused graph-replayed frames vs the oracle."""
     sc = _scene(bcs, "c0")
