@@ -375,6 +375,23 @@ static int choose_shift(uint64_t cell_count, int64_t cap)
         if (e_ != hipSuccess) return fail(BCS_ERR_HIP, std::string("grid: ") + hipGetErrorString(e_)); \
     } while (0)
 
+// Bucket arrays of a given shift: histogram (zeroed), offsets, scan tile
+// status and the big-bucket list.
+static int alloc_buckets(DeviceGrid& G, int shift, hipStream_t st)
+{
+    G.shift = shift;
+    G.nb = ((G.g.cell_count - 1) >> G.shift) + 1;
+    G.ntiles = (int32_t)((G.nb + 1 + kScanTile - 1) / kScanTile);
+    const size_t n = (size_t)std::max(G.cap, 1);
+    GRID_HIP(hipMalloc(&G.bcount, (G.nb + 1) * 4));
+    GRID_HIP(hipMemsetAsync(G.bcount, 0, (G.nb + 1) * 4, st));
+    GRID_HIP(hipMalloc(&G.bstart, (G.nb + 2) * 4));
+    GRID_HIP(hipMalloc(&G.status, (size_t)std::max(G.ntiles, 1) * 8));
+    G.big_cap = (uint32_t)std::min<uint64_t>(G.nb, (uint64_t)n / kRankThreshold + 1);
+    GRID_HIP(hipMalloc(&G.biglist, (size_t)(G.big_cap + 1) * 4));
+    return BCS_OK;
+}
+
 int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot, bool with_runs, bool with_map,
                bool with_copies, hipStream_t st)
 {
@@ -385,8 +402,6 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
         return fail(BCS_ERR_GRID_TOO_LARGE, "grid: cell count " + std::to_string(geom.cell_count) +
                                                 " outside [1, 2^32) (choose a larger cell size)");
     G.shift = choose_shift(geom.cell_count, cap);
-    G.nb = ((geom.cell_count - 1) >> G.shift) + 1;
-    G.ntiles = (int32_t)((G.nb + 1 + kScanTile - 1) / kScanTile);
     const size_t n = (size_t)std::max(cap, 1);
     GRID_HIP(hipMalloc(&G.keys, n * 4));
     GRID_HIP(hipMalloc(&G.items_k, n * 4));
@@ -421,15 +436,66 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
         GRID_HIP(hipMalloc(&G.gpos, n * 4));
         GRID_HIP(hipMemsetAsync(G.gpos, 0xFF, n * 4, st));
     }
-    GRID_HIP(hipMalloc(&G.bcount, (G.nb + 1) * 4));
-    GRID_HIP(hipMemsetAsync(G.bcount, 0, (G.nb + 1) * 4, st));
-    GRID_HIP(hipMalloc(&G.bstart, (G.nb + 2) * 4));
-    GRID_HIP(hipMalloc(&G.status, (size_t)std::max(G.ntiles, 1) * 8));
     GRID_HIP(hipMalloc(&G.ctrl, 16 * 4));
     GRID_HIP(hipMemsetAsync(G.ctrl, 0, 16 * 4, st));
-    G.big_cap = (uint32_t)std::min<uint64_t>(G.nb, (uint64_t)n / kRankThreshold + 1);
-    GRID_HIP(hipMalloc(&G.biglist, (size_t)(G.big_cap + 1) * 4));
-    return BCS_OK;
+    return alloc_buckets(G, G.shift, st);
+}
+
+int grid_set_shift(DeviceGrid& G, int shift, hipStream_t st)
+{
+    if (shift == G.shift) return BCS_OK;
+    GRID_HIP(hipStreamSynchronize(st));  // no build in flight uses the old arrays
+    void* old[] = {G.bcount, G.bstart, G.status, G.biglist};
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    G.bcount = G.bstart = G.biglist = nullptr;
+    G.status = nullptr;
+    return alloc_buckets(G, shift, st);
+}
+
+int grid_shift_for(const GridGeom& g, const float* pos, int64_t stride, int32_t n)
+{
+    // Sample up to 2^20 objects (every k-th), form their keys on the host,
+    // and pick the coarsest bucket (2^s cells) whose object-weighted mean size
+    // sum(c^2)/sum(c) -- the bucket an object typically sorts in -- stays
+    // <= 32 objects; the sampling bias of sum(c^2) is removed
+    // (E[c_s^2] = f^2 c^2 + f(1 - f) c for a fraction f).  Also keeps the
+    // bucket count <= 2^27.
+    if (n <= 0 || !pos) return choose_shift(g.cell_count, n);
+    const int64_t step = std::max<int64_t>(1, (int64_t)n >> 20);
+    std::vector<uint32_t> keys;
+    keys.reserve((size_t)(n / step + 1));
+    for (int64_t i = 0; i < n; i += step) {
+        int64_t id[3];
+        for (int d = 0; d < 3; ++d) {
+            const float q = (pos[d * stride + i] - g.minv[d]) / g.cell[d];
+            const float m = q > 0.0f ? q : 0.0f;
+            const float c = m < g.extent[d] ? m : g.extent[d];
+            id[d] = (c == c && c < 2.0e9f) ? (int64_t)c : 0;
+        }
+        int64_t k = (id[2] * g.cnt[1] + id[1]) * g.cnt[0] + id[0];
+        if (k < 0 || (uint64_t)k >= g.cell_count) k = (int64_t)g.cell_count - 1;
+        keys.push_back((uint32_t)k);
+    }
+    std::sort(keys.begin(), keys.end());
+    const double f = (double)keys.size() / (double)n;
+    int best = 0;
+    for (int sh = 0; sh <= 24; ++sh) {
+        if (((g.cell_count - 1) >> sh) + 1 > (1ull << 27)) continue;  // too many buckets
+        double s1 = 0, s2 = 0;
+        for (size_t a = 0; a < keys.size();) {
+            size_t b = a + 1;
+            while (b < keys.size() && (keys[b] >> sh) == (keys[a] >> sh)) ++b;
+            const double c = (double)(b - a);
+            s1 += c;
+            s2 += c * c;
+            a = b;
+        }
+        const double weighted = (s2 - (1.0 - f) * s1) / std::max(f * s1, 1e-30);
+        if (weighted <= 32.0) best = sh;
+        else break;
+    }
+    return best;
 }
 
 void grid_free(DeviceGrid& G)

@@ -59,6 +59,11 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
                hipStream_t st, const KernelTimer* timer = nullptr);
 constexpr int kGridKernels = 6;
 
+// Bucket size from a host sample of the objects (SoA, stride apart), and the
+// switch to another bucket size (waits for the stream; buckets start empty).
+int grid_shift_for(const GridGeom& g, const float* pos, int64_t stride, int32_t n);
+int grid_set_shift(DeviceGrid& G, int shift, hipStream_t st);
+
 int scan_exclusive(const uint32_t* count, uint32_t* out, int64_t m, uint64_t* status, int32_t ntiles, uint32_t* ctrl,
                    uint32_t* flags, hipStream_t st);
 int scan_tiles(int64_t m);

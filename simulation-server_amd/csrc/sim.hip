@@ -553,12 +553,21 @@ static int copy_out(bcs_sim* s, float* dst, const float* src, size_t n)
     return BCS_OK;
 }
 
+static void drop_graph(bcs_sim* s);
+
 extern "C" int bcs_sim_set_particles(bcs_sim* s, const float* p, const float* v, const float* f)
 {
     RC(set_dev(s));
     const size_t n = 3 * (size_t)s->np;
     RC(copy_in(s, s->pos, p, n)); RC(copy_in(s, s->vel, v, n)); RC(copy_in(s, s->frc, f, n));
     if (p || v) s->copies_current = false;
+    if (p) {  // bucket size of the particle sort from where the particles are
+        const int sh = grid_shift_for(s->pg.g, p, s->np, s->np);
+        if (sh != s->pg.shift) {
+            drop_graph(s);
+            RC(grid_set_shift(s->pg, sh, s->stream));
+        }
+    }
     HIPCHK(hipStreamSynchronize(s->stream));
     return BCS_OK;
 }
@@ -647,6 +656,14 @@ extern "C" int bcs_sim_propagate_all(bcs_sim* s)
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     return BCS_OK;
+}
+
+static void drop_graph(bcs_sim* s)
+{
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
 }
 
 static int ensure_graph(bcs_sim* s)
