@@ -90,7 +90,10 @@ typedef struct bcs_physics {
     int32_t dynamic_triangle_grid;     /* 0: triangle centres frozen after creation
                                           (reference single-GPU behaviour, src/main.cu:164-176);
                                           1: recompute centres + triangle grid every frame */
-    int32_t reserved;
+    int32_t ray_work_items;            /* ray work-list capacity in items: 0 = 8 per particle (at
+                                          least 65,536 in all and 1,024 per cell block); n > 0 = n in
+                                          all.  Rays that do not fit are scanned serially (same
+                                          result, BCS_FLAG_RAY_SERIAL). */
 } bcs_physics;
 
 /* ---- blood-cell type ---------------------------------------------------

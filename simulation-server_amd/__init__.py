@@ -74,7 +74,7 @@ class Physics(C.Structure):
         ("cylinder_radius", C.c_float),
         ("seed", C.c_uint64),
         ("dynamic_triangle_grid", C.c_int32),
-        ("reserved", C.c_int32),
+        ("ray_work_items", C.c_int32),
     ]
 
 

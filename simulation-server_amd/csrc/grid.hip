@@ -30,7 +30,7 @@
 namespace bcs {
 
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 16;  // per thread, loaded and stored as uint4
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kRankThreshold = 64;
 constexpr uint32_t kSortLds = 4096;  // bucket size sorted in LDS by k_big
@@ -101,10 +101,23 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restric
     const int64_t base = tile * kScanTile + (int64_t)tid * kScanItems;
     uint32_t v[kScanItems];
     uint32_t sum = 0;
+    if (base + kScanItems <= m) {  // whole chunk in range: 16-byte loads (base is 64-byte aligned)
+        const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const uint4 t = c4[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) v[k] = (base + k < m) ? count[base + k] : 0u;
+    }
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         int64_t i = base + k;
-        v[k] = (i < m) ? count[i] : 0u;
         if (biglist && i < m && v[k] > big_thr) {
             uint32_t slot = atomicAdd(&ctrl[1], 1u);
             biglist[slot] = (uint32_t)i;
@@ -157,11 +170,20 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restric
     }
     __syncthreads();
     uint32_t run = s_prefix + wave_off + (inc - sum);
+    uint32_t o[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        int64_t i = base + k;
-        if (i <= m) out[i] = run;
+        o[k] = run;
         run += v[k];
+    }
+    if (base + kScanItems <= m + 1) {  // out holds m + 1 entries
+        uint4* o4 = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) o4[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k <= m) out[base + k] = o[k];
     }
 }
 

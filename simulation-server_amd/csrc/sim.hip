@@ -419,8 +419,10 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     // ray work list: 8 items per particle, split evenly over the k_cells_a blocks
     s->nblocks = 8 * ((s->nchunks + 7) / 8);
     {
-        const uint64_t total = std::max<uint64_t>(8 * N, 65536);
-        const uint64_t per = std::max<uint64_t>(total / std::max(s->nblocks, 1), 1024);
+        const int32_t items = phys->ray_work_items;
+        const uint64_t nb = (uint64_t)std::max(s->nblocks, 1);
+        const uint64_t per = items > 0 ? std::max<uint64_t>((uint64_t)items / nb, 1)
+                                       : std::max<uint64_t>(std::max<uint64_t>(8 * N, 65536) / nb, 1024);
         s->region_cap = (uint32_t)std::min<uint64_t>(per, 0x7FFFFFFFull);
     }
     RC(s->dalloc(&s->best, N));

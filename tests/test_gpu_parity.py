@@ -215,6 +215,22 @@ def test_fused_1000_steps_c0_bit_exact(bcs, orc):
     assert sim.get_info().step_index == 1000
 
 
+def test_ray_work_overflow_falls_back_to_serial_scan(bcs, orc):
+    """A work list of 10 items per cell block overflows near the wall: those
+    rays are scanned serially in k_cells_b, with the same result."""
+    sc = _scene(bcs, "c0")
+    ph = sc.physics
+    ph.ray_work_items = 240
+    sim = bcs.Simulation.from_scene(sc, physics=ph)
+    o = orc.Oracle(ph, sc.types, sc.mesh)
+    o.set_particles(sc.positions, sc.velocities, sc.forces)
+    sim.step(60)
+    o.step(60)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "serial ray fallback " + n)
+    assert sim.error_flags() & bcs.FLAG_RAY_SERIAL
+
+
 def test_fused_40_steps_mixed_types(bcs, orc):
     """Two cell types (discs and icosahedra): 40 fused frames, across two
     refreshes of the spatial processing order of the cells."""
