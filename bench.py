@@ -54,8 +54,9 @@ def step_bytes(np_, n_occ, nc, nv, nt):
 def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
     """Algorithmic HBM bytes of one launch of each frame kernel (every datum
     counted once, as a perfect cache would move it):
-      k_key        x read 12, key written 4, previous map-word slot read 4
-                   (the words it names cleared: counted in k_occ)  20 / particle
+      k_key        x, v read 24, key written 4, packed (x, v) copies written 32,
+                   previous map-word slot read 4 (the words it names cleared:
+                   counted in k_occ)                               64 / particle
       k_scatter    key read 4, (key, index) written 8              12 / particle
       k_rank       (key, index) read 8, sorted key, index, inverse written 12
                    + {start, count} per occupied cell 8           20 / particle + 8 / cell
@@ -77,7 +78,7 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
       k_vein       X, V, Fv read 36, X, V, Fv written 36, offsets 4
                                                                    76 / vertex"""
     table = {
-        "k_key": 20 * np_,
+        "k_key": 64 * np_,
         "k_scatter": 12 * np_,
         "k_rank": 20 * np_ + 8 * n_occ,
         "k_occ": 8 * np_ + 10 * n_occ,

@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restric
 
 __global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ keys, int32_t n, int shift,
                                                  const uint32_t* __restrict__ bstart, uint32_t* __restrict__ bcount,
-                                                 uint32_t* __restrict__ items_k, int32_t* __restrict__ items_i,
+                                                 uint2* __restrict__ items,
                                                  uint32_t* __restrict__ flags)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,8 +203,7 @@ __global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ ke
         atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);
         return;
     }
-    items_k[pos] = k;
-    items_i[pos] = i;
+    items[pos] = make_uint2(k, (uint32_t)i);
 }
 
 struct RankOut {
@@ -228,13 +227,14 @@ __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32
     }
 }
 
-__global__ void __launch_bounds__(256) k_rank(const uint32_t* __restrict__ items_k, const int32_t* __restrict__ items_i,
+__global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items,
                                               int32_t n, int shift, const uint32_t* __restrict__ bstart, RankOut o)
 {
     const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const uint32_t key = items_k[q];
-    const int32_t idx = items_i[q];
+    const uint2 me = items[q];
+    const uint32_t key = me.x;
+    const int32_t idx = (int32_t)me.y;
     const uint32_t b = key >> shift;
     const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
     if (s1 < s0 || s1 > (uint32_t)n) return;  // inconsistent offsets (flagged by k_scatter)
@@ -242,12 +242,13 @@ __global__ void __launch_bounds__(256) k_rank(const uint32_t* __restrict__ items
     uint32_t rank = 0, same = 0;
     bool first = true;
     for (uint32_t u = s0; u < s1; ++u) {
-        const uint32_t k2 = items_k[u];
+        const uint2 it = items[u];
+        const uint32_t k2 = it.x;
         if (k2 < key) {
             ++rank;
         } else if (k2 == key) {
             ++same;
-            if (items_i[u] < idx) {
+            if ((int32_t)it.y < idx) {
                 ++rank;
                 first = false;
             }
@@ -258,7 +259,7 @@ __global__ void __launch_bounds__(256) k_rank(const uint32_t* __restrict__ items
 
 constexpr int kBigChunk = 1024;
 
-__global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_k, const int32_t* __restrict__ items_i,
+__global__ void __launch_bounds__(256) k_big(const uint2* __restrict__ items,
                                              int32_t n, int shift, const uint32_t* __restrict__ bstart,
                                              const uint32_t* __restrict__ biglist, const uint32_t* __restrict__ ctrl,
                                              RankOut o)
@@ -278,7 +279,7 @@ __global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_
             uint32_t P = 1;
             while (P < m) P <<= 1;
             for (uint32_t t = tid; t < P; t += blockDim.x)
-                sv[t] = t < m ? ((uint64_t)items_k[s0 + t] << 32) | (uint32_t)items_i[s0 + t] : ~0ull;
+                sv[t] = t < m ? ((uint64_t)items[s0 + t].x << 32) | items[s0 + t].y : ~0ull;
             __syncthreads();
             for (uint32_t k = 2; k <= P; k <<= 1) {
                 for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -312,16 +313,18 @@ __global__ void __launch_bounds__(256) k_big(const uint32_t* __restrict__ items_
         for (uint32_t gbase = s0; gbase < s1; gbase += blockDim.x) {
             const uint32_t q = gbase + tid;
             const bool own = q < s1;
-            const uint32_t key = own ? items_k[q] : 0u;
-            const int32_t idx = own ? items_i[q] : 0;
+            const uint2 me = own ? items[q] : make_uint2(0u, 0u);
+            const uint32_t key = me.x;
+            const int32_t idx = (int32_t)me.y;
             uint32_t rank = 0, same = 0;
             bool first = true;
             for (uint32_t cbase = s0; cbase < s1; cbase += kBigChunk) {
                 const uint32_t cn = min((uint32_t)kBigChunk, s1 - cbase);
                 __syncthreads();
                 for (uint32_t t = tid; t < cn; t += blockDim.x) {
-                    sk[t] = items_k[cbase + t];
-                    si[t] = items_i[cbase + t];
+                    const uint2 it = items[cbase + t];
+                    sk[t] = it.x;
+                    si[t] = (int32_t)it.y;
                 }
                 __syncthreads();
                 if (own) {
@@ -426,8 +429,7 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     G.shift = choose_shift(geom.cell_count, cap);
     const size_t n = (size_t)std::max(cap, 1);
     GRID_HIP(hipMalloc(&G.keys, n * 4));
-    GRID_HIP(hipMalloc(&G.items_k, n * 4));
-    GRID_HIP(hipMalloc(&G.items_i, n * 4));
+    GRID_HIP(hipMalloc(&G.items, n * 8));
     GRID_HIP(hipMalloc(&G.keys_sorted, n * 4));
     if (with_runs) GRID_HIP(hipMalloc(&G.runs, n * 4));
     if (with_copies) {
@@ -522,7 +524,7 @@ int grid_shift_for(const GridGeom& g, const float* pos, int64_t stride, int32_t 
 
 void grid_free(DeviceGrid& G)
 {
-    void* ptrs[] = {G.keys, G.items_k, G.items_i, G.keys_sorted, G.runs, G.perm, G.slot_of, G.px4, G.vx4,
+    void* ptrs[] = {G.keys, G.items, G.keys_sorted, G.runs, G.perm, G.slot_of, G.px4, G.vx4,
                     G.table, G.bm.words, G.gpos, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -551,15 +553,15 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
                        G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold, flags);
     te(1);
     tb(2);
-    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.bcount, G.items_k,
-                       G.items_i, flags);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.bcount, G.items,
+                       flags);
     te(2);
     RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2};
     tb(3);
-    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items_k, G.items_i, n, G.shift, G.bstart, o);
+    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items, n, G.shift, G.bstart, o);
     te(3);
     tb(4);
-    hipLaunchKernelGGL(k_big, dim3(std::max(1u, std::min(G.big_cap, 4096u))), dim3(256), 0, st, G.items_k, G.items_i, n,
+    hipLaunchKernelGGL(k_big, dim3(std::max(1u, std::min(G.big_cap, 4096u))), dim3(256), 0, st, G.items, n,
                        G.shift, G.bstart, G.biglist, G.ctrl, o);
     te(4);
     tb(5);

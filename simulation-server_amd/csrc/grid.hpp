@@ -29,8 +29,7 @@ struct DeviceGrid {
     int32_t ntiles = 0;         // scan tiles over nb+1 entries
     uint32_t big_cap = 0;
     uint32_t* keys = nullptr;         // per object, input order (last build; clears its cells next build)
-    uint32_t* items_k = nullptr;      // bucketed (key, index)
-    int32_t* items_i = nullptr;
+    uint2* items = nullptr;           // bucketed (key, index)
     uint32_t* keys_sorted = nullptr;  // gridCellIds after the sort
     int32_t* perm = nullptr;          // particleIds after the sort
     int32_t* slot_of = nullptr;       // inverse of perm (optional)
