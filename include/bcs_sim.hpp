@@ -24,6 +24,7 @@
 #define BCS_SIM_HPP
 
 #include "bcs.h"
+#include "bcs_refconfig.h"
 
 #include <algorithm>
 #include <cstdint>
@@ -225,6 +226,55 @@ private:
     int32_t which = BCS_GRID_PARTICLES;
     bcs::Vec3View attachedTo;
 };
+
+namespace bcs {
+
+/* Cell types (definition order) and vein mesh read from configuration headers
+ * in the reference's format (src/config/blood_cells_definition.hpp,
+ * blood_cell_presets.hpp, vein_definition.hpp); BloodCells(types) folds them
+ * like the reference. */
+struct ReferenceConfig {
+    std::vector<CellTypeDef> types;
+    std::vector<float> veinPositions, endingCenters, endingRadii;
+    std::vector<uint32_t> veinIndices;
+    VeinTriangles vein() const { return VeinTriangles(veinPositions, veinIndices, endingCenters, endingRadii); }
+};
+
+inline ReferenceConfig loadReferenceConfig(const char* definitions, const char* presets, const char* vein)
+{
+    bcs_ref_config* c = nullptr;
+    check(bcs_ref_config_load(definitions, presets, vein, &c), "bcs_ref_config_load");
+    ReferenceConfig out;
+    int32_t n = 0;
+    const bcs_cell_type* t = nullptr;
+    bcs_vein_mesh m{};
+    const int rc = bcs_ref_config_get(c, &n, &t, &m);
+    if (rc == BCS_OK) {
+        for (int32_t k = 0; k < n; ++k) {
+            CellTypeDef d;
+            d.count = t[k].count;
+            d.particlesInCell = t[k].particles_in_cell;
+            d.springStart.assign(t[k].spring_start, t[k].spring_start + t[k].spring_count);
+            d.springEnd.assign(t[k].spring_end, t[k].spring_end + t[k].spring_count);
+            d.springLength.assign(t[k].spring_length, t[k].spring_length + t[k].spring_count);
+            d.vertices.assign(t[k].vertices, t[k].vertices + 3 * t[k].particles_in_cell);
+            out.types.push_back(d);
+        }
+        if (m.vertex_count > 0) {
+            out.veinPositions.assign(m.positions, m.positions + 3 * (size_t)m.vertex_count);
+            out.veinIndices.assign(m.indices, m.indices + 3 * (size_t)m.triangle_count);
+            if (m.ending_count > 0) {
+                out.endingCenters.assign(m.ending_centers, m.ending_centers + 3 * (size_t)m.ending_count);
+                out.endingRadii.assign(m.ending_radii, m.ending_radii + m.ending_count);
+            }
+        }
+    }
+    bcs_ref_config_destroy(c);
+    check(rc, "bcs_ref_config_get");
+    return out;
+}
+
+}  // namespace bcs
 
 namespace sim {
 

@@ -230,6 +230,9 @@ def _sig(lib):
     lib.bcs_scene_get_view.argtypes = [vp, P(SceneView)]
     lib.bcs_scene_destroy.argtypes = [vp]
     lib.bcs_scene_lumen_sdf.argtypes = [vp, f32p, C.c_int32, f32p]
+    lib.bcs_ref_config_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, P(vp)]
+    lib.bcs_ref_config_get.argtypes = [vp, i32p, P(P(CellType)), P(VeinMesh)]
+    lib.bcs_ref_config_destroy.argtypes = [vp]
     return lib
 
 
@@ -517,6 +520,46 @@ class Simulation:
         f = C.c_uint32(0)
         check(self.lib.bcs_sim_error_flags(self.h, 1 if clear else 0, C.byref(f)), "error_flags")
         return f.value
+
+
+def load_reference_config(definitions=None, presets=None, vein=None):
+    """Cell types (definition order) and vein mesh from configuration headers
+    in the reference's format (src/config/blood_cells_definition.hpp,
+    blood_cell_presets.hpp, vein_definition.hpp).  Returns (types, mesh);
+    either is None when its files were not given."""
+    lib = load_library()
+    enc = lambda p: None if p is None else str(p).encode()
+    h = C.c_void_p()
+    check(lib.bcs_ref_config_load(enc(definitions), enc(presets), enc(vein), C.byref(h)), "bcs_ref_config_load")
+    try:
+        n = C.c_int32(0)
+        tp = C.POINTER(CellType)()
+        m = VeinMesh()
+        check(lib.bcs_ref_config_get(h, C.byref(n), C.byref(tp), C.byref(m)), "bcs_ref_config_get")
+        types = None
+        if definitions is not None:
+            types = []
+            for k in range(n.value):
+                t = tp[k]
+                ppc, ns = t.particles_in_cell, t.spring_count
+                verts = np.ctypeslib.as_array(t.vertices, shape=(3 * ppc,)).copy()
+                sa = np.ctypeslib.as_array(t.spring_start, shape=(ns,)).copy()
+                sb = np.ctypeslib.as_array(t.spring_end, shape=(ns,)).copy()
+                sl = np.ctypeslib.as_array(t.spring_length, shape=(ns,)).copy()
+                spec = CellTypeSpec(t.count, verts, np.stack([sa, sb, sl], 1), 0.0)
+                spec.s_len = sl.astype(np.float32)
+                types.append(spec)
+        mesh = None
+        if vein is not None:
+            pos = np.ctypeslib.as_array(m.positions, shape=(3 * m.vertex_count,)).copy()
+            idx = np.ctypeslib.as_array(m.indices, shape=(3 * m.triangle_count,)).copy()
+            ne = m.ending_count
+            endc = np.ctypeslib.as_array(m.ending_centers, shape=(3 * ne,)).copy() if ne else []
+            endr = np.ctypeslib.as_array(m.ending_radii, shape=(ne,)).copy() if ne else []
+            mesh = MeshSpec(pos, idx, endc, endr)
+        return types, mesh
+    finally:
+        lib.bcs_ref_config_destroy(h)
 
 
 def calibration_copy(n_bytes: int, repeats: int = 10) -> float:
