@@ -282,6 +282,31 @@ int bcs_sim_get_contacts(bcs_sim* sim, int32_t* counts, int32_t* pairs_j, int64_
  * <= 16 whose cell box the ray can reach) and particle-vein hits. */
 int bcs_sim_frame_stats(bcs_sim* sim, uint32_t* ray_work_items, uint32_t* vein_hits);
 
+/* Frame export in the renderer's buffer layouts
+ * (calculatePositionsKernel / calculateTriangleVerticesKernel,
+ * src/graphics/glcontroller.cu:23-50, 168-202):
+ *   offsets            3 floats per particle, xyz interleaved (all particles);
+ *   type_vertices[t]   6 floats per particle of type t, xyz at 0..2 of each
+ *                      group of 6 (3..5, the normals, are left untouched);
+ *   vein_vertices      6 floats per vein vertex, xyz at 0..2 (3..5 untouched).
+ * All are DEVICE pointers (e.g. buffers a renderer mapped for HIP); any may be
+ * NULL.  Enqueued on the simulation's stream: asynchronous. */
+int bcs_sim_export_frame(bcs_sim* sim, float* offsets, float* const* type_vertices, int32_t n_types,
+                         float* vein_vertices);
+
+/* Trajectory files (no reference counterpart: the reference only renders).
+ * Layout, little endian: a 64-byte header
+ *   char magic[8] = "BCSTRAJ1"; int32 particles, vein_vertices, types, flags
+ *   (bit 0: velocities stored); float dt; int32 reserved[9]
+ * then per frame: uint64 step index, particle positions (SoA, 3N f32),
+ * [particle velocities (3N f32)], vein positions (SoA, 3Nv f32).
+ * bcs_traj_append copies the current state from the device (synchronises the
+ * stream) and appends one frame. */
+typedef struct bcs_traj bcs_traj;
+int bcs_traj_open(bcs_sim* sim, const char* path, int32_t with_velocities, bcs_traj** out);
+int bcs_traj_append(bcs_traj* traj);
+int bcs_traj_close(bcs_traj* traj);
+
 /* Device error word (BCS_FLAG_*); clear != 0 resets it. */
 int bcs_sim_error_flags(bcs_sim* sim, int32_t clear, uint32_t* flags);
 

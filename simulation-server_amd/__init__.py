@@ -230,6 +230,10 @@ def _sig(lib):
     lib.bcs_scene_get_view.argtypes = [vp, P(SceneView)]
     lib.bcs_scene_destroy.argtypes = [vp]
     lib.bcs_scene_lumen_sdf.argtypes = [vp, f32p, C.c_int32, f32p]
+    lib.bcs_sim_export_frame.argtypes = [vp, vp, P(vp), C.c_int32, vp]
+    lib.bcs_traj_open.argtypes = [vp, C.c_char_p, C.c_int32, P(vp)]
+    lib.bcs_traj_append.argtypes = [vp]
+    lib.bcs_traj_close.argtypes = [vp]
     lib.bcs_ref_config_load.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, P(vp)]
     lib.bcs_ref_config_get.argtypes = [vp, i32p, P(P(CellType)), P(VeinMesh)]
     lib.bcs_ref_config_destroy.argtypes = [vp]
@@ -516,10 +520,73 @@ class Simulation:
         check(self.lib.bcs_sim_frame_stats(self.h, C.byref(w), C.byref(h)), "frame_stats")
         return {"ray_work_items": w.value, "vein_hits": h.value}
 
+    def export_frame(self, offsets=0, type_vertices=(), vein_vertices=0):
+        """Frame export into DEVICE buffers given as integer addresses (e.g.
+        torch tensor data_ptr()); asynchronous on the simulation's stream."""
+        tv = (C.c_void_p * max(len(type_vertices), 1))(*[C.c_void_p(int(a)) for a in type_vertices])
+        check(self.lib.bcs_sim_export_frame(self.h, C.c_void_p(int(offsets) or None), tv, len(type_vertices),
+                                            C.c_void_p(int(vein_vertices) or None)), "export_frame")
+
+    def open_trajectory(self, path, with_velocities=False):
+        return Trajectory(self, path, with_velocities)
+
     def error_flags(self, clear=False) -> int:
         f = C.c_uint32(0)
         check(self.lib.bcs_sim_error_flags(self.h, 1 if clear else 0, C.byref(f)), "error_flags")
         return f.value
+
+
+class Trajectory:
+    """Appends frames of a simulation to a trajectory file (bcs_traj_*)."""
+
+    def __init__(self, sim, path, with_velocities=False):
+        self.lib = sim.lib
+        self.h = C.c_void_p()
+        check(self.lib.bcs_traj_open(sim.h, str(path).encode(), 1 if with_velocities else 0, C.byref(self.h)),
+              "traj_open")
+
+    def append(self):
+        check(self.lib.bcs_traj_append(self.h), "traj_append")
+
+    def close(self):
+        if self.h:
+            check(self.lib.bcs_traj_close(self.h), "traj_close")
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def read_trajectory(path):
+    """Reads a trajectory file: dict with particles, vein_vertices, types, dt,
+    steps[F] (uint64) and positions[F, 3, N], velocities[F, 3, N] or None,
+    vein[F, 3, Nv] (float32)."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    if raw.size < 64 or raw[:8].tobytes() != b"BCSTRAJ1":
+        raise ValueError(f"{path}: not a BCSTRAJ1 trajectory")
+    n, nv, nt, flags = np.frombuffer(raw[8:24].tobytes(), dtype="<i4")
+    dt = float(np.frombuffer(raw[24:28].tobytes(), dtype="<f4")[0])
+    vel = bool(flags & 1)
+    frame = 8 + 4 * (3 * n * (2 if vel else 1) + 3 * nv)
+    body = raw[64:]
+    if body.size % frame:
+        raise ValueError(f"{path}: truncated frame")
+    nf = body.size // frame
+    fr = body.reshape(nf, frame)
+    steps = np.frombuffer(fr[:, :8].tobytes(), dtype="<u8")
+    f32 = np.frombuffer(fr[:, 8:].tobytes(), dtype="<f4").reshape(nf, -1)
+    pos = f32[:, : 3 * n].reshape(nf, 3, n)
+    o = 3 * n
+    velo = None
+    if vel:
+        velo = f32[:, o: o + 3 * n].reshape(nf, 3, n)
+        o += 3 * n
+    vein = f32[:, o: o + 3 * nv].reshape(nf, 3, nv)
+    return {"particles": int(n), "vein_vertices": int(nv), "types": int(nt), "dt": dt, "steps": steps,
+            "positions": pos, "velocities": velo, "vein": vein}
 
 
 def load_reference_config(definitions=None, presets=None, vein=None):

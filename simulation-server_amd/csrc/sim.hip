@@ -10,6 +10,7 @@
 #include "step.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -31,6 +32,7 @@ void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uin
 void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
                     float4* aabb, hipStream_t st);
 void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st);
+void launch_export(const float* soa, int64_t n, int64_t first, int64_t count, int stride, float* out, hipStream_t st);
 void launch_refresh_copies(const float* pos, const float* vel, int32_t n, float4* px4, float4* vx4, hipStream_t st);
 int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
                  uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
@@ -861,6 +863,80 @@ extern "C" int bcs_sim_get_contacts(bcs_sim* s, int32_t* counts, int32_t* pairs_
     if (doffs) (void)hipFree(doffs);
     if (dpairs) (void)hipFree(dpairs);
     return rc;
+}
+
+extern "C" int bcs_sim_export_frame(bcs_sim* s, float* offsets, float* const* type_vertices, int32_t n_types,
+                                    float* vein_vertices)
+{
+    RC(set_dev(s));
+    if (n_types < 0 || (n_types > 0 && !type_vertices)) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_export_frame: types");
+    if (offsets) launch_export(s->pos, s->np, 0, s->np, 3, offsets, s->stream);
+    for (int32_t t = 0; t < std::min<int32_t>(n_types, s->tt.nt); ++t)
+        if (type_vertices[t])
+            launch_export(s->pos, s->np, s->tt.pstart[t], (int64_t)s->tt.cnt[t] * s->tt.ppc[t], 6, type_vertices[t],
+                          s->stream);
+    if (vein_vertices) launch_export(s->X, s->nv, 0, s->nv, 6, vein_vertices, s->stream);
+    HIPCHK(hipGetLastError());
+    return BCS_OK;
+}
+
+struct bcs_traj {
+    bcs_sim* sim = nullptr;
+    std::FILE* f = nullptr;
+    bool vel = false;
+    std::vector<float> p, v, X;
+};
+
+extern "C" int bcs_traj_open(bcs_sim* s, const char* path, int32_t with_velocities, bcs_traj** out)
+{
+    if (!s || !path || !out) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_traj_open: null argument");
+    *out = nullptr;
+    std::unique_ptr<bcs_traj> t(new bcs_traj());
+    t->sim = s;
+    t->vel = with_velocities != 0;
+    t->f = std::fopen(path, "wb");
+    if (!t->f) return fail(BCS_ERR_INVALID_ARGUMENT, std::string("bcs_traj_open: cannot write ") + path);
+    char head[64];
+    std::memset(head, 0, sizeof(head));
+    std::memcpy(head, "BCSTRAJ1", 8);
+    const int32_t ints[4] = {s->np, s->nv, s->tt.nt, t->vel ? 1 : 0};
+    std::memcpy(head + 8, ints, sizeof(ints));
+    const float dt = s->phys.dt;
+    std::memcpy(head + 24, &dt, 4);
+    if (std::fwrite(head, 1, 64, t->f) != 64) {
+        std::fclose(t->f);
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_traj_open: write failed");
+    }
+    t->p.resize(3 * (size_t)s->np);
+    if (t->vel) t->v.resize(3 * (size_t)s->np);
+    t->X.resize(3 * (size_t)s->nv);
+    *out = t.release();
+    return BCS_OK;
+}
+
+extern "C" int bcs_traj_append(bcs_traj* t)
+{
+    if (!t || !t->f) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_traj_append: closed trajectory");
+    bcs_sim* s = t->sim;
+    RC(bcs_sim_get_particles(s, t->p.data(), t->vel ? t->v.data() : nullptr, nullptr));
+    RC(bcs_sim_get_vein(s, t->X.data(), nullptr, nullptr));
+    bcs_sim_info info{};
+    RC(bcs_sim_get_info(s, &info));
+    const uint64_t step = info.step_index;
+    bool ok = std::fwrite(&step, 8, 1, t->f) == 1;
+    ok = ok && std::fwrite(t->p.data(), 4, t->p.size(), t->f) == t->p.size();
+    if (t->vel) ok = ok && std::fwrite(t->v.data(), 4, t->v.size(), t->f) == t->v.size();
+    ok = ok && std::fwrite(t->X.data(), 4, t->X.size(), t->f) == t->X.size();
+    if (!ok) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_traj_append: write failed");
+    return BCS_OK;
+}
+
+extern "C" int bcs_traj_close(bcs_traj* t)
+{
+    if (!t) return BCS_OK;
+    const bool ok = !t->f || std::fclose(t->f) == 0;
+    delete t;
+    return ok ? BCS_OK : fail(BCS_ERR_INVALID_ARGUMENT, "bcs_traj_close: close failed");
 }
 
 extern "C" int bcs_sim_frame_stats(bcs_sim* s, uint32_t* ray_work_items, uint32_t* vein_hits)

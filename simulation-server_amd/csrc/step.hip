@@ -1072,6 +1072,17 @@ __global__ void __launch_bounds__(256) k_refresh_copies(const float* __restrict_
     vx4[i] = make_float4(v.x, v.y, v.z, 0.0f);
 }
 
+// Frame export: SoA -> xyz interleaved with a given stride (3 or 6 floats).
+__global__ void __launch_bounds__(256) k_export(const float* __restrict__ soa, int64_t n, int64_t first, int64_t count,
+                                                int stride, float* __restrict__ out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    out[stride * r] = soa[first + r];
+    out[stride * r + 1] = soa[n + first + r];
+    out[stride * r + 2] = soa[2 * n + first + r];
+}
+
 // Gather of cell-table starts (parity views of the grid).
 __global__ void __launch_bounds__(256) k_gather_start(const uint2* __restrict__ table, const uint32_t* __restrict__ idx,
                                                       int32_t n, uint32_t* __restrict__ out)
@@ -1106,6 +1117,13 @@ int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, u
 void launch_refresh_copies(const float* pos, const float* vel, int32_t n, float4* px4, float4* vx4, hipStream_t st)
 {
     if (n > 0 && px4) hipLaunchKernelGGL(k_refresh_copies, dim3((n + 255) / 256), dim3(256), 0, st, pos, vel, n, px4, vx4);
+}
+
+void launch_export(const float* soa, int64_t n, int64_t first, int64_t count, int stride, float* out, hipStream_t st)
+{
+    if (count > 0)
+        hipLaunchKernelGGL(k_export, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, soa, n, first, count, stride,
+                           out);
 }
 
 void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st)
