@@ -12,9 +12,9 @@
 //   k_rank     in-bucket rank by counting over (key, index): the output order is
 //              exactly thrust's stable order (ties by object index); run starts
 //              write the dense cell-start table
-//   k_big      buckets larger than the k_rank threshold: bitonic sort of the
-//              (key, index) pairs in LDS (up to kSortLds), else ranking by
-//              counting over LDS tiles
+//              buckets larger than the threshold are then shared out over the
+//              blocks: bitonic sort of the (key, index) pairs in LDS (up to
+//              kSortLds), else ranking by counting over LDS tiles
 //   k_occ      (optional) the 2-bit brick occupancy map from the sorted keys:
 //              the first slot of every 8-cell x-group writes the group's word
 // Cell table: dense {start, count} per cell id (8 bytes, one load per lookup),
@@ -33,7 +33,7 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;  // per thread, loaded and stored as uint4
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kRankThreshold = 64;
-constexpr uint32_t kSortLds = 4096;  // bucket size sorted in LDS by k_big
+constexpr uint32_t kSortLds = 2048;  // bucket size sorted in LDS (larger ones: tiled counting)
 constexpr uint64_t kStatusAgg = 1ull << 62;
 constexpr uint64_t kStatusInc = 2ull << 62;
 constexpr uint64_t kSpinLimit = 1ull << 22;
@@ -227,46 +227,13 @@ __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32
     }
 }
 
-__global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items,
-                                              int32_t n, int shift, const uint32_t* __restrict__ bstart, RankOut o)
-{
-    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const uint2 me = items[q];
-    const uint32_t key = me.x;
-    const int32_t idx = (int32_t)me.y;
-    const uint32_t b = key >> shift;
-    const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
-    if (s1 < s0 || s1 > (uint32_t)n) return;  // inconsistent offsets (flagged by k_scatter)
-    if (s1 - s0 > kRankThreshold) return;     // handled by k_big
-    uint32_t rank = 0, same = 0;
-    bool first = true;
-    for (uint32_t u = s0; u < s1; ++u) {
-        const uint2 it = items[u];
-        const uint32_t k2 = it.x;
-        if (k2 < key) {
-            ++rank;
-        } else if (k2 == key) {
-            ++same;
-            if ((int32_t)it.y < idx) {
-                ++rank;
-                first = false;
-            }
-        }
-    }
-    rank_write(o, key, idx, s0 + rank, first, same);
-}
+constexpr int kBigChunk = 512;
 
-constexpr int kBigChunk = 1024;
-
-__global__ void __launch_bounds__(256) k_big(const uint2* __restrict__ items,
+__device__ __forceinline__ void big_buckets(uint64_t* sv, uint32_t* sk, int32_t* si, const uint2* __restrict__ items,
                                              int32_t n, int shift, const uint32_t* __restrict__ bstart,
                                              const uint32_t* __restrict__ biglist, const uint32_t* __restrict__ ctrl,
                                              RankOut o)
 {
-    __shared__ uint64_t sv[kSortLds];
-    __shared__ uint32_t sk[kBigChunk];
-    __shared__ int32_t si[kBigChunk];
     const uint32_t nbig = ctrl[1];
     const uint32_t tid = threadIdx.x;
     for (uint32_t e = blockIdx.x; e < nbig; e += gridDim.x) {
@@ -346,6 +313,44 @@ __global__ void __launch_bounds__(256) k_big(const uint2* __restrict__ items,
         }
         __syncthreads();
     }
+}
+
+// In-bucket ranking: every thread first ranks its item if the bucket holds at
+// most kRankThreshold objects (counting over the bucket), then the blocks
+// share out the larger buckets (big_buckets).
+__global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items, int32_t n, int shift,
+                                              const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ biglist,
+                                              const uint32_t* __restrict__ ctrl, RankOut o)
+{
+    __shared__ uint64_t sv[kSortLds];
+    __shared__ uint32_t sk[kBigChunk];
+    __shared__ int32_t si[kBigChunk];
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) {
+        const uint2 me = items[q];
+        const uint32_t key = me.x;
+        const int32_t idx = (int32_t)me.y;
+        const uint32_t b = key >> shift;
+        const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
+        if (s1 >= s0 && s1 <= (uint32_t)n && s1 - s0 <= kRankThreshold) {
+            uint32_t rank = 0, same = 0;
+            bool first = true;
+            for (uint32_t u = s0; u < s1; ++u) {
+                const uint2 it = items[u];
+                if (it.x < key) {
+                    ++rank;
+                } else if (it.x == key) {
+                    ++same;
+                    if ((int32_t)it.y < idx) {
+                        ++rank;
+                        first = false;
+                    }
+                }
+            }
+            rank_write(o, key, idx, s0 + rank, first, same);
+        }
+    }
+    big_buckets(sv, sk, si, items, n, shift, bstart, biglist, ctrl, o);
 }
 
 // Brick occupancy map: the first sorted slot of each 8-cell x-group (cells
@@ -558,17 +563,13 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     te(2);
     RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2};
     tb(3);
-    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items, n, G.shift, G.bstart, o);
+    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items, n, G.shift, G.bstart, G.biglist, G.ctrl, o);
     te(3);
     tb(4);
-    hipLaunchKernelGGL(k_big, dim3(std::max(1u, std::min(G.big_cap, 4096u))), dim3(256), 0, st, G.items, n,
-                       G.shift, G.bstart, G.biglist, G.ctrl, o);
-    te(4);
-    tb(5);
     if (G.gpos)
         hipLaunchKernelGGL(k_occ, dim3((std::max(G.cap, 1) + 255) / 256), dim3(256), 0, st, G.keys_sorted, n, G.cap,
                            G.table, G.bm, G.gpos);
-    te(5);
+    te(4);
     GRID_HIP(hipGetLastError());
     return BCS_OK;
 }

@@ -56,7 +56,7 @@ void grid_free(DeviceGrid& G);
 // layout, nullable) with a grid allocated with copies also writes px4 / vx4.
 int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
                hipStream_t st, const KernelTimer* timer = nullptr);
-constexpr int kGridKernels = 6;
+constexpr int kGridKernels = 5;
 
 // Bucket size from a host sample of the objects (SoA, stride apart), and the
 // switch to another bucket size (waits for the stream; buckets start empty).

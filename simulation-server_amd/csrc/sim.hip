@@ -246,7 +246,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
 }
 
 const char* kKernelNames =
-    "k_key;k_scan;k_scatter;k_rank;k_big;k_occ;k_vein_springs+k_tri_box;k_cells_a;k_ray;k_cells_b;k_vein";
+    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_springs+k_tri_box;k_cells_a;k_ray;k_cells_b;k_vein";
 constexpr int kFusedKernels = kGridKernels + 5;
 
 }  // namespace
