@@ -54,19 +54,21 @@ def step_bytes(np_, n_occ, nc, nv, nt):
 def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
     """Algorithmic HBM bytes of one launch of each frame kernel (every datum
     counted once, as a perfect cache would move it):
-      k_key        x, v read 24, key written 4, packed (x, v) copies written 32,
-                   previous map-word slot read 4 (the words it names cleared:
-                   counted in k_occ)                               64 / particle
-      k_scatter    key read 4, (key, index) written 8              12 / particle
-      k_rank       (key, index) read 8, sorted key, index, inverse written 12
-                   + {start, count} per occupied cell 8           20 / particle + 8 / cell
+      k_key        x read 12, key written 4, previous map-word slot read 4 (the
+                   words it names cleared: counted in k_occ)       20 / particle
+      k_scatter    key read 4, x, v read 24, (key, index) written 8, the
+                   32-byte (x, index | v) record written beside it 68 / particle
+      k_rank       (key, index) and record read 40, sorted key, index, inverse
+                   written 12, record written to its sorted slot 32
+                   + {start, count} per occupied cell 8           84 / particle + 8 / cell
       k_occ        sorted key read 4, map-word slot written 4; count read 8
                    and 2-bit code written (<= 2 bytes) per occupied cell
                                                                    8 / particle + 10 / cell
-      k_cells_a    x, v, F read 36, own key 4, F and first-hit key written 16;
+      k_cells_a    x, v, F read 36, own key 4, F and first-hit key written 16,
+                   sorted-slot records of the contact candidates read 32;
                    centre written 12 per blood cell; table entry + map code
                    read 10 per occupied cell; ray work items written 16 each
-                                                                   56 / particle + 12 / blood cell
+                                                                   88 / particle + 12 / blood cell
                                                                    + 10 / cell + 16 / item
       k_ray        work item 16 + x, v 24 per item (triangle data counted in k_tri_box)
       k_cells_b    x, v, F and first-hit key read 40, x, v, F written 36
@@ -78,11 +80,11 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
       k_vein       X, V, Fv read 36, X, V, Fv written 36, offsets 4
                                                                    76 / vertex"""
     table = {
-        "k_key": 64 * np_,
-        "k_scatter": 12 * np_,
-        "k_rank": 20 * np_ + 8 * n_occ,
+        "k_key": 20 * np_,
+        "k_scatter": 68 * np_,
+        "k_rank": 84 * np_ + 8 * n_occ,
         "k_occ": 8 * np_ + 10 * n_occ,
-        "k_cells_a": 56 * np_ + 12 * nc + 10 * n_occ + 16 * n_items,
+        "k_cells_a": 88 * np_ + 12 * nc + 10 * n_occ + 16 * n_items,
         "k_ray": 40 * n_items,
         "k_cells_b": 76 * np_,
         "k_vein_springs+k_tri_box": 120 * nv + 76 * nt,

@@ -43,9 +43,7 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
                                              uint2* __restrict__ table, int clear_table, BrickMap bm,
                                              const uint32_t* __restrict__ gpos, uint32_t* __restrict__ bcount,
                                              uint64_t* __restrict__ status, int32_t ntiles,
-                                             uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags,
-                                             const float* __restrict__ vel, float4* __restrict__ px4,
-                                             float4* __restrict__ vx4)
+                                             uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags)
 {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -71,10 +69,6 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
         uint32_t k = cell_key(g, x, y, z, fl);
         keys[gid] = k;
         atomicAdd(&bcount[k >> shift], 1u);
-        if (px4) {  // packed copies for neighbour reads of this frame
-            px4[gid] = make_float4(x, y, z, 0.0f);
-            vx4[gid] = make_float4(vel[gid], vel[stride + gid], vel[2 * stride + gid], 0.0f);
-        }
     }
     if (fl) atomicOr(flags, fl);
 }
@@ -187,13 +181,22 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restric
     }
 }
 
+// brec (nullable): the object's position and velocity travel with its item
+// (read here in object order, copied to the sorted slot by k_rank).
 __global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ keys, int32_t n, int shift,
                                                  const uint32_t* __restrict__ bstart, uint32_t* __restrict__ bcount,
                                                  uint2* __restrict__ items,
-                                                 uint32_t* __restrict__ flags)
+                                                 uint32_t* __restrict__ flags, const float* __restrict__ xyz,
+                                                 const float* __restrict__ vel, int64_t stride,
+                                                 float4* __restrict__ brec)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    float4 rx, rv;
+    if (brec) {
+        rx = make_float4(xyz[i], xyz[stride + i], xyz[2 * stride + i], __int_as_float(i));
+        rv = make_float4(vel[i], vel[stride + i], vel[2 * stride + i], 0.0f);
+    }
     const uint32_t k = keys[i];
     const uint32_t b = k >> shift;
     // counts return to zero: the histogram is ready for the next build
@@ -204,6 +207,10 @@ __global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ ke
         return;
     }
     items[pos] = make_uint2(k, (uint32_t)i);
+    if (brec) {
+        brec[2 * (size_t)pos] = rx;
+        brec[2 * (size_t)pos + 1] = rv;
+    }
 }
 
 struct RankOut {
@@ -213,13 +220,20 @@ struct RankOut {
     uint2* table;          // {start, count} per cell
     uint32_t* runs;        // run starts of occupied cells, in no particular order (nullable)
     uint32_t* nruns;
+    const float4* brec;    // per-item records (nullable) ...
+    float4* pvs;           // ... copied to their sorted slots
 };
 
+// Object idx goes to sorted slot pos; (ra, rb) is its record (when o.pvs).
 __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32_t idx, uint32_t pos, bool first,
-                                           uint32_t count)
+                                           uint32_t count, float4 ra, float4 rb)
 {
     o.keys_sorted[pos] = key;
     o.perm[pos] = idx;
+    if (o.pvs) {
+        o.pvs[2 * (size_t)pos] = ra;
+        o.pvs[2 * (size_t)pos + 1] = rb;
+    }
     if (o.slot_of) o.slot_of[idx] = (int32_t)pos;
     if (first) {
         o.table[key] = make_uint2(pos, count);
@@ -227,9 +241,22 @@ __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32
     }
 }
 
+// The same with the record read from item q (bucketed position).
+__device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32_t idx, uint32_t pos, bool first,
+                                           uint32_t count, uint32_t q)
+{
+    float4 ra = make_float4(0.f, 0.f, 0.f, 0.f), rb = ra;
+    if (o.pvs) {
+        ra = o.brec[2 * (size_t)q];
+        rb = o.brec[2 * (size_t)q + 1];
+    }
+    rank_write(o, key, idx, pos, first, count, ra, rb);
+}
+
 constexpr int kBigChunk = 512;
 
-__device__ __forceinline__ void big_buckets(uint64_t* sv, uint32_t* sk, int32_t* si, const uint2* __restrict__ items,
+__device__ __forceinline__ void big_buckets(uint64_t* sv, uint16_t* sq, uint32_t* sk, int32_t* si,
+                                            const uint2* __restrict__ items,
                                              int32_t n, int shift, const uint32_t* __restrict__ bstart,
                                              const uint32_t* __restrict__ biglist, const uint32_t* __restrict__ ctrl,
                                              RankOut o)
@@ -245,8 +272,11 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint32_t* sk, int32_t*
             // (key << 32 | index) orders exactly like the stable sort by key
             uint32_t P = 1;
             while (P < m) P <<= 1;
-            for (uint32_t t = tid; t < P; t += blockDim.x)
+            // sq: the item position (within the bucket) of each entry
+            for (uint32_t t = tid; t < P; t += blockDim.x) {
                 sv[t] = t < m ? ((uint64_t)items[s0 + t].x << 32) | items[s0 + t].y : ~0ull;
+                sq[t] = (uint16_t)t;
+            }
             __syncthreads();
             for (uint32_t k = 2; k <= P; k <<= 1) {
                 for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -257,6 +287,9 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint32_t* sk, int32_t*
                             if ((a > c) == ((t & k) == 0)) {
                                 sv[t] = c;
                                 sv[u] = a;
+                                const uint16_t qt = sq[t];
+                                sq[t] = sq[u];
+                                sq[u] = qt;
                             }
                         }
                     }
@@ -272,7 +305,7 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint32_t* sk, int32_t*
                     count = 1;
                     while (t + count < m && (uint32_t)(sv[t + count] >> 32) == key) ++count;
                 }
-                rank_write(o, key, (int32_t)(uint32_t)v, s0 + t, first, count);
+                rank_write(o, key, (int32_t)(uint32_t)v, s0 + t, first, count, s0 + sq[t]);
             }
             __syncthreads();  // sv is reused by the next bucket
             continue;
@@ -309,48 +342,82 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint32_t* sk, int32_t*
                     }
                 }
             }
-            if (own) rank_write(o, key, idx, s0 + rank, first, same);
+            if (own) rank_write(o, key, idx, s0 + rank, first, same, q);
         }
         __syncthreads();
     }
 }
 
+// Rank of item (key, idx) among the items get(s0 .. s1-1) of its bucket.
+template <class Get>
+__device__ __forceinline__ void count_rank(Get get, uint32_t s0, uint32_t s1, uint32_t key, int32_t idx,
+                                           uint32_t& rank, uint32_t& same, bool& first)
+{
+    uint32_t before = 0;  // same key, lower index
+#pragma unroll 4
+    for (uint32_t u = s0; u < s1; ++u) {
+        const uint2 it = get(u);
+        const uint32_t eq = it.x == key ? 1u : 0u;
+        rank += it.x < key ? 1u : 0u;
+        same += eq;
+        before += eq & ((int32_t)it.y < idx ? 1u : 0u);
+    }
+    rank += before;
+    first = before == 0u;
+}
+
 // In-bucket ranking: every thread first ranks its item if the bucket holds at
 // most kRankThreshold objects (counting over the bucket), then the blocks
-// share out the larger buckets (big_buckets).
+// share out the larger buckets (big_buckets).  The small buckets of a block's
+// items lie within kRankThreshold items of the block's range; that window is
+// read once into LDS.
 __global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items, int32_t n, int shift,
                                               const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ biglist,
                                               const uint32_t* __restrict__ ctrl, RankOut o)
 {
-    __shared__ uint64_t sv[kSortLds];
+    __shared__ uint64_t sv[kSortLds];  // the item window first, then the big-bucket sort
+    __shared__ uint16_t sq[kSortLds];
     __shared__ uint32_t sk[kBigChunk];
     __shared__ int32_t si[kBigChunk];
-    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    static_assert(256 + 2 * kRankThreshold <= kSortLds, "item window exceeds the sort buffer");
+    const int32_t B = blockIdx.x * blockDim.x;
+    const int32_t q = B + threadIdx.x;
+    const int32_t w0 = max(B - (int32_t)kRankThreshold, 0);
+    const int32_t w1 = min(B + (int32_t)blockDim.x + (int32_t)kRankThreshold, n);
+    uint2* win = reinterpret_cast<uint2*>(sv);
+    {  // at most 2 items per thread (window <= 256 + 2 * kRankThreshold <= 512); both loads in flight together
+        static_assert(2 * kRankThreshold <= 256, "window exceeds two items per thread");
+        const int32_t t0 = w0 + (int32_t)threadIdx.x, t1 = t0 + 256;
+        uint2 a = make_uint2(0u, 0u), c = a;
+        if (t0 < w1) a = items[t0];
+        if (t1 < w1) c = items[t1];
+        if (t0 < w1) win[t0 - w0] = a;
+        if (t1 < w1) win[t1 - w0] = c;
+    }
+    __syncthreads();
     if (q < n) {
-        const uint2 me = items[q];
+        const uint2 me = win[q - w0];
         const uint32_t key = me.x;
         const int32_t idx = (int32_t)me.y;
         const uint32_t b = key >> shift;
+        float4 ra = make_float4(0.f, 0.f, 0.f, 0.f), rb = ra;
+        if (o.pvs) {  // issued before the count, used after it
+            ra = o.brec[2 * (size_t)q];
+            rb = o.brec[2 * (size_t)q + 1];
+        }
         const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
         if (s1 >= s0 && s1 <= (uint32_t)n && s1 - s0 <= kRankThreshold) {
             uint32_t rank = 0, same = 0;
             bool first = true;
-            for (uint32_t u = s0; u < s1; ++u) {
-                const uint2 it = items[u];
-                if (it.x < key) {
-                    ++rank;
-                } else if (it.x == key) {
-                    ++same;
-                    if ((int32_t)it.y < idx) {
-                        ++rank;
-                        first = false;
-                    }
-                }
-            }
-            rank_write(o, key, idx, s0 + rank, first, same);
+            if (s0 >= (uint32_t)w0 && s1 <= (uint32_t)w1)
+                count_rank([&](uint32_t u) { return win[u - w0]; }, s0, s1, key, idx, rank, same, first);
+            else  // inconsistent offsets (flagged by k_scatter): read the items directly
+                count_rank([&](uint32_t u) { return items[u]; }, s0, s1, key, idx, rank, same, first);
+            rank_write(o, key, idx, s0 + rank, first, same, ra, rb);
         }
     }
-    big_buckets(sv, sk, si, items, n, shift, bstart, biglist, ctrl, o);
+    __syncthreads();  // the window's LDS is reused by the big-bucket sort
+    big_buckets(sv, sq, sk, si, items, n, shift, bstart, biglist, ctrl, o);
 }
 
 // Brick occupancy map: the first sorted slot of each 8-cell x-group (cells
@@ -438,8 +505,8 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     GRID_HIP(hipMalloc(&G.keys_sorted, n * 4));
     if (with_runs) GRID_HIP(hipMalloc(&G.runs, n * 4));
     if (with_copies) {
-        GRID_HIP(hipMalloc(&G.px4, n * 16));
-        GRID_HIP(hipMalloc(&G.vx4, n * 16));
+        GRID_HIP(hipMalloc(&G.brec, n * 32));
+        GRID_HIP(hipMalloc(&G.pvs, n * 32));
     }
     GRID_HIP(hipMalloc(&G.perm, n * 4));
     if (with_slot) GRID_HIP(hipMalloc(&G.slot_of, n * 4));
@@ -529,7 +596,7 @@ int grid_shift_for(const GridGeom& g, const float* pos, int64_t stride, int32_t 
 
 void grid_free(DeviceGrid& G)
 {
-    void* ptrs[] = {G.keys, G.items, G.keys_sorted, G.runs, G.perm, G.slot_of, G.px4, G.vx4,
+    void* ptrs[] = {G.keys, G.items, G.keys_sorted, G.runs, G.perm, G.slot_of, G.brec, G.pvs,
                     G.table, G.bm.words, G.gpos, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -551,17 +618,18 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     tb(0);
     hipLaunchKernelGGL(k_key, dim3(std::max(blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride, n,
                        n_prev, G.shift, G.keys, G.table, G.clear_table ? 1 : 0, G.bm, G.gpos, G.bcount, G.status,
-                       G.ntiles, G.ctrl, flags, vel, vel ? G.px4 : nullptr, vel ? G.vx4 : nullptr);
+                       G.ntiles, G.ctrl, flags);
     te(0);
     tb(1);
     hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,
                        G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold, flags);
     te(1);
     tb(2);
+    float4* brec = vel ? G.brec : nullptr;
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.bcount, G.items,
-                       flags);
+                       flags, pos, vel, stride, brec);
     te(2);
-    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2};
+    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr};
     tb(3);
     hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items, n, G.shift, G.bstart, G.biglist, G.ctrl, o);
     te(3);

@@ -37,8 +37,9 @@ struct DeviceGrid {
     bool clear_table = true;          // zero the entries of cells that emptied (else: consult bm)
     BrickMap bm{};                    // occupancy map (optional; words == nullptr when absent)
     uint32_t* gpos = nullptr;         // per sorted slot: map word written by the last build, or kNoWord
-    float4* px4 = nullptr;            // packed copies in object order (optional): (x, y, z, 0)
-    float4* vx4 = nullptr;            //                                           (vx, vy, vz, 0)
+    float4* brec = nullptr;           // per bucketed item q (optional): brec[2q] = (x, y, z, object
+                                      // index as int bits), brec[2q+1] = (vx, vy, vz, 0)
+    float4* pvs = nullptr;            // the same records per sorted slot
     uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds
     uint32_t* bstart = nullptr;       // bucket offsets [nb+2]
     uint64_t* status = nullptr;       // scan tile status
@@ -53,7 +54,8 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
                bool with_copies, hipStream_t st);
 void grid_free(DeviceGrid& G);
 // pos: SoA base pointer (x, y, z at +0, +stride, +2*stride).  vel (same
-// layout, nullable) with a grid allocated with copies also writes px4 / vx4.
+// layout, nullable) with a grid allocated with copies also writes the
+// sorted-slot records pvs.
 int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
                hipStream_t st, const KernelTimer* timer = nullptr);
 constexpr int kGridKernels = 5;

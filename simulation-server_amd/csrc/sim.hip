@@ -33,7 +33,8 @@ void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, i
                     float4* aabb, hipStream_t st);
 void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st);
 void launch_export(const float* soa, int64_t n, int64_t first, int64_t count, int stride, float* out, hipStream_t st);
-void launch_refresh_copies(const float* pos, const float* vel, int32_t n, float4* px4, float4* vx4, hipStream_t st);
+void launch_refresh_copies(const float* pos, const float* vel, int32_t n, const int32_t* perm, float4* pvs,
+                           hipStream_t st);
 int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
                  uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
                  uint32_t* flags, hipStream_t st);
@@ -129,7 +130,7 @@ struct bcs_sim {
         A.chunk_type = chunk_type; A.chunk_first = chunk_first; A.chunk_n = chunk_n;
         A.corder = corder; A.nchunks = nchunks;
         A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table; A.p_map = pg.bm; A.p_key_of = pg.keys;
-        A.px4 = pg.px4; A.vx4 = pg.vx4;
+        A.pvs = pg.pvs;
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.hits = hits; A.vhead = vhead; A.vnext = vnext; A.hctrl = hctrl;
@@ -631,7 +632,7 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
     // particles moved since the grid was built: the reference then scans the
     // old cells with the current positions, so refresh the packed copies
     if (!s->copies_current) {
-        launch_refresh_copies(s->pos, s->vel, s->np, s->pg.px4, s->pg.vx4, s->stream);
+        launch_refresh_copies(s->pos, s->vel, s->np, s->pg.perm, s->pg.pvs, s->stream);
         s->copies_current = true;
     }
     const StepArgs A = s->step_args();

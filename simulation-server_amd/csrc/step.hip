@@ -351,6 +351,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
     __shared__ float sfx[kBlock], sfy[kBlock], sfz[kBlock];
     __shared__ uint32_t s_wsum[kBlock / 64];
     __shared__ uint32_t s_end;
+    __shared__ float s_bs[kBsLds];
 
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
@@ -360,6 +361,12 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         if (l == 0) A.wcount[blockIdx.x] = 0u;
         return;
     }
+    // contact radii per model vertex (cellModelsBoundingSpheres), from LDS
+    // when the table fits
+    const bool bs_lds = tt.nm <= kBsLds;
+    if (bs_lds)
+        for (int t = l; t < tt.nm; t += kBlock) s_bs[t] = A.bs[t];
+    auto radius = [&](int32_t m) -> float { return bs_lds ? s_bs[m] : A.bs[m]; };
     const int T = A.chunk_type[chunk];
     const int first = A.chunk_first[chunk];
     const int ncell = A.chunk_n[chunk];
@@ -371,10 +378,14 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
     const int64_t np = tt.np;
     int32_t cell = 0;
     const int32_t i = act ? chunk_particle(A, T, first, cl, a, p, cell) : 0;
+    uint32_t key_of = 0;
+    int32_t slot_i = 0;
     if (act) {
         sx[l] = A.pos[i]; sy[l] = A.pos[np + i]; sz[l] = A.pos[2 * np + i];
         svx[l] = A.vel[i]; svy[l] = A.vel[np + i]; svz[l] = A.vel[2 * np + i];
         sfx[l] = A.frc[i]; sfy[l] = A.frc[np + i]; sfz[l] = A.frc[2 * np + i];
+        key_of = A.p_key_of[i];
+        if (tt.nt > 1) slot_i = A.p_slot[i];
     }
     __syncthreads();
 
@@ -439,11 +450,11 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         {
             const GridGeom& g = A.pg;
             int Tl = T;
-            if (tt.nt > 1) Tl = type_of_index(tt, A.p_slot[i]);
-            const float r1 = A.bs[model_index(tt, Tl, i, fl)];
+            if (tt.nt > 1) Tl = type_of_index(tt, slot_i);
+            const float r1 = radius(model_index(tt, Tl, i, fl));
             // the cell the grid build gave i (gridCellIds of its slot); the face
             // clipping below uses the current position, as the reference does
-            const uint32_t cellId = A.p_key_of[i];
+            const uint32_t cellId = key_of;
             const int32_t xId = sat_i32((x1.x - g.minv[0]) / g.cell[0]);
             const int32_t yId = sat_i32((x1.y - g.minv[1]) / g.cell[1]);
             const int32_t zId = sat_i32((x1.z - g.minv[2]) / g.cell[2]);
@@ -452,27 +463,51 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
             axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
             const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-            uint32_t mask = neighbour_mask(A.p_map, g, cellId, nr, cX, cXY, true);
-            while (mask) {
-                const int r = __builtin_ctz(mask);
-                mask &= mask - 1u;
-                const int dx = r / 9 - 1, c = r % 9;
-                const int dy = c / 3 - 1, dz = c % 3 - 1;
-                const int64_t nk = (int64_t)cellId + dz * cXY + dy * cX + dx;
-                const uint2 ent = A.p_table[nk];
-                const uint32_t u1 = min(ent.x + ent.y, (uint32_t)np);
-                for (uint32_t u = ent.x; u < u1; ++u) {
-                    const int32_t j = A.p_perm[u];
-                    if (j == i) continue;
-                    const float r2 = A.bs[model_index(tt, Tl, j, fl)];
-                    const float4 q = A.px4[j];
-                    const f3 rel = sub(x1, mk(q.x, q.y, q.z));
-                    const float d2 = dot(rel, rel);
-                    const float minD = r1 + r2;
-                    if (d2 <= minD * minD && d2 >= 0.0001f) {
-                        const float4 w = A.vx4[j];
-                        const f3 rv = sub(v1, mk(w.x, w.y, w.z));
-                        F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
+            const uint32_t mask = neighbour_mask(A.p_map, g, cellId, nr, cX, cXY, true);
+            // One x-slab (9 cells, visit ranks (dx+1)*9 + 0..8) at a time: the
+            // table entries of its occupied cells are loaded together, then
+            // each cell's run of sorted-slot records is read 4 slots per load
+            // batch.  Contributions are added in visit order, slot by slot.
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                uint32_t sm = (mask >> ((dx + 1) * 9)) & 0x1FFu;
+                if (!sm) continue;
+                uint2 ent[9];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    ent[c] = make_uint2(0u, 0u);
+                    if ((sm >> c) & 1u) ent[c] = A.p_table[(int64_t)cellId + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
+                }
+                while (sm) {
+                    const int c = __builtin_ctz(sm);
+                    sm &= sm - 1u;
+                    uint2 e = ent[0];
+#pragma unroll
+                    for (int k = 1; k < 9; ++k)
+                        if (c == k) e = ent[k];
+                    const uint32_t u1 = min(e.x + e.y, (uint32_t)np);
+                    for (uint32_t u = e.x; u < u1; u += kContactBatch) {
+                        float4 q[kContactBatch], w[kContactBatch];
+#pragma unroll
+                        for (int k = 0; k < kContactBatch; ++k)
+                            if (u + k < u1) {
+                                q[k] = A.pvs[2 * (size_t)(u + k)];
+                                w[k] = A.pvs[2 * (size_t)(u + k) + 1];
+                            }
+#pragma unroll
+                        for (int k = 0; k < kContactBatch; ++k) {
+                            if (u + k >= u1) break;
+                            const int32_t j = __float_as_int(q[k].w);
+                            if (j == i) continue;
+                            const float r2 = radius(model_index(tt, Tl, j, fl));
+                            const f3 rel = sub(x1, mk(q[k].x, q[k].y, q[k].z));
+                            const float d2 = dot(rel, rel);
+                            const float minD = r1 + r2;
+                            if (d2 <= minD * minD && d2 >= 0.0001f) {
+                                const f3 rv = sub(v1, mk(w[k].x, w[k].y, w[k].z));
+                                F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
+                            }
+                        }
                     }
                 }
             }
@@ -1060,16 +1095,19 @@ __global__ void __launch_bounds__(256) k_contacts(StepArgs A, int pass, int32_t*
     if (pass == 0) counts[i] = c;
 }
 
-// Packed copies from the current state (when particles moved after the last
-// grid build).
+// Sorted-slot records from the current state (when particles moved after the
+// last grid build): slot u keeps the particle perm[u] of that build.
 __global__ void __launch_bounds__(256) k_refresh_copies(const float* __restrict__ pos, const float* __restrict__ vel,
-                                                        int32_t n, float4* __restrict__ px4, float4* __restrict__ vx4)
+                                                        int32_t n, const int32_t* __restrict__ perm,
+                                                        float4* __restrict__ pvs)
 {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const int32_t i = perm[u];
+    if (i < 0 || i >= n) return;
     const f3 x = load3(pos, n, i), v = load3(vel, n, i);
-    px4[i] = make_float4(x.x, x.y, x.z, 0.0f);
-    vx4[i] = make_float4(v.x, v.y, v.z, 0.0f);
+    pvs[2 * (size_t)u] = make_float4(x.x, x.y, x.z, __int_as_float(i));
+    pvs[2 * (size_t)u + 1] = make_float4(v.x, v.y, v.z, 0.0f);
 }
 
 // Frame export: SoA -> xyz interleaved with a given stride (3 or 6 floats).
@@ -1114,9 +1152,11 @@ int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, u
     return BCS_OK;
 }
 
-void launch_refresh_copies(const float* pos, const float* vel, int32_t n, float4* px4, float4* vx4, hipStream_t st)
+void launch_refresh_copies(const float* pos, const float* vel, int32_t n, const int32_t* perm, float4* pvs,
+                           hipStream_t st)
 {
-    if (n > 0 && px4) hipLaunchKernelGGL(k_refresh_copies, dim3((n + 255) / 256), dim3(256), 0, st, pos, vel, n, px4, vx4);
+    if (n > 0 && pvs)
+        hipLaunchKernelGGL(k_refresh_copies, dim3((n + 255) / 256), dim3(256), 0, st, pos, vel, n, perm, pvs);
 }
 
 void launch_export(const float* soa, int64_t n, int64_t first, int64_t count, int stride, float* out, hipStream_t st)

@@ -7,6 +7,8 @@ namespace bcs {
 
 constexpr int kMaxEndings = 16;
 constexpr int kBlock = 256;  // threads per cell-chunk block
+constexpr int kBsLds = 1024;       // model-vertex radii held in LDS by the contact search (more: read globally)
+constexpr int kContactBatch = 4;   // sorted-slot records loaded together by the contact search
 constexpr int kOrderBits = 15;      // Morton bits of the cell processing order
 constexpr int kReorderPeriod = 16;  // frames between updates of the order
 
@@ -63,8 +65,8 @@ struct StepArgs {
     const uint2* p_table;   // {start, count} per cell (current where occupied in p_map)
     BrickMap p_map;         // 2-bit brick occupancy map of the particle grid
     const uint32_t* p_key_of;  // per particle: its cell key in the last build (gridCellIds by particle)
-    const float4* px4;      // per particle (x, y, z, 0) as of the last grid build
-    const float4* vx4;      // per particle (vx, vy, vz, 0)
+    const float4* pvs;      // per sorted slot u, as of the last grid build: pvs[2u] = (x, y, z,
+                            // particle perm[u] as int bits), pvs[2u+1] = (vx, vy, vz, 0)
     // triangle grid + vein
     GridGeom tg;
     const uint2* t_table;
