@@ -53,19 +53,22 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
         ctrl[1] = 0u;  // big-bucket count
         ctrl[2] = 0u;  // occupied cells (run list)
     }
-    if (gid < n_prev) {  // clear the previous build's cell of this object
-        if (clear_table) {
-            const uint32_t old = keys[gid];
-            if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
-        }
-        if (gpos) {  // map words written by the previous build (indexed by its sorted slots)
-            const uint32_t w = gpos[gid];
-            if (w != kNoWord) bm.words[w] = 0;
-        }
+    // every load first (one round trip), then the clears and the new key
+    uint32_t old = 0xFFFFFFFFu, w = kNoWord;
+    float x = 0.f, y = 0.f, z = 0.f;
+    if (gid < n_prev) {  // the previous build's cell of this object
+        if (clear_table) old = keys[gid];
+        if (gpos) w = gpos[gid];  // map word written by the previous build (indexed by its sorted slots)
     }
+    if (gid < n) {
+        x = pos[gid];
+        y = pos[stride + gid];
+        z = pos[2 * stride + gid];
+    }
+    if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
+    if (w != kNoWord) bm.words[w] = 0;
     uint32_t fl = 0;
     if (gid < n) {
-        const float x = pos[gid], y = pos[stride + gid], z = pos[2 * stride + gid];
         uint32_t k = cell_key(g, x, y, z, fl);
         keys[gid] = k;
         atomicAdd(&bcount[k >> shift], 1u);
@@ -79,7 +82,8 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
 __global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restrict__ count, uint32_t* __restrict__ out,
                                                        int64_t m, uint64_t* __restrict__ status, int32_t ntiles,
                                                        uint32_t* __restrict__ ctrl, uint32_t* __restrict__ biglist,
-                                                       uint32_t big_thr, uint32_t* __restrict__ flags)
+                                                       uint32_t big_thr, uint32_t big_cap,
+                                                       uint32_t* __restrict__ flags)
 {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kScanThreads / 64];
@@ -109,17 +113,34 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restric
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) v[k] = (base + k < m) ? count[base + k] : 0u;
     }
+    const int lane = tid & 63, wave = tid >> 6;
+    uint32_t nbig = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        int64_t i = base + k;
-        if (biglist && i < m && v[k] > big_thr) {
-            uint32_t slot = atomicAdd(&ctrl[1], 1u);
-            biglist[slot] = (uint32_t)i;
-        }
+        nbig += (biglist && base + k < m && v[k] > big_thr) ? 1u : 0u;
         sum += v[k];
     }
+    if (biglist) {  // big buckets appended with one counter update per wave
+        uint32_t binc = nbig;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(binc, d, 64);
+            if (lane >= d) binc += o;
+        }
+        const uint32_t wtotal = __shfl(binc, 63, 64);
+        uint32_t wbase = 0;
+        if (lane == 0 && wtotal) wbase = atomicAdd(&ctrl[1], wtotal);
+        wbase = __shfl(wbase, 0, 64);
+        uint32_t slot = wbase + binc - nbig;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < m && v[k] > big_thr) {
+                if (slot < big_cap) biglist[slot] = (uint32_t)(base + k);
+                else atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);  // more than n / big_thr: inconsistent counts
+                ++slot;
+            }
+    }
     // wave64 inclusive scan of the per-thread sums
-    const int lane = tid & 63, wave = tid >> 6;
     uint32_t inc = sum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -259,9 +280,9 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint16_t* sq, uint32_t
                                             const uint2* __restrict__ items,
                                              int32_t n, int shift, const uint32_t* __restrict__ bstart,
                                              const uint32_t* __restrict__ biglist, const uint32_t* __restrict__ ctrl,
-                                             RankOut o)
+                                             uint32_t big_cap, RankOut o)
 {
-    const uint32_t nbig = ctrl[1];
+    const uint32_t nbig = min(ctrl[1], big_cap);
     const uint32_t tid = threadIdx.x;
     for (uint32_t e = blockIdx.x; e < nbig; e += gridDim.x) {
         const uint32_t b = biglist[e];
@@ -373,7 +394,7 @@ __device__ __forceinline__ void count_rank(Get get, uint32_t s0, uint32_t s1, ui
 // read once into LDS.
 __global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items, int32_t n, int shift,
                                               const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ biglist,
-                                              const uint32_t* __restrict__ ctrl, RankOut o)
+                                              const uint32_t* __restrict__ ctrl, uint32_t big_cap, RankOut o)
 {
     __shared__ uint64_t sv[kSortLds];  // the item window first, then the big-bucket sort
     __shared__ uint16_t sq[kSortLds];
@@ -417,7 +438,7 @@ __global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items, i
         }
     }
     __syncthreads();  // the window's LDS is reused by the big-bucket sort
-    big_buckets(sv, sq, sk, si, items, n, shift, bstart, biglist, ctrl, o);
+    big_buckets(sv, sq, sk, si, items, n, shift, bstart, biglist, ctrl, big_cap, o);
 }
 
 // Brick occupancy map: the first sorted slot of each 8-cell x-group (cells
@@ -433,20 +454,27 @@ __global__ void __launch_bounds__(256) k_occ(const uint32_t* __restrict__ keys_s
     uint32_t w = kNoWord;
     if (u < n) {
         const uint32_t key = keys_sorted[u];
+        const uint32_t prev = u > 0 ? keys_sorted[u - 1] : 0u;
         uint32_t x, y, z;
         key_coords(bm, key, x, y, z);
         const uint32_t gs = key - (x & 7u);
-        if (u == 0 || keys_sorted[u - 1] < gs) {
+        if (u == 0 || prev < gs) {
             const uint32_t ge = gs + min(8u, (uint32_t)bm.cnt[0] - (x & ~7u));
             uint32_t code = 0;
             int32_t v = u;
-            while (v < n) {
-                const uint32_t k2 = keys_sorted[v];
-                if (k2 >= ge) break;
+            uint32_t k2 = key;  // keys_sorted[v]
+            while (v < n && k2 < ge) {
                 // run length from the next sorted key; the table only for runs of 2+
-                const bool multi = v + 1 < n && keys_sorted[v + 1] == k2;
+                const uint32_t nxt = v + 1 < n ? keys_sorted[v + 1] : 0xFFFFFFFFu;
+                const bool multi = nxt == k2;
                 code |= (multi ? 2u : 1u) << (2u * (k2 - gs));
-                v += multi ? (int32_t)max(table[k2].y, 2u) : 1;
+                if (multi) {
+                    v += (int32_t)max(table[k2].y, 2u);
+                    k2 = v < n ? keys_sorted[v] : 0xFFFFFFFFu;
+                } else {
+                    v += 1;
+                    k2 = nxt;
+                }
             }
             w = brick_word(bm, x, y, z);
             bm.words[w] = (uint16_t)code;
@@ -622,7 +650,7 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     te(0);
     tb(1);
     hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,
-                       G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold, flags);
+                       G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold, G.big_cap, flags);
     te(1);
     tb(2);
     float4* brec = vel ? G.brec : nullptr;
@@ -631,7 +659,8 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     te(2);
     RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr};
     tb(3);
-    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items, n, G.shift, G.bstart, G.biglist, G.ctrl, o);
+    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items, n, G.shift, G.bstart, G.biglist, G.ctrl,
+                       G.big_cap, o);
     te(3);
     tb(4);
     if (G.gpos)
@@ -647,7 +676,7 @@ int scan_exclusive(const uint32_t* count, uint32_t* out, int64_t m, uint64_t* st
                    uint32_t* flags, hipStream_t st)
 {
     hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, out, m, status, ntiles, ctrl,
-                       (uint32_t*)nullptr, 0u, flags);
+                       (uint32_t*)nullptr, 0u, 0u, flags);
     GRID_HIP(hipGetLastError());
     return BCS_OK;
 }

@@ -352,6 +352,9 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
     __shared__ uint32_t s_wsum[kBlock / 64];
     __shared__ uint32_t s_end;
     __shared__ float s_bs[kBsLds];
+    __shared__ int32_t s_spoff[kBlock + 1];  // the block's cell type: spring list offsets per vertex,
+    __shared__ int32_t s_spb[kSpringLds];    // partner vertex and rest length per spring
+    __shared__ float s_spL[kSpringLds];
 
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
@@ -387,6 +390,20 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         key_of = A.p_key_of[i];
         if (tt.nt > 1) slot_i = A.p_slot[i];
     }
+    // spring lists of this type's model vertices (one type per block), from
+    // LDS when they fit
+    const int m0 = tt.mstart[T];
+    const int sp0 = A.sp_off[m0];
+    const bool sp_lds = A.sp_off[m0 + p] - sp0 <= kSpringLds;
+    if (sp_lds) {
+        const int nsp = A.sp_off[m0 + p] - sp0;
+        for (int t = l; t <= p; t += kBlock) s_spoff[t] = A.sp_off[m0 + t] - sp0;
+        for (int t = l; t < nsp; t += kBlock) {
+            s_spb[t] = A.sp_b[sp0 + t];
+            s_spL[t] = A.sp_L[sp0 + t];
+        }
+    }
+    const float r0a = act ? A.r0[m0 + a] : 1.0f;
     __syncthreads();
 
     uint32_t fl = 0;
@@ -409,12 +426,12 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             A.centers[2 * (int64_t)tt.nc + cell] = cen.z;
         }
         // ---- springs + environment (F of neighbours from the LDS snapshot)
-        const int ma = tt.mstart[T] + a;
+        const int ma = m0 + a;
         f3 newF = mk(0.f, 0.f, 0.f);
-        const int e0 = A.sp_off[ma], e1 = A.sp_off[ma + 1];
+        auto spring_sum = [&](auto spb, auto spL, int e0, int e1) {
         for (int e = e0; e < e1; ++e) {
-            const int nb = base + A.sp_b[e];
-            const float L = A.sp_L[e];
+            const int nb = base + spb(e);
+            const float L = spL(e);
             f3 dP = sub(x1, mk(sx[nb], sy[nb], sz[nb]));
             f3 dv = sub(v1, mk(svx[nb], svy[nb], svz[nb]));
             f3 f2 = mk(sfx[nb], sfy[nb], sfz[nb]);
@@ -435,7 +452,13 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             float mag = (sl - L) * pc.k + dot(nrm, dvv) * pc.d;
             newF = add(newF, smul(mag, shift));
         }
-        const float ratio = len(sub(x1, cen)) / A.r0[ma];
+        };
+        if (sp_lds)
+            spring_sum([&](int e) { return s_spb[e]; }, [&](int e) { return s_spL[e]; }, s_spoff[a], s_spoff[a + 1]);
+        else
+            spring_sum([&](int e) { return A.sp_b[e]; }, [&](int e) { return A.sp_L[e]; }, A.sp_off[ma],
+                       A.sp_off[ma + 1]);
+        const float ratio = len(sub(x1, cen)) / r0a;
         f3 env;
         const f3 G = mk(pc.G[0], pc.G[1], pc.G[2]);
         if (pc.brake_on && ratio > pc.brake) env = sub(G, smul(pc.vdamp * ratio * pc.bint, v1));
@@ -522,7 +545,8 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             if (om) {
                 const f3 inv = inv_dir(dir);
                 const float bm = box_margin(A);
-                // boxes of one x-slab's occupied cells loaded together, then tested
+                // boxes of one x-slab's occupied cells loaded together, then
+                // tested; a passing cell's triangle count is in hi.w
 #pragma unroll
                 for (int dx = -1; dx <= 1; ++dx) {
                     const uint32_t sm = (om >> ((dx + 1) * 9)) & 0x1FFu;
@@ -538,25 +562,10 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
                     }
 #pragma unroll
                     for (int c = 0; c < 9; ++c)
-                        if (((sm >> c) & 1u) && ray_box(x1, inv, dir, lo[c], hi[c], bm))
+                        if (((sm >> c) & 1u) && ray_box(x1, inv, dir, lo[c], hi[c], bm)) {
                             pmask |= 1u << ((dx + 1) * 9 + c);
-                }
-                // work items of the passing cells (independent table loads)
-#pragma unroll
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const uint32_t sm = (pmask >> ((dx + 1) * 9)) & 0x1FFu;
-                    if (!sm) continue;
-                    uint2 ent[9];
-#pragma unroll
-                    for (int c = 0; c < 9; ++c) {
-                        if ((sm >> c) & 1u) {
-                            const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
-                            ent[c] = A.t_table[nk];
+                            nitems += (min(__float_as_uint(hi[c].w), (uint32_t)A.ntri) + kItemTris - 1) / kItemTris;
                         }
-                    }
-#pragma unroll
-                    for (int c = 0; c < 9; ++c)
-                        if ((sm >> c) & 1u) nitems += (min(ent[c].y, (uint32_t)A.ntri) + kItemTris - 1) / kItemTris;
                 }
             }
         }
@@ -953,9 +962,9 @@ __global__ void __launch_bounds__(256) k_tri_box(const uint32_t* __restrict__ ru
                 hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], off, 64));
             }
         }
-        if (lane == 0) {
-            aabb[2 * (size_t)key] = make_float4(lo[0], lo[1], lo[2], 0.f);
-            aabb[2 * (size_t)key + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+        if (lane == 0) {  // the cell's run {start, count} rides in the w lanes
+            aabb[2 * (size_t)key] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(s0));
+            aabb[2 * (size_t)key + 1] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(cnt));
         }
     }
 }

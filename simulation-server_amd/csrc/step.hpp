@@ -9,6 +9,7 @@ constexpr int kMaxEndings = 16;
 constexpr int kBlock = 256;  // threads per cell-chunk block
 constexpr int kBsLds = 1024;       // model-vertex radii held in LDS by the contact search (more: read globally)
 constexpr int kContactBatch = 4;   // sorted-slot records loaded together by the contact search
+constexpr int kSpringLds = 1024;   // springs of one cell type held in LDS by k_cells_a (more: read globally)
 constexpr int kOrderBits = 15;      // Morton bits of the cell processing order
 constexpr int kReorderPeriod = 16;  // frames between updates of the order
 
@@ -80,7 +81,7 @@ struct StepArgs {
     int32_t* vhead;   // per vein vertex: first hit entry (3h + corner) of its list, -1 = none
     int32_t* vnext;   // per hit entry: next entry of the same vertex, -1 = end
     uint32_t* hctrl;  // [0] hit count, [4] hits of the last frame
-    const float4* t_aabb;     // per occupied triangle cell key k: [2k] = lo, [2k+1] = hi
+    const float4* t_aabb;     // per occupied triangle cell key k: [2k] = (lo, run start), [2k+1] = (hi, run count)
     const uint32_t* t_nmask;  // per triangle cell: occupied cells of its neighbourhood
     uint32_t* best;           // per particle: min (cell rank << 27 | slot) of its ray hits
     uint4* work;              // ray work items: (particle, first slot, cell rank, count), one
