@@ -73,6 +73,7 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
       k_ray        work item 16 + x, v 24 per item (triangle data counted in k_tri_box)
       k_cells_b    x, v, F and first-hit key read 40, x, v, F written 36
                                                                    76 / particle
+      k_ray+k_cells_b  both, or the fused k_ray_b                  40 / item + 76 / particle
       k_vein_springs+k_tri_box  X, V read 24, 9 neighbour ids + lengths 72,
                    Fv read + written 24 per vertex; perm + indices 16, the
                    float4 triangle data written 48 and one box (32) per
@@ -87,6 +88,7 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
         "k_cells_a": 88 * np_ + 12 * nc + 10 * n_occ + 16 * n_items,
         "k_ray": 40 * n_items,
         "k_cells_b": 76 * np_,
+        "k_ray+k_cells_b": 40 * n_items + 76 * np_,
         "k_vein_springs+k_tri_box": 120 * nv + 76 * nt,
         "k_vein": 76 * nv,
     }

@@ -20,6 +20,7 @@ namespace bcs {
 void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st);
 void launch_ray(const StepArgs& A, hipStream_t st);
 void launch_cells_b(const StepArgs& A, int nchunks, bool fused, hipStream_t st);
+void launch_ray_b(const StepArgs& A, hipStream_t st);
 void launch_integrate(const StepArgs& A, hipStream_t st);
 void launch_vein_end(const StepArgs& A, int nchunks, hipStream_t st);
 void launch_bump(uint64_t* step, hipStream_t st);
@@ -83,6 +84,9 @@ struct bcs_sim {
     uint32_t* best = nullptr;  // per particle first-hit key
     uint4* work = nullptr;     // ray work items
     uint32_t* wcount = nullptr;  // per k_cells_a block: work items
+    uint32_t* region_done = nullptr;  // per region: k_ray_b blocks finished (kept 0 between frames)
+    bool fuse_ray_b = false;          // fused frames run k_ray_b (few regions) or k_ray + k_cells_b
+    bool fuse_ray_b_auto = false;     // the size rule's choice (mode 0)
     uint32_t region_cap = 0;     // work items per block region
     int nblocks = 0;             // k_cells_a grid (8 * ceil(nchunks / 8))
     // grids
@@ -135,7 +139,7 @@ struct bcs_sim {
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.hits = hits; A.vhead = vhead; A.vnext = vnext; A.hctrl = hctrl;
         A.t_aabb = t_aabb; A.t_nmask = t_nmask;
-        A.best = best; A.work = work; A.wcount = wcount; A.region_cap = region_cap;
+        A.best = best; A.work = work; A.wcount = wcount; A.region_cap = region_cap; A.region_done = region_done;
         A.flags = flags; A.step = step;
         return A;
     }
@@ -234,21 +238,23 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
     launch_cells_a(A, s->nchunks, st);
     te(kGridKernels + 1);
     tb(kGridKernels + 2);
-    launch_ray(A, st);
+    if (s->fuse_ray_b) {
+        launch_ray_b(A, st);
+    } else {
+        launch_ray(A, st);
+        launch_cells_b(A, s->nchunks, true, st);
+    }
     te(kGridKernels + 2);
     tb(kGridKernels + 3);
-    launch_cells_b(A, s->nchunks, true, st);
-    te(kGridKernels + 3);
-    tb(kGridKernels + 4);
     launch_vein(Va, 7, st);
-    te(kGridKernels + 4);
+    te(kGridKernels + 3);
     HIPCHK(hipGetLastError());
     return BCS_OK;
 }
 
 const char* kKernelNames =
-    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_springs+k_tri_box;k_cells_a;k_ray;k_cells_b;k_vein";
-constexpr int kFusedKernels = kGridKernels + 5;
+    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_springs+k_tri_box;k_cells_a;k_ray+k_cells_b;k_vein";
+constexpr int kFusedKernels = kGridKernels + 4;
 
 }  // namespace
 
@@ -422,6 +428,16 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     // ray work list: 8 items per particle, split evenly over the k_cells_a blocks
     s->nblocks = 8 * ((s->nchunks + 7) / 8);
     {
+        // While every region's blocks are resident at once (up to ~8 per CU)
+        // the frame is latency-bound and one launch for ray + hit response
+        // saves a dependent launch; beyond that the separate kernels keep the
+        // ray loop at its higher occupancy.
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) cus = 0;
+        s->fuse_ray_b_auto = cus > 0 && s->nblocks <= 8 * cus;
+        s->fuse_ray_b = s->fuse_ray_b_auto;
+    }
+    {
         const int32_t items = phys->ray_work_items;
         const uint64_t nb = (uint64_t)std::max(s->nblocks, 1);
         const uint64_t per = items > 0 ? std::max<uint64_t>((uint64_t)items / nb, 1)
@@ -431,6 +447,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(s->dalloc(&s->best, N));
     RC(s->dalloc(&s->work, (size_t)s->region_cap * std::max(s->nblocks, 1)));
     RC(s->dalloc(&s->wcount, std::max(s->nblocks, 1)));
+    RC(s->dalloc(&s->region_done, std::max(s->nblocks, 1)));
     // ---- uploads
     HIPCHK(hipMemsetAsync(s->pos, 0, 12 * std::max<size_t>(N, 1), s->stream));
     HIPCHK(hipMemsetAsync(s->vel, 0, 12 * std::max<size_t>(N, 1), s->stream));
@@ -476,6 +493,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     HIPCHK(hipMemsetAsync(s->flags, 0, 4, s->stream));
     HIPCHK(hipMemsetAsync(s->step, 0, 8, s->stream));
     HIPCHK(hipMemsetAsync(s->wcount, 0, 4 * (size_t)std::max(s->nblocks, 1), s->stream));
+    HIPCHK(hipMemsetAsync(s->region_done, 0, 4 * (size_t)std::max(s->nblocks, 1), s->stream));
     // ---- grids
     const GridGeom pgeom = make_geom(bmin, bmax, phys->particle_cell);
     const GridGeom tgeom = make_geom(bmin, bmax, phys->triangle_cell);
@@ -715,6 +733,20 @@ extern "C" int bcs_sim_use_graph(bcs_sim* s, int32_t enabled)
 {
     if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
     s->use_graph = enabled != 0;
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_set_ray_fusion(bcs_sim* s, int32_t mode)
+{
+    if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
+    if (mode < 0 || mode > 2) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_set_ray_fusion: mode must be 0, 1 or 2");
+    RC(set_dev(s));
+    const bool f = mode == 0 ? s->fuse_ray_b_auto : mode == 1;
+    if (f != s->fuse_ray_b) {
+        HIPCHK(hipStreamSynchronize(s->stream));
+        drop_graph(s);  // the captured frame holds the other kernels
+        s->fuse_ray_b = f;
+    }
     return BCS_OK;
 }
 

@@ -47,12 +47,13 @@ __device__ __forceinline__ int type_of_index(const TypeTable& tt, int32_t s)
 // the chunk list (and, with corder spatially sorted, one compact region of the
 // vein, whose grid-table lines then stay in that XCD's L2).  The grid is
 // 8 * ceil(nchunks / 8) blocks; blocks past the end return -1.
-__device__ __forceinline__ int chunk_of_block(int32_t nchunks)
+__device__ __forceinline__ int chunk_of_block(int32_t nchunks, uint32_t block)
 {
     const int per = (nchunks + 7) >> 3;
-    const int c = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    const int c = (int)(block & 7u) * per + (int)(block >> 3);
     return c < nchunks ? c : -1;
 }
+__device__ __forceinline__ int chunk_of_block(int32_t nchunks) { return chunk_of_block(nchunks, blockIdx.x); }
 
 // Particle of vertex a of the cl-th cell of chunk (T, first).
 __device__ __forceinline__ int32_t chunk_particle(const StepArgs& A, int T, int first, int cl, int a, int p,
@@ -617,12 +618,9 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
 
 // One thread per work item: first hit among its <= 16 triangles, folded into
 // the particle's minimum (cell rank, sorted slot).  One block per work region.
-__global__ void __launch_bounds__(256) k_ray(StepArgs A)
+// The items of work region r taken by block `sub` of its kRaySplit blocks.
+__device__ __forceinline__ void ray_region(const StepArgs& A, uint32_t r, uint32_t sub)
 {
-    // kRaySplit blocks share the work region of k_cells_a block r; block
-    // b = s * nregions + r runs on the XCD (b % 8 == r % 8) whose L2 holds it
-    const uint32_t nreg = gridDim.x / kRaySplit;
-    const uint32_t r = blockIdx.x % nreg, sub = blockIdx.x / nreg;
     const uint32_t n = min(A.wcount[r], A.region_cap);
     const uint4* items = A.work + (size_t)r * A.region_cap;
     const int64_t np = A.tt.np;
@@ -660,15 +658,53 @@ __global__ void __launch_bounds__(256) k_ray(StepArgs A)
     }
 }
 
-template <bool FUSED>
-__global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
+__global__ void __launch_bounds__(256) k_ray(StepArgs A)
 {
+    // kRaySplit blocks share the work region of k_cells_a block r; block
+    // b = s * nregions + r runs on the XCD (b % 8 == r % 8) whose L2 holds it
+    const uint32_t nreg = gridDim.x / kRaySplit;
+    ray_region(A, blockIdx.x % nreg, blockIdx.x / nreg);
+}
+
+template <bool FUSED>
+__device__ __forceinline__ void cells_b_body(const StepArgs& A, int chunk, int* s_tele);
+
+// k_ray and k_cells_b<true> in one launch: the kRaySplit blocks of region r
+// count themselves in after their items; the last to arrive runs the hit
+// response, integration and vein end of the chunk of k_cells_a block r.  No
+// block waits.  The only data passed between the blocks is best[], written
+// with device atomics (atomicMin) and read back with agent-scope atomic loads;
+// every wave waits until its atomics are acknowledged (s_waitcnt on all
+// counters) before the block's count, so the last block's loads see them.  A
+// device-scope fence would also write back and invalidate the L2 in every
+// block, which costs far more than the work here.
+__global__ void __launch_bounds__(256) k_ray_b(StepArgs A)
+{
+    static_assert(kBlock == 256, "the chunk body needs the whole block");
     __shared__ int s_tele[kBlock];
+    __shared__ uint32_t s_last;
+    const uint32_t nreg = gridDim.x / kRaySplit;
+    const uint32_t r = blockIdx.x % nreg;
+    ray_region(A, r, blockIdx.x / nreg);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's atomicMin acknowledged ...
+    __syncthreads();                // ... in every wave, before the count
+    if (threadIdx.x == 0) s_last = atomicAdd(&A.region_done[r], 1u) == kRaySplit - 1 ? 1u : 0u;
+    __syncthreads();
+    if (!s_last) return;
+    if (threadIdx.x == 0) A.region_done[r] = 0u;  // all kRaySplit arrived: ready for the next frame
+    const int chunk = chunk_of_block(A.nchunks, r);
+    if (chunk < 0) return;
+    cells_b_body<true>(A, chunk, s_tele);
+}
+
+// Hit response, integration and vein end of one chunk (one block of kBlock
+// threads; the chunk of k_cells_a block `block`).
+template <bool FUSED>
+__device__ __forceinline__ void cells_b_body(const StepArgs& A, int chunk, int* s_tele)
+{
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
     const int l = threadIdx.x;
-    const int chunk = chunk_of_block(A.nchunks);
-    if (chunk < 0) return;
     const int T = A.chunk_type[chunk];
     const int first = A.chunk_first[chunk];
     const int ncell = A.chunk_n[chunk];
@@ -686,7 +722,8 @@ __global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
         x1 = load3(A.pos, np, i);
         v1 = load3(A.vel, np, i);
         F = load3(A.frc, np, i);
-        const uint32_t best = A.best[i];
+        // atomic load: in k_ray_b other blocks set it in this launch
+        const uint32_t best = __hip_atomic_load(&A.best[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const f3 dir = normalize(v1, fl);
         float rt = 1e10f;
         f3 normal = mk(0.f, 0.f, 0.f), refl = mk(0.f, 0.f, 0.f);
@@ -761,6 +798,15 @@ __global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
         A.frc[i] = F.x; A.frc[np + i] = F.y; A.frc[2 * np + i] = F.z;
     }
     if (fl) atomicOr(A.flags, fl);
+}
+
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
+{
+    __shared__ int s_tele[kBlock];
+    const int chunk = chunk_of_block(A.nchunks);
+    if (chunk < 0) return;
+    cells_b_body<FUSED>(A, chunk, s_tele);
 }
 
 // propagateParticleForcesKernel, src/objects/blood_cells.cu:155-179 (API path).
@@ -1190,6 +1236,10 @@ void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st)
 void launch_ray(const StepArgs& A, hipStream_t st)
 {
     if (A.nchunks > 0) hipLaunchKernelGGL(k_ray, dim3(chunk_grid(A.nchunks).x * kRaySplit), dim3(256), 0, st, A);
+}
+void launch_ray_b(const StepArgs& A, hipStream_t st)
+{
+    if (A.nchunks > 0) hipLaunchKernelGGL(k_ray_b, dim3(chunk_grid(A.nchunks).x * kRaySplit), dim3(256), 0, st, A);
 }
 void launch_cells_b(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
 {

@@ -87,6 +87,7 @@ struct StepArgs {
     uint4* work;              // ray work items: (particle, first slot, cell rank, count), one
                               // region of region_cap items per k_cells_a block
     uint32_t* wcount;         // items per region (last frame)
+    uint32_t* region_done;    // per region: k_ray_b blocks finished (0 between launches)
     uint32_t region_cap;
     uint32_t* flags;
     uint64_t* step;
