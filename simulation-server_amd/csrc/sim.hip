@@ -96,9 +96,10 @@ struct bcs_sim {
     // misc
     uint32_t* flags = nullptr;
     uint64_t* step = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec = nullptr;    // frame without the leading vein update ...
+    hipGraphExec_t graph_exec_b = nullptr;  // ... and with it (the previous frame's k_vein)
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr, graph_b = nullptr;
     std::vector<float> host_bs;
     std::vector<float> host_smallest;
 
@@ -106,6 +107,8 @@ struct bcs_sim {
     {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (graph_exec_b) (void)hipGraphExecDestroy(graph_exec_b);
+        if (graph_b) (void)hipGraphDestroy(graph_b);
         grid_free(pg);
         grid_free(tg);
         for (void* p : allocs) (void)hipFree(p);
@@ -209,7 +212,12 @@ int enqueue_particle_grid(bcs_sim* s, const KernelTimer* timer = nullptr)
 
 // One fused frame: particle grid, vein springs, k_cells (springs, contacts,
 // vein ray, integrate, vein end), vein-hit segments, vein sum + integrate.
-int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
+// lead_vein: the previous frame's vein update (k_vein) has not run yet; it
+// heads the vein branch (it needs nothing from this frame's particle grid).
+// tail_vein: this frame's vein update runs at its end (else the caller runs
+// it, as the next frame's lead or on its own).
+int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lead_vein = false,
+                        bool tail_vein = true)
 {
     hipStream_t st = s->stream;
     auto tb = [&](int id) { if (timer) timer->begin(id, st); };
@@ -222,12 +230,14 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
         // second branch runs them while the particle grid is built.
         HIPCHK(hipEventRecord(s->ev_fork, st));
         HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+        if (lead_vein) launch_vein(Va, 7, s->side);
         launch_vein_springs(Va, s->side);
         launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, s->side);
         HIPCHK(hipEventRecord(s->ev_join, s->side));
         RC(enqueue_particle_grid(s));
         HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
     } else {
+        if (lead_vein) launch_vein(Va, 7, st);
         RC(enqueue_particle_grid(s, timer));
         tb(kGridKernels + 0);
         launch_vein_springs(Va, st);
@@ -246,7 +256,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr)
     }
     te(kGridKernels + 2);
     tb(kGridKernels + 3);
-    launch_vein(Va, 7, st);
+    if (tail_vein) launch_vein(Va, 7, st);
     te(kGridKernels + 3);
     HIPCHK(hipGetLastError());
     return BCS_OK;
@@ -685,15 +695,21 @@ static void drop_graph(bcs_sim* s)
 {
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
-    s->graph_exec = nullptr;
-    s->graph = nullptr;
+    if (s->graph_exec_b) (void)hipGraphExecDestroy(s->graph_exec_b);
+    if (s->graph_b) (void)hipGraphDestroy(s->graph_b);
+    s->graph_exec = s->graph_exec_b = nullptr;
+    s->graph = s->graph_b = nullptr;
 }
 
-static int ensure_graph(bcs_sim* s)
+// The vein update of frame f overlaps the particle grid build of frame f+1
+// unless the triangle grid is rebuilt from the vein every frame (that build
+// precedes the branches and needs the updated vein).
+static bool defer_vein(const bcs_sim* s) { return !s->phys.dynamic_triangle_grid; }
+
+static int capture_frame(bcs_sim* s, bool lead_vein, hipGraph_t* graph, hipGraphExec_t* exec)
 {
-    if (s->graph_exec) return BCS_OK;
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_fused_frame(s);
+    int rc = enqueue_fused_frame(s, nullptr, lead_vein, !defer_vein(s));
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (rc) {
@@ -701,8 +717,15 @@ static int ensure_graph(bcs_sim* s)
         return rc;
     }
     if (e != hipSuccess) return fail(BCS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    s->graph = g;
-    HIPCHK(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+    *graph = g;
+    HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return BCS_OK;
+}
+
+static int ensure_graph(bcs_sim* s)
+{
+    if (!s->graph_exec) RC(capture_frame(s, false, &s->graph, &s->graph_exec));
+    if (!s->graph_exec_b && defer_vein(s)) RC(capture_frame(s, true, &s->graph_b, &s->graph_exec_b));
     return BCS_OK;
 }
 
@@ -720,10 +743,13 @@ extern "C" int bcs_sim_step(bcs_sim* s, int32_t n_steps)
                             s->octrl, s->corder, s->flags, s->stream));
             s->frames_since_order = 0;
         }
-        if (s->use_graph) HIPCHK(hipGraphLaunch(s->graph_exec, s->stream));
-        else RC(enqueue_fused_frame(s));
+        // frames after the first start with the previous frame's vein update
+        const bool lead = defer_vein(s) && k > 0;
+        if (s->use_graph) HIPCHK(hipGraphLaunch(lead ? s->graph_exec_b : s->graph_exec, s->stream));
+        else RC(enqueue_fused_frame(s, nullptr, lead, !defer_vein(s)));
         ++s->frames_since_order;
     }
+    if (defer_vein(s)) launch_vein(s->vein_args(), 7, s->stream);  // the last frame's vein update
     s->copies_current = false;  // the frames moved the particles after their grid builds
     HIPCHK(hipGetLastError());
     return BCS_OK;
