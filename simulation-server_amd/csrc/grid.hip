@@ -43,7 +43,8 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
                                              uint2* __restrict__ table, int clear_table, BrickMap bm,
                                              const uint32_t* __restrict__ gpos, uint32_t* __restrict__ bcount,
                                              uint64_t* __restrict__ status, int32_t ntiles,
-                                             uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags)
+                                             uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags,
+                                             uint32_t* __restrict__ arrival)
 {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -71,7 +72,9 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
     if (gid < n) {
         uint32_t k = cell_key(g, x, y, z, fl);
         keys[gid] = k;
-        atomicAdd(&bcount[k >> shift], 1u);
+        // the count before this object: its slot within the bucket (the
+        // scatter needs no atomics; slot order within a bucket is free)
+        arrival[gid] = atomicAdd(&bcount[k >> shift], 1u);
     }
     if (fl) atomicOr(flags, fl);
 }
@@ -79,11 +82,13 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
 // Exclusive scan of `count[0..m)` into out[0..m] (out[m] = total), single
 // pass with decoupled look-back over tiles taken in ticket order.  Entries
 // larger than big_thr are appended to biglist (when biglist != nullptr).
-__global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restrict__ count, uint32_t* __restrict__ out,
+// clear_count: the histogram is zeroed as it is read (ready for the next
+// build).
+__global__ void __launch_bounds__(kScanThreads) k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ out,
                                                        int64_t m, uint64_t* __restrict__ status, int32_t ntiles,
                                                        uint32_t* __restrict__ ctrl, uint32_t* __restrict__ biglist,
                                                        uint32_t big_thr, uint32_t big_cap,
-                                                       uint32_t* __restrict__ flags)
+                                                       uint32_t* __restrict__ flags, int clear_count)
 {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kScanThreads / 64];
@@ -109,9 +114,17 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restric
             v[4 * q + 2] = t.z;
             v[4 * q + 3] = t.w;
         }
+        if (clear_count) {  // the histogram starts the next build at zero
+            uint4* w4 = reinterpret_cast<uint4*>(count + base);
+#pragma unroll
+            for (int q = 0; q < kScanItems / 4; ++q) w4[q] = make_uint4(0u, 0u, 0u, 0u);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) v[k] = (base + k < m) ? count[base + k] : 0u;
+        if (clear_count)
+            for (int k = 0; k < kScanItems; ++k)
+                if (base + k < m) count[base + k] = 0u;
     }
     const int lane = tid & 63, wave = tid >> 6;
     uint32_t nbig = 0;
@@ -205,8 +218,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(const uint32_t* __restric
 // brec (nullable): the object's position and velocity travel with its item
 // (read here in object order, copied to the sorted slot by k_rank).
 __global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ keys, int32_t n, int shift,
-                                                 const uint32_t* __restrict__ bstart, uint32_t* __restrict__ bcount,
-                                                 uint2* __restrict__ items,
+                                                 const uint32_t* __restrict__ bstart,
+                                                 const uint32_t* __restrict__ arrival, uint2* __restrict__ items,
                                                  uint32_t* __restrict__ flags, const float* __restrict__ xyz,
                                                  const float* __restrict__ vel, int64_t stride,
                                                  float4* __restrict__ brec)
@@ -219,11 +232,11 @@ __global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ ke
         rv = make_float4(vel[i], vel[stride + i], vel[2 * stride + i], 0.0f);
     }
     const uint32_t k = keys[i];
+    const uint32_t a = arrival[i];
     const uint32_t b = k >> shift;
-    // counts return to zero: the histogram is ready for the next build
-    const uint32_t old = atomicSub(&bcount[b], 1u);
-    const uint32_t pos = bstart[b] + old - 1u;
-    if (old == 0u || pos >= (uint32_t)n) {  // inconsistent histogram: never write out of bounds
+    const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
+    const uint32_t pos = s0 + a;
+    if (pos >= s1 || s1 > (uint32_t)n) {  // inconsistent histogram: never write out of bounds
         atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);
         return;
     }
@@ -530,6 +543,7 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     const size_t n = (size_t)std::max(cap, 1);
     GRID_HIP(hipMalloc(&G.keys, n * 4));
     GRID_HIP(hipMalloc(&G.items, n * 8));
+    GRID_HIP(hipMalloc(&G.arrival, n * 4));
     GRID_HIP(hipMalloc(&G.keys_sorted, n * 4));
     if (with_runs) GRID_HIP(hipMalloc(&G.runs, n * 4));
     if (with_copies) {
@@ -634,7 +648,7 @@ int grid_shift_for(const GridGeom& g, const float* pos, int64_t stride, int32_t 
 
 void grid_free(DeviceGrid& G)
 {
-    void* ptrs[] = {G.keys, G.items, G.keys_sorted, G.runs, G.perm, G.slot_of, G.brec, G.pvs,
+    void* ptrs[] = {G.keys, G.items, G.keys_sorted, G.runs, G.perm, G.slot_of, G.brec, G.pvs, G.arrival,
                     G.table, G.bm.words, G.gpos, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -656,15 +670,15 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     tb(0);
     hipLaunchKernelGGL(k_key, dim3(std::max(blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride, n,
                        n_prev, G.shift, G.keys, G.table, G.clear_table ? 1 : 0, G.bm, G.gpos, G.bcount, G.status,
-                       G.ntiles, G.ctrl, flags);
+                       G.ntiles, G.ctrl, flags, G.arrival);
     te(0);
     tb(1);
     hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,
-                       G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold, G.big_cap, flags);
+                       G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold, G.big_cap, flags, 1);
     te(1);
     tb(2);
     float4* brec = vel ? G.brec : nullptr;
-    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.bcount, G.items,
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.arrival, G.items,
                        flags, pos, vel, stride, brec);
     te(2);
     RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr};
@@ -685,8 +699,10 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
 int scan_exclusive(const uint32_t* count, uint32_t* out, int64_t m, uint64_t* status, int32_t ntiles, uint32_t* ctrl,
                    uint32_t* flags, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, count, out, m, status, ntiles, ctrl,
-                       (uint32_t*)nullptr, 0u, 0u, flags);
+    // count is only read here (clear_count = 0)
+    hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, const_cast<uint32_t*>(count), out, m, status,
+                       ntiles, ctrl,
+                       (uint32_t*)nullptr, 0u, 0u, flags, 0);
     GRID_HIP(hipGetLastError());
     return BCS_OK;
 }

@@ -40,7 +40,8 @@ struct DeviceGrid {
     float4* brec = nullptr;           // per bucketed item q (optional): brec[2q] = (x, y, z, object
                                       // index as int bits), brec[2q+1] = (vx, vy, vz, 0)
     float4* pvs = nullptr;            // the same records per sorted slot
-    uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds
+    uint32_t* arrival = nullptr;      // per object: its slot within its bucket (histogram order)
+    uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds (the scan clears it)
     uint32_t* bstart = nullptr;       // bucket offsets [nb+2]
     uint64_t* status = nullptr;       // scan tile status
     uint32_t* ctrl = nullptr;         // [0] scan ticket, [1] big-bucket count, [2] run count
