@@ -64,16 +64,16 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
       k_occ        sorted key read 4, map-word slot written 4; count read 8
                    and 2-bit code written (<= 2 bytes) per occupied cell
                                                                    8 / particle + 10 / cell
-      k_cells_a    x, v, F read 36, own key 4, F and first-hit key written 16,
-                   sorted-slot records of the contact candidates read 32;
-                   centre written 12 per blood cell; table entry + map code
-                   read 10 per occupied cell; ray work items written 16 each
-                                                                   88 / particle + 12 / blood cell
-                                                                   + 10 / cell + 16 / item
+      k_raycand    x, v read 24, first-hit key written 4; work items written
+                   16 each (triangle-cell boxes counted in k_tri_box)
+                                                                   28 / particle + 16 / item
       k_ray        work item 16 + x, v 24 per item (triangle data counted in k_tri_box)
-      k_cells_b    x, v, F and first-hit key read 40, x, v, F written 36
-                                                                   76 / particle
-      k_ray+k_cells_b  both, or the fused k_ray_b                  40 / item + 76 / particle
+      k_cells      x, v, F read 36, own key 4, sorted-slot records of the
+                   contact candidates read 32, first-hit key read 4, x, v, F
+                   written 36; centre written 12 per blood cell; table entry
+                   + map code read 10 per occupied cell
+                                                                   112 / particle + 12 / blood cell
+                                                                   + 10 / cell
       k_vein_springs+k_tri_box  X, V read 24, 9 neighbour ids + lengths 72,
                    Fv read + written 24 per vertex; perm + indices 16, the
                    float4 triangle data written 48 and one box (32) per
@@ -85,10 +85,9 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
         "k_scatter": 68 * np_,
         "k_rank": 84 * np_ + 8 * n_occ,
         "k_occ": 8 * np_ + 10 * n_occ,
-        "k_cells_a": 88 * np_ + 12 * nc + 10 * n_occ + 16 * n_items,
+        "k_raycand": 28 * np_ + 16 * n_items,
         "k_ray": 40 * n_items,
-        "k_cells_b": 76 * np_,
-        "k_ray+k_cells_b": 40 * n_items + 76 * np_,
+        "k_cells": 112 * np_ + 12 * nc + 10 * n_occ,
         "k_vein_springs+k_tri_box": 120 * nv + 76 * nt,
         "k_vein": 76 * nv,
     }

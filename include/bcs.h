@@ -248,13 +248,6 @@ int bcs_sim_step(bcs_sim* sim, int32_t n_steps);
  * counterpart in the reference (its frames are direct launches). */
 int bcs_sim_use_graph(bcs_sim* sim, int32_t enabled);
 
-/* How fused frames run the vein ray test and the hit response: mode 0 picks
- * by size (one launch, k_ray_b, while the whole grid of blocks is resident at
- * once, up to ~8 blocks per CU; else two launches), 1 always fuses, 2 never.
- * Results are identical in every mode.  No counterpart in the reference (a
- * tuning switch of this implementation). */
-int bcs_sim_set_ray_fusion(bcs_sim* sim, int32_t mode);
-
 int bcs_sim_synchronize(bcs_sim* sim);
 
 /* Time n fused steps with HIP events on the simulation's stream (ms total). */

@@ -215,7 +215,6 @@ def _sig(lib):
     lib.bcs_sim_propagate_all.argtypes = [vp]
     lib.bcs_sim_step.argtypes = [vp, C.c_int32]
     lib.bcs_sim_use_graph.argtypes = [vp, C.c_int32]
-    lib.bcs_sim_set_ray_fusion.argtypes = [vp, C.c_int32]
     lib.bcs_sim_synchronize.argtypes = [vp]
     lib.bcs_sim_time_steps.argtypes = [vp, C.c_int32, f32p]
     lib.bcs_sim_profile_kernels.argtypes = [vp, C.c_int32, f32p, C.c_int32, i32p, C.c_char_p, C.c_int32]
@@ -477,10 +476,6 @@ class Simulation:
 
     def use_graph(self, enabled: bool = True):
         check(self.lib.bcs_sim_use_graph(self.h, 1 if enabled else 0), "use_graph")
-
-    def set_ray_fusion(self, mode: int = 0):
-        """0: by size, 1: ray test and hit response in one launch, 2: two launches."""
-        check(self.lib.bcs_sim_set_ray_fusion(self.h, int(mode)), "set_ray_fusion")
 
     def synchronize(self):
         check(self.lib.bcs_sim_synchronize(self.h), "synchronize")

@@ -17,10 +17,9 @@
 #include <vector>
 
 namespace bcs {
-void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st);
+void launch_raycand(const StepArgs& A, int nchunks, hipStream_t st);
 void launch_ray(const StepArgs& A, hipStream_t st);
-void launch_cells_b(const StepArgs& A, int nchunks, bool fused, hipStream_t st);
-void launch_ray_b(const StepArgs& A, hipStream_t st);
+void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st);
 void launch_integrate(const StepArgs& A, hipStream_t st);
 void launch_vein_end(const StepArgs& A, int nchunks, hipStream_t st);
 void launch_bump(uint64_t* step, hipStream_t st);
@@ -83,12 +82,9 @@ struct bcs_sim {
     float4* tdata = nullptr;   // per sorted triangle: (v0 | id), e1, e2 (per frame)
     uint32_t* best = nullptr;  // per particle first-hit key
     uint4* work = nullptr;     // ray work items
-    uint32_t* wcount = nullptr;  // per k_cells_a block: work items
-    uint32_t* region_done = nullptr;  // per region: k_ray_b blocks finished (kept 0 between frames)
-    bool fuse_ray_b = false;          // fused frames run k_ray_b (few regions) or k_ray + k_cells_b
-    bool fuse_ray_b_auto = false;     // the size rule's choice (mode 0)
+    uint32_t* wcount = nullptr;  // per k_raycand block: work items
     uint32_t region_cap = 0;     // work items per block region
-    int nblocks = 0;             // k_cells_a grid (8 * ceil(nchunks / 8))
+    int nblocks = 0;             // k_raycand / k_cells grid (8 * ceil(nchunks / 8))
     // grids
     DeviceGrid pg, tg;
     bool tg_built = false;
@@ -142,7 +138,7 @@ struct bcs_sim {
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.hits = hits; A.vhead = vhead; A.vnext = vnext; A.hctrl = hctrl;
         A.t_aabb = t_aabb; A.t_nmask = t_nmask;
-        A.best = best; A.work = work; A.wcount = wcount; A.region_cap = region_cap; A.region_done = region_done;
+        A.best = best; A.work = work; A.wcount = wcount; A.region_cap = region_cap;
         A.flags = flags; A.step = step;
         return A;
     }
@@ -210,8 +206,10 @@ int enqueue_particle_grid(bcs_sim* s, const KernelTimer* timer = nullptr)
 }
 
 
-// One fused frame: particle grid, vein springs, k_cells (springs, contacts,
-// vein ray, integrate, vein end), vein-hit segments, vein sum + integrate.
+// One fused frame.  Main branch: particle grid, then k_cells (centres,
+// springs, contacts, vein hit response, integrate, vein end).  Vein branch,
+// beside the grid build: triangle data and boxes, vein ray candidates, ray
+// test, vein springs.  Then the vein sum + integrate (k_vein).
 // lead_vein: the previous frame's vein update (k_vein) has not run yet; it
 // heads the vein branch (it needs nothing from this frame's particle grid).
 // tail_vein: this frame's vein update runs at its end (else the caller runs
@@ -226,13 +224,16 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
     const StepArgs A = s->step_args();
     const VeinArgs Va = s->vein_args();
     if (!timer) {
-        // Vein springs and the triangle data depend only on the vein state: a
-        // second branch runs them while the particle grid is built.
+        // The vein ray test needs only the frame's starting x, v and the vein
+        // state, and the vein springs only the vein state: a second branch
+        // runs them while the particle grid is built.
         HIPCHK(hipEventRecord(s->ev_fork, st));
         HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
         if (lead_vein) launch_vein(Va, 7, s->side);
-        launch_vein_springs(Va, s->side);
         launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, s->side);
+        launch_raycand(A, s->nchunks, s->side);
+        launch_ray(A, s->side);
+        launch_vein_springs(Va, s->side);
         HIPCHK(hipEventRecord(s->ev_join, s->side));
         RC(enqueue_particle_grid(s));
         HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
@@ -243,28 +244,26 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         launch_vein_springs(Va, st);
         launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, st);
         te(kGridKernels + 0);
-    }
-    tb(kGridKernels + 1);
-    launch_cells_a(A, s->nchunks, st);
-    te(kGridKernels + 1);
-    tb(kGridKernels + 2);
-    if (s->fuse_ray_b) {
-        launch_ray_b(A, st);
-    } else {
+        tb(kGridKernels + 1);
+        launch_raycand(A, s->nchunks, st);
+        te(kGridKernels + 1);
+        tb(kGridKernels + 2);
         launch_ray(A, st);
-        launch_cells_b(A, s->nchunks, true, st);
+        te(kGridKernels + 2);
     }
-    te(kGridKernels + 2);
     tb(kGridKernels + 3);
-    if (tail_vein) launch_vein(Va, 7, st);
+    launch_cells(A, s->nchunks, true, st);
     te(kGridKernels + 3);
+    tb(kGridKernels + 4);
+    if (tail_vein) launch_vein(Va, 7, st);
+    te(kGridKernels + 4);
     HIPCHK(hipGetLastError());
     return BCS_OK;
 }
 
 const char* kKernelNames =
-    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_springs+k_tri_box;k_cells_a;k_ray+k_cells_b;k_vein";
-constexpr int kFusedKernels = kGridKernels + 4;
+    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_springs+k_tri_box;k_raycand;k_ray;k_cells;k_vein";
+constexpr int kFusedKernels = kGridKernels + 5;
 
 }  // namespace
 
@@ -435,18 +434,8 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(s->dalloc(&s->hctrl, 16));
     RC(s->dalloc(&s->flags, 4)); RC(s->dalloc(&s->step, 1));
     RC(s->dalloc(&s->tdata, 3 * NT));
-    // ray work list: 8 items per particle, split evenly over the k_cells_a blocks
+    // ray work list: 8 items per particle, split evenly over the k_raycand blocks
     s->nblocks = 8 * ((s->nchunks + 7) / 8);
-    {
-        // While every region's blocks are resident at once (up to ~8 per CU)
-        // the frame is latency-bound and one launch for ray + hit response
-        // saves a dependent launch; beyond that the separate kernels keep the
-        // ray loop at its higher occupancy.
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) cus = 0;
-        s->fuse_ray_b_auto = cus > 0 && s->nblocks <= 8 * cus;
-        s->fuse_ray_b = s->fuse_ray_b_auto;
-    }
     {
         const int32_t items = phys->ray_work_items;
         const uint64_t nb = (uint64_t)std::max(s->nblocks, 1);
@@ -457,7 +446,6 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(s->dalloc(&s->best, N));
     RC(s->dalloc(&s->work, (size_t)s->region_cap * std::max(s->nblocks, 1)));
     RC(s->dalloc(&s->wcount, std::max(s->nblocks, 1)));
-    RC(s->dalloc(&s->region_done, std::max(s->nblocks, 1)));
     // ---- uploads
     HIPCHK(hipMemsetAsync(s->pos, 0, 12 * std::max<size_t>(N, 1), s->stream));
     HIPCHK(hipMemsetAsync(s->vel, 0, 12 * std::max<size_t>(N, 1), s->stream));
@@ -503,7 +491,6 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     HIPCHK(hipMemsetAsync(s->flags, 0, 4, s->stream));
     HIPCHK(hipMemsetAsync(s->step, 0, 8, s->stream));
     HIPCHK(hipMemsetAsync(s->wcount, 0, 4 * (size_t)std::max(s->nblocks, 1), s->stream));
-    HIPCHK(hipMemsetAsync(s->region_done, 0, 4 * (size_t)std::max(s->nblocks, 1), s->stream));
     // ---- grids
     const GridGeom pgeom = make_geom(bmin, bmax, phys->particle_cell);
     const GridGeom tgeom = make_geom(bmin, bmax, phys->triangle_cell);
@@ -667,9 +654,9 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
     const VeinArgs Va = s->vein_args();
     launch_vein_springs(Va, s->stream);
     launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, s->stream);
-    launch_cells_a(A, s->nchunks, s->stream);
+    launch_raycand(A, s->nchunks, s->stream);
     launch_ray(A, s->stream);
-    launch_cells_b(A, s->nchunks, false, s->stream);
+    launch_cells(A, s->nchunks, false, s->stream);
     launch_vein(Va, 1, s->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -759,20 +746,6 @@ extern "C" int bcs_sim_use_graph(bcs_sim* s, int32_t enabled)
 {
     if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
     s->use_graph = enabled != 0;
-    return BCS_OK;
-}
-
-extern "C" int bcs_sim_set_ray_fusion(bcs_sim* s, int32_t mode)
-{
-    if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
-    if (mode < 0 || mode > 2) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_set_ray_fusion: mode must be 0, 1 or 2");
-    RC(set_dev(s));
-    const bool f = mode == 0 ? s->fuse_ray_b_auto : mode == 1;
-    if (f != s->fuse_ray_b) {
-        HIPCHK(hipStreamSynchronize(s->stream));
-        drop_graph(s);  // the captured frame holds the other kernels
-        s->fuse_ray_b = f;
-    }
     return BCS_OK;
 }
 

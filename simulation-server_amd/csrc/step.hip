@@ -1,23 +1,25 @@
 // step.hip -- physics kernels of one frame for gfx950.
 //
-// The particle part of a frame is three kernels.  k_cells_a and k_cells_b
-// give one block a chunk of whole blood cells of one type (<= 256 particles).
-//   k_cells_a  centres   calculateBloodCellsCenters   src/objects/blood_cells.cu:44-61
-//              springs   gatherForcesKernel (Heun)     src/objects/blood_cells.cu:66-120
-//              contacts  calculateParticleCollisions   src/simulation/particle_collisions.cuh:104-269
-//              vein ray candidates: the triangle cells of the 27-cell
+// The particle part of a frame is three kernels.  k_raycand and k_cells give
+// one block a chunk of whole blood cells of one type (<= 256 particles).
+//   k_raycand  vein ray candidates: the triangle cells of the 27-cell
 //              neighbourhood whose box the ray can reach, as work items of
-//              <= 16 sorted triangles
+//              <= 16 sorted triangles (needs only the frame's starting x, v
+//              and the triangle data: runs beside the particle grid build)
 //   k_ray      one thread per work item: first Moller-Trumbore hit of the item,
 //              combined per particle with atomicMin over (cell-visit rank,
 //              sorted slot) = the reference's first hit in x->y->z order
 //              (src/simulation/vein_collisions.cuh:60-93)
-//   k_cells_b  hit response   detectVeinCollisions      src/simulation/vein_collisions.cu:234-275
+//   k_cells    centres        calculateBloodCellsCenters   src/objects/blood_cells.cu:44-61
+//              springs        gatherForcesKernel (Heun)     src/objects/blood_cells.cu:66-120
+//              contacts       calculateParticleCollisions   src/simulation/particle_collisions.cuh:104-269
+//              hit response   detectVeinCollisions          src/simulation/vein_collisions.cu:234-275
 //              integrate      propagateParticleForcesKernel src/objects/blood_cells.cu:155-179 (fused)
-//              vein end       handleVeinEnds*           src/simulation/vein_end.cu:57-138 (fused)
+//              vein end       handleVeinEnds*               src/simulation/vein_end.cu:57-138 (fused)
 // Reads that the reference shares between threads come from snapshots: the
 // cell's F from LDS (loaded before any write), neighbour x/v from the sorted
-// copies made by the grid build; x and v are only written by k_cells_b.
+// copies made by the grid build; x and v are only written by k_cells, which
+// never reads another block's particles from x or v.
 // Vein-vertex contributions of ray hits are recorded and summed per vertex
 // in particle-index order by k_vein (deterministic; the reference adds them
 // without atomics).
@@ -175,7 +177,7 @@ __device__ __forceinline__ f3 teleport_pos(const PhysConst& pc, const float* mod
 }
 
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
-constexpr uint32_t kScanSerial = 0xFFFFFFFEu;  // work list full: k_cells_b scans serially
+constexpr uint32_t kScanSerial = 0xFFFFFFFEu;  // work list full: k_cells scans serially
 constexpr uint32_t kSlotBits = 27;             // best = rank << 27 | sorted triangle slot
 constexpr uint32_t kItemTris = 16;
 constexpr uint32_t kRaySplit = 4;  // k_ray blocks per work region
@@ -345,13 +347,129 @@ __device__ bool ray_first_hit_serial(const StepArgs& A, f3 x1, f3 dir, float& rt
 }
 
 
-__global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
+// Vein ray candidates (part of detectVeinCollisions, src/simulation/vein_collisions.cu:181-233):
+// per particle, the occupied triangle cells of its 27-neighbourhood whose box
+// the ray from x along v/|v| can reach, split into work items of <= 16 sorted
+// triangles written to this block's own region of the work list.  Needs only
+// the frame's starting x, v and the triangle data, so it runs beside the
+// particle grid build.
+__global__ void __launch_bounds__(kBlock) k_raycand(StepArgs A)
+{
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    __shared__ uint32_t s_end;
+    const TypeTable& tt = A.tt;
+    const int l = threadIdx.x;
+    const int chunk = chunk_of_block(A.nchunks);
+    if (chunk < 0) {
+        if (l == 0) A.wcount[blockIdx.x] = 0u;
+        return;
+    }
+    const int T = A.chunk_type[chunk];
+    const int first = A.chunk_first[chunk];
+    const int ncell = A.chunk_n[chunk];
+    const int p = tt.ppc[T];
+    const bool act = l < ncell * p;
+    const int a = act ? l % p : 0;
+    const int cl = act ? l / p : 0;
+    const int64_t np = tt.np;
+    int32_t cell = 0;
+    const int32_t i = act ? chunk_particle(A, T, first, cl, a, p, cell) : 0;
+    uint32_t fl = 0;
+    uint32_t nitems = 0, pmask = 0;
+    f3 dir = mk(0.f, 0.f, 0.f), x1 = dir;
+    RayCells rc{};
+    if (act) {
+        x1 = load3(A.pos, np, i);
+        const f3 v1 = load3(A.vel, np, i);
+        // ---- vein ray candidates: occupied triangle cells whose box the ray
+        // can reach (pmask), split into work items of <= 16 sorted triangles
+        dir = normalize(v1, fl);
+        if (dir.x != 0.0f || dir.y != 0.0f || dir.z != 0.0f) {
+            rc = ray_cells(A.tg, x1);
+            const uint32_t om = tri_occupied(A, rc);
+            if (om) {
+                const f3 inv = inv_dir(dir);
+                const float bm = box_margin(A);
+                // boxes of one x-slab's occupied cells loaded together, then
+                // tested; a passing cell's triangle count is in hi.w
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint32_t sm = (om >> ((dx + 1) * 9)) & 0x1FFu;
+                    if (!sm) continue;
+                    float4 lo[9], hi[9];
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) {
+                        if ((sm >> c) & 1u) {
+                            const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
+                            lo[c] = A.t_aabb[2 * nk];
+                            hi[c] = A.t_aabb[2 * nk + 1];
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 9; ++c)
+                        if (((sm >> c) & 1u) && ray_box(x1, inv, dir, lo[c], hi[c], bm)) {
+                            pmask |= 1u << ((dx + 1) * 9 + c);
+                            nitems += (min(__float_as_uint(hi[c].w), (uint32_t)A.ntri) + kItemTris - 1) / kItemTris;
+                        }
+                }
+            }
+        }
+    }
+    // ---- work-list space: every block owns region blockIdx.x of region_cap
+    // items; offsets from a block-wide exclusive scan (no global atomics)
+    const int lane = l & 63, wv = l >> 6;
+    uint32_t incl = nitems;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_wsum[wv] = incl;
+    if (l == 0) s_end = 0u;
+    __syncthreads();
+    uint32_t wave_off = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w)
+        if (w < wv) wave_off += s_wsum[w];
+    const uint32_t off = wave_off + incl - nitems;  // within the region
+    // particles whose items fit form a prefix of the scan order; the region's
+    // count ends with the last of them (no unwritten slots below it)
+    const bool fits = (uint64_t)off + nitems <= (uint64_t)A.region_cap;
+    if (act && nitems > 0 && fits) atomicMax(&s_end, off + nitems);
+    __syncthreads();
+    if (l == 0) A.wcount[blockIdx.x] = s_end;
+    if (act) {
+        A.best[i] = (nitems > 0 && !fits) ? kScanSerial : kNoHit;
+        if (nitems > 0 && !fits) fl |= BCS_FLAG_RAY_SERIAL;
+        if (nitems > 0 && fits) {
+            uint4* out = A.work + (size_t)blockIdx.x * A.region_cap;
+            uint32_t w = off;
+            uint32_t m = pmask;
+            while (m) {
+                const int r = __builtin_ctz(m);
+                m &= m - 1u;
+                const int dx = r / 9 - 1, c = r % 9;
+                const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
+                const uint2 ent = A.t_table[nk];
+                for (uint32_t u = ent.x; u < ent.x + ent.y && w < off + nitems; u += kItemTris)
+                    out[w++] = make_uint4((uint32_t)i, u, (uint32_t)r, min(kItemTris, ent.x + ent.y - u));
+            }
+        }
+    }
+    if (fl) atomicOr(A.flags, fl);
+}
+
+template <bool FUSED>
+__device__ __forceinline__ void respond_integrate(const StepArgs& A, bool act, int32_t i, int T, int a, int cl,
+                                                  int32_t cell, f3 x1, f3 v1, f3 F, uint32_t fl, int* s_tele);
+
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
 {
     __shared__ float sx[kBlock], sy[kBlock], sz[kBlock];
     __shared__ float svx[kBlock], svy[kBlock], svz[kBlock];
     __shared__ float sfx[kBlock], sfy[kBlock], sfz[kBlock];
-    __shared__ uint32_t s_wsum[kBlock / 64];
-    __shared__ uint32_t s_end;
+    __shared__ int s_tele[kBlock];
     __shared__ float s_bs[kBsLds];
     __shared__ int32_t s_spoff[kBlock + 1];  // the block's cell type: spring list offsets per vertex,
     __shared__ int32_t s_spb[kSpringLds];    // partner vertex and rest length per spring
@@ -361,10 +479,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
     const PhysConst& pc = A.pc;
     const int l = threadIdx.x;
     const int chunk = chunk_of_block(A.nchunks);
-    if (chunk < 0) {
-        if (l == 0) A.wcount[blockIdx.x] = 0u;
-        return;
-    }
+    if (chunk < 0) return;
     // contact radii per model vertex (cellModelsBoundingSpheres), from LDS
     // when the table fits
     const bool bs_lds = tt.nm <= kBsLds;
@@ -405,13 +520,11 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
         }
     }
     const float r0a = act ? A.r0[m0 + a] : 1.0f;
+    if (l < ncell) s_tele[l] = 0;
     __syncthreads();
 
     uint32_t fl = 0;
     f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
-    uint32_t nitems = 0, pmask = 0;
-    f3 dir = x1;
-    RayCells rc{};
     if (act) {
         x1 = mk(sx[l], sy[l], sz[l]);
         v1 = mk(svx[l], svy[l], svz[l]);
@@ -537,83 +650,8 @@ __global__ void __launch_bounds__(kBlock) k_cells_a(StepArgs A)
             }
         }
 
-        // ---- vein ray candidates: occupied triangle cells whose box the ray
-        // can reach (pmask), split into work items of <= 16 sorted triangles
-        dir = normalize(v1, fl);
-        if (dir.x != 0.0f || dir.y != 0.0f || dir.z != 0.0f) {
-            rc = ray_cells(A.tg, x1);
-            const uint32_t om = tri_occupied(A, rc);
-            if (om) {
-                const f3 inv = inv_dir(dir);
-                const float bm = box_margin(A);
-                // boxes of one x-slab's occupied cells loaded together, then
-                // tested; a passing cell's triangle count is in hi.w
-#pragma unroll
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const uint32_t sm = (om >> ((dx + 1) * 9)) & 0x1FFu;
-                    if (!sm) continue;
-                    float4 lo[9], hi[9];
-#pragma unroll
-                    for (int c = 0; c < 9; ++c) {
-                        if ((sm >> c) & 1u) {
-                            const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
-                            lo[c] = A.t_aabb[2 * nk];
-                            hi[c] = A.t_aabb[2 * nk + 1];
-                        }
-                    }
-#pragma unroll
-                    for (int c = 0; c < 9; ++c)
-                        if (((sm >> c) & 1u) && ray_box(x1, inv, dir, lo[c], hi[c], bm)) {
-                            pmask |= 1u << ((dx + 1) * 9 + c);
-                            nitems += (min(__float_as_uint(hi[c].w), (uint32_t)A.ntri) + kItemTris - 1) / kItemTris;
-                        }
-                }
-            }
-        }
-        A.frc[i] = F.x; A.frc[np + i] = F.y; A.frc[2 * np + i] = F.z;
     }
-    // ---- work-list space: every block owns region blockIdx.x of region_cap
-    // items; offsets from a block-wide exclusive scan (no global atomics)
-    const int lane = l & 63, wv = l >> 6;
-    uint32_t incl = nitems;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += o;
-    }
-    if (lane == 63) s_wsum[wv] = incl;
-    if (l == 0) s_end = 0u;
-    __syncthreads();
-    uint32_t wave_off = 0;
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w)
-        if (w < wv) wave_off += s_wsum[w];
-    const uint32_t off = wave_off + incl - nitems;  // within the region
-    // particles whose items fit form a prefix of the scan order; the region's
-    // count ends with the last of them (no unwritten slots below it)
-    const bool fits = (uint64_t)off + nitems <= (uint64_t)A.region_cap;
-    if (act && nitems > 0 && fits) atomicMax(&s_end, off + nitems);
-    __syncthreads();
-    if (l == 0) A.wcount[blockIdx.x] = s_end;
-    if (act) {
-        A.best[i] = (nitems > 0 && !fits) ? kScanSerial : kNoHit;
-        if (nitems > 0 && !fits) fl |= BCS_FLAG_RAY_SERIAL;
-        if (nitems > 0 && fits) {
-            uint4* out = A.work + (size_t)blockIdx.x * A.region_cap;
-            uint32_t w = off;
-            uint32_t m = pmask;
-            while (m) {
-                const int r = __builtin_ctz(m);
-                m &= m - 1u;
-                const int dx = r / 9 - 1, c = r % 9;
-                const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
-                const uint2 ent = A.t_table[nk];
-                for (uint32_t u = ent.x; u < ent.x + ent.y && w < off + nitems; u += kItemTris)
-                    out[w++] = make_uint4((uint32_t)i, u, (uint32_t)r, min(kItemTris, ent.x + ent.y - u));
-            }
-        }
-    }
-    if (fl) atomicOr(A.flags, fl);
+    respond_integrate<FUSED>(A, act, i, T, a, cl, cell, x1, v1, F, fl, s_tele);  // raises fl
 }
 
 // One thread per work item: first hit among its <= 16 triangles, folded into
@@ -660,70 +698,26 @@ __device__ __forceinline__ void ray_region(const StepArgs& A, uint32_t r, uint32
 
 __global__ void __launch_bounds__(256) k_ray(StepArgs A)
 {
-    // kRaySplit blocks share the work region of k_cells_a block r; block
+    // kRaySplit blocks share the work region of k_raycand block r; block
     // b = s * nregions + r runs on the XCD (b % 8 == r % 8) whose L2 holds it
     const uint32_t nreg = gridDim.x / kRaySplit;
     ray_region(A, blockIdx.x % nreg, blockIdx.x / nreg);
 }
 
+// Vein hit response of particle i (detectVeinCollisions response,
+// src/simulation/vein_collisions.cu:234-275), then for FUSED frames the Heun
+// integration and the vein end of its cell (src/objects/blood_cells.cu:150-179,
+// src/simulation/vein_end.cu:12-173); writes x (FUSED), v and F.  The whole
+// block calls it (barriers); s_tele[0 .. ncell) is zero on entry.
 template <bool FUSED>
-__device__ __forceinline__ void cells_b_body(const StepArgs& A, int chunk, int* s_tele);
-
-// k_ray and k_cells_b<true> in one launch: the kRaySplit blocks of region r
-// count themselves in after their items; the last to arrive runs the hit
-// response, integration and vein end of the chunk of k_cells_a block r.  No
-// block waits.  The only data passed between the blocks is best[], written
-// with device atomics (atomicMin) and read back with agent-scope atomic loads;
-// every wave waits until its atomics are acknowledged (s_waitcnt on all
-// counters) before the block's count, so the last block's loads see them.  A
-// device-scope fence would also write back and invalidate the L2 in every
-// block, which costs far more than the work here.
-__global__ void __launch_bounds__(256) k_ray_b(StepArgs A)
-{
-    static_assert(kBlock == 256, "the chunk body needs the whole block");
-    __shared__ int s_tele[kBlock];
-    __shared__ uint32_t s_last;
-    const uint32_t nreg = gridDim.x / kRaySplit;
-    const uint32_t r = blockIdx.x % nreg;
-    ray_region(A, r, blockIdx.x / nreg);
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's atomicMin acknowledged ...
-    __syncthreads();                // ... in every wave, before the count
-    if (threadIdx.x == 0) s_last = atomicAdd(&A.region_done[r], 1u) == kRaySplit - 1 ? 1u : 0u;
-    __syncthreads();
-    if (!s_last) return;
-    if (threadIdx.x == 0) A.region_done[r] = 0u;  // all kRaySplit arrived: ready for the next frame
-    const int chunk = chunk_of_block(A.nchunks, r);
-    if (chunk < 0) return;
-    cells_b_body<true>(A, chunk, s_tele);
-}
-
-// Hit response, integration and vein end of one chunk (one block of kBlock
-// threads; the chunk of k_cells_a block `block`).
-template <bool FUSED>
-__device__ __forceinline__ void cells_b_body(const StepArgs& A, int chunk, int* s_tele)
+__device__ __forceinline__ void respond_integrate(const StepArgs& A, bool act, int32_t i, int T, int a, int cl,
+                                                  int32_t cell, f3 x1, f3 v1, f3 F, uint32_t fl, int* s_tele)
 {
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
-    const int l = threadIdx.x;
-    const int T = A.chunk_type[chunk];
-    const int first = A.chunk_first[chunk];
-    const int ncell = A.chunk_n[chunk];
-    const int p = tt.ppc[T];
-    const bool act = l < ncell * p;
-    const int a = act ? l % p : 0;
-    const int cl = act ? l / p : 0;
     const int64_t np = tt.np;
-    int32_t cell = 0;
-    const int32_t i = act ? chunk_particle(A, T, first, cl, a, p, cell) : 0;
-    if (l < ncell) s_tele[l] = 0;
-    uint32_t fl = 0;
-    f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
     if (act) {
-        x1 = load3(A.pos, np, i);
-        v1 = load3(A.vel, np, i);
-        F = load3(A.frc, np, i);
-        // atomic load: in k_ray_b other blocks set it in this launch
-        const uint32_t best = __hip_atomic_load(&A.best[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t best = A.best[i];
         const f3 dir = normalize(v1, fl);
         float rt = 1e10f;
         f3 normal = mk(0.f, 0.f, 0.f), refl = mk(0.f, 0.f, 0.f);
@@ -798,15 +792,6 @@ __device__ __forceinline__ void cells_b_body(const StepArgs& A, int chunk, int* 
         A.frc[i] = F.x; A.frc[np + i] = F.y; A.frc[2 * np + i] = F.z;
     }
     if (fl) atomicOr(A.flags, fl);
-}
-
-template <bool FUSED>
-__global__ void __launch_bounds__(kBlock) k_cells_b(StepArgs A)
-{
-    __shared__ int s_tele[kBlock];
-    const int chunk = chunk_of_block(A.nchunks);
-    if (chunk < 0) return;
-    cells_b_body<FUSED>(A, chunk, s_tele);
 }
 
 // propagateParticleForcesKernel, src/objects/blood_cells.cu:155-179 (API path).
@@ -1229,23 +1214,19 @@ void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st)
 
 static dim3 chunk_grid(int nchunks) { return dim3(8 * ((nchunks + 7) / 8)); }
 
-void launch_cells_a(const StepArgs& A, int nchunks, hipStream_t st)
+void launch_raycand(const StepArgs& A, int nchunks, hipStream_t st)
 {
-    if (nchunks > 0) hipLaunchKernelGGL(k_cells_a, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
+    if (nchunks > 0) hipLaunchKernelGGL(k_raycand, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
 }
 void launch_ray(const StepArgs& A, hipStream_t st)
 {
     if (A.nchunks > 0) hipLaunchKernelGGL(k_ray, dim3(chunk_grid(A.nchunks).x * kRaySplit), dim3(256), 0, st, A);
 }
-void launch_ray_b(const StepArgs& A, hipStream_t st)
-{
-    if (A.nchunks > 0) hipLaunchKernelGGL(k_ray_b, dim3(chunk_grid(A.nchunks).x * kRaySplit), dim3(256), 0, st, A);
-}
-void launch_cells_b(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
+void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
 {
     if (nchunks <= 0) return;
-    if (fused) hipLaunchKernelGGL(k_cells_b<true>, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
-    else hipLaunchKernelGGL(k_cells_b<false>, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
+    if (fused) hipLaunchKernelGGL(k_cells<true>, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL(k_cells<false>, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
 }
 void launch_integrate(const StepArgs& A, hipStream_t st)
 {

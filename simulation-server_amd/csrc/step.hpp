@@ -9,7 +9,7 @@ constexpr int kMaxEndings = 16;
 constexpr int kBlock = 256;  // threads per cell-chunk block
 constexpr int kBsLds = 1024;       // model-vertex radii held in LDS by the contact search (more: read globally)
 constexpr int kContactBatch = 4;   // sorted-slot records loaded together by the contact search
-constexpr int kSpringLds = 1024;   // springs of one cell type held in LDS by k_cells_a (more: read globally)
+constexpr int kSpringLds = 1024;   // springs of one cell type held in LDS by k_cells (more: read globally)
 constexpr int kOrderBits = 15;      // Morton bits of the cell processing order
 constexpr int kReorderPeriod = 16;  // frames between updates of the order
 
@@ -85,9 +85,8 @@ struct StepArgs {
     const uint32_t* t_nmask;  // per triangle cell: occupied cells of its neighbourhood
     uint32_t* best;           // per particle: min (cell rank << 27 | slot) of its ray hits
     uint4* work;              // ray work items: (particle, first slot, cell rank, count), one
-                              // region of region_cap items per k_cells_a block
+                              // region of region_cap items per k_raycand block
     uint32_t* wcount;         // items per region (last frame)
-    uint32_t* region_done;    // per region: k_ray_b blocks finished (0 between launches)
     uint32_t region_cap;
     uint32_t* flags;
     uint64_t* step;

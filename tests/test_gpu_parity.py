@@ -271,26 +271,6 @@ def test_graph_replay_equals_direct_launch(bcs):
     assert a.get_info().step_index == b.get_info().step_index == 12
 
 
-def test_ray_fusion_modes_equal(bcs):
-    """k_ray_b (ray test + hit response in one launch) and the two-launch form
-    give the same bits."""
-    sc = _scene(bcs, "c1_small")
-    a = bcs.Simulation.from_scene(sc)
-    b = bcs.Simulation.from_scene(sc)
-    a.set_ray_fusion(1)
-    b.set_ray_fusion(2)
-    for _ in range(4):
-        a.step(5)
-        b.step(5)
-        assert a.frame_stats() == b.frame_stats()
-    for x, y, n in zip(a.get_particles(), b.get_particles(), ("pos", "vel", "force")):
-        assert_bits_equal(x, y, "fused vs two-launch " + n)
-    assert_bits_equal(a.get_vein()[0], b.get_vein()[0], "fused vs two-launch vein")
-    assert a.error_flags() == b.error_flags()
-    with pytest.raises(bcs.BcsError):
-        a.set_ray_fusion(3)
-
-
 def test_determinism_two_runs(bcs):
     sc = _scene(bcs, "c1_small")
     a = bcs.Simulation.from_scene(sc)
