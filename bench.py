@@ -74,7 +74,7 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
                    + map code read 10 per occupied cell
                                                                    112 / particle + 12 / blood cell
                                                                    + 10 / cell
-      k_vein_springs+k_tri_box  X, V read 24, 9 neighbour ids + lengths 72,
+      k_vein_prep  vein springs + triangle data: X, V read 24, 9 neighbour ids + lengths 72,
                    Fv read + written 24 per vertex; perm + indices 16, the
                    float4 triangle data written 48 and one box (32) per
                    occupied triangle cell, counted per triangle 12 120 / vertex + 76 / triangle
@@ -88,7 +88,7 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
         "k_raycand": 28 * np_ + 16 * n_items,
         "k_ray": 40 * n_items,
         "k_cells": 112 * np_ + 12 * nc + 10 * n_occ,
-        "k_vein_springs+k_tri_box": 120 * nv + 76 * nt,
+        "k_vein_prep": 120 * nv + 76 * nt,
         "k_vein": 76 * nv,
     }
     return table.get(name)

@@ -29,6 +29,8 @@ void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t
 void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st);
 void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
+void launch_vein_prep(const VeinArgs& V, const DeviceGrid& tg, const uint32_t* idx, float4* tdata, float4* aabb,
+                      hipStream_t st);
 void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
                     float4* aabb, hipStream_t st);
 void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st);
@@ -230,10 +232,9 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         HIPCHK(hipEventRecord(s->ev_fork, st));
         HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
         if (lead_vein) launch_vein(Va, 7, s->side);
-        launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, s->side);
+        launch_vein_prep(Va, s->tg, s->tidx, s->tdata, s->t_aabb, s->side);
         launch_raycand(A, s->nchunks, s->side);
         launch_ray(A, s->side);
-        launch_vein_springs(Va, s->side);
         HIPCHK(hipEventRecord(s->ev_join, s->side));
         RC(enqueue_particle_grid(s));
         HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
@@ -241,8 +242,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         if (lead_vein) launch_vein(Va, 7, st);
         RC(enqueue_particle_grid(s, timer));
         tb(kGridKernels + 0);
-        launch_vein_springs(Va, st);
-        launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, st);
+        launch_vein_prep(Va, s->tg, s->tidx, s->tdata, s->t_aabb, st);
         te(kGridKernels + 0);
         tb(kGridKernels + 1);
         launch_raycand(A, s->nchunks, st);
@@ -262,7 +262,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
 }
 
 const char* kKernelNames =
-    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_springs+k_tri_box;k_raycand;k_ray;k_cells;k_vein";
+    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_prep;k_raycand;k_ray;k_cells;k_vein";
 constexpr int kFusedKernels = kGridKernels + 5;
 
 }  // namespace
@@ -652,8 +652,7 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
     }
     const StepArgs A = s->step_args();
     const VeinArgs Va = s->vein_args();
-    launch_vein_springs(Va, s->stream);
-    launch_tri_box(s->tg, s->tidx, s->X, s->nv, s->tdata, s->t_aabb, s->stream);
+    launch_vein_prep(Va, s->tg, s->tidx, s->tdata, s->t_aabb, s->stream);
     launch_raycand(A, s->nchunks, s->stream);
     launch_ray(A, s->stream);
     launch_cells(A, s->nchunks, false, s->stream);

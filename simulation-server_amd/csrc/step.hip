@@ -844,9 +844,8 @@ __global__ void k_bump_step(uint64_t* step) { *step += 1ull; }
 
 // Vein springs: gatherForcesKernel, src/objects/vein_triangles.cu:126-154.
 // Forces were zeroed by the previous integrate, so Fv = 0 + spring force.
-__global__ void __launch_bounds__(256) k_vein_springs(VeinArgs A)
+__device__ __forceinline__ void vein_springs_at(const VeinArgs& A, int32_t id)
 {
-    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= A.nv) return;
     const int64_t nv = A.nv;
     uint32_t fl = 0;
@@ -867,6 +866,11 @@ __global__ void __launch_bounds__(256) k_vein_springs(VeinArgs A)
     const f3 F0 = load3(A.Fv, nv, id);
     store3(A.Fv, nv, id, add(F0, force));
     if (fl) atomicOr(A.flags, fl);
+}
+
+__global__ void __launch_bounds__(256) k_vein_springs(VeinArgs A)
+{
+    vein_springs_at(A, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Per vertex: add the recorded contributions in (particle, corner) order, then
@@ -944,16 +948,38 @@ __global__ void __launch_bounds__(256) k_tri_centers(const float* __restrict__ X
 // the current vein positions, stored at its sorted slot -- and the wave
 // reduces the cell's box (exact vertex bounds), stored under the cell key.
 // Non-finite coordinates widen the box to everything.
-__global__ void __launch_bounds__(256) k_tri_box(const uint32_t* __restrict__ runs, const uint32_t* __restrict__ ctrl,
-                                                 const uint2* __restrict__ table,
-                                                 const uint32_t* __restrict__ keys_sorted,
-                                                 const int32_t* __restrict__ perm, int32_t ntri,
-                                                 const uint32_t* __restrict__ idx, const float* __restrict__ X,
-                                                 int32_t nv, float4* __restrict__ tdata, float4* __restrict__ aabb)
+struct TriBoxArgs {
+    const uint32_t* runs;
+    const uint32_t* ctrl;
+    const uint2* table;
+    const uint32_t* keys_sorted;
+    const int32_t* perm;
+    int32_t ntri;
+    const uint32_t* idx;
+    const float* X;
+    int32_t nv;
+    float4* tdata;
+    float4* aabb;
+};
+
+// Blocks [0, nblocks) of 256 threads share the occupied triangle cells, one
+// wave per cell at a time.
+__device__ __forceinline__ void tri_box_blocks(const TriBoxArgs& B, uint32_t block, uint32_t nblocks)
 {
+    const uint32_t* __restrict__ runs = B.runs;
+    const uint2* __restrict__ table = B.table;
+    const uint32_t* __restrict__ keys_sorted = B.keys_sorted;
+    const int32_t* __restrict__ perm = B.perm;
+    const int32_t ntri = B.ntri;
+    const uint32_t* __restrict__ idx = B.idx;
+    const float* __restrict__ X = B.X;
+    const int32_t nv = B.nv;
+    float4* __restrict__ tdata = B.tdata;
+    float4* __restrict__ aabb = B.aabb;
     const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t wave = (block * 256u + threadIdx.x) >> 6;
+    const uint32_t nwaves = (nblocks * 256u) >> 6;
+    const uint32_t* __restrict__ ctrl = B.ctrl;
     const uint32_t nr = ctrl[2];
     for (uint32_t r = wave; r < nr; r += nwaves) {
         const uint32_t s0 = runs[r];
@@ -998,6 +1024,16 @@ __global__ void __launch_bounds__(256) k_tri_box(const uint32_t* __restrict__ ru
             aabb[2 * (size_t)key + 1] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(cnt));
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_tri_box(TriBoxArgs B) { tri_box_blocks(B, blockIdx.x, gridDim.x); }
+
+// Vein springs and triangle data in one launch (both need only the updated
+// vein): the first nsb blocks take the vertices, the rest the triangle cells.
+__global__ void __launch_bounds__(256) k_vein_prep(VeinArgs A, TriBoxArgs B, uint32_t nsb)
+{
+    if (blockIdx.x < nsb) vein_springs_at(A, blockIdx.x * 256u + threadIdx.x);
+    else tri_box_blocks(B, blockIdx.x - nsb, gridDim.x - nsb);
 }
 
 // ---- processing order of the cells: by type, then by the Morton code of the
@@ -1261,12 +1297,26 @@ void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t
     if (A.tt.np > 0)
         hipLaunchKernelGGL(k_contacts, dim3((A.tt.np + 255) / 256), dim3(256), 0, st, A, pass, counts, offs, pairs);
 }
+static TriBoxArgs tri_box_args(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv,
+                               float4* tdata, float4* aabb)
+{
+    return TriBoxArgs{tg.runs, tg.ctrl, tg.table, tg.keys_sorted, tg.perm, tg.n, idx, X, nv, tdata, aabb};
+}
+constexpr uint32_t kTriBoxBlocks = 1024;
 void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
                     float4* aabb, hipStream_t st)
 {
     if (tg.n > 0)
-        hipLaunchKernelGGL(k_tri_box, dim3(1024), dim3(256), 0, st, tg.runs, tg.ctrl, tg.table, tg.keys_sorted,
-                           tg.perm, tg.n, idx, X, nv, tdata, aabb);
+        hipLaunchKernelGGL(k_tri_box, dim3(kTriBoxBlocks), dim3(256), 0, st, tri_box_args(tg, idx, X, nv, tdata, aabb));
+}
+void launch_vein_prep(const VeinArgs& V, const DeviceGrid& tg, const uint32_t* idx, float4* tdata, float4* aabb,
+                      hipStream_t st)
+{
+    const uint32_t nsb = (uint32_t)((std::max(V.nv, 0) + 255) / 256);
+    const uint32_t ntb = tg.n > 0 ? kTriBoxBlocks : 0u;
+    if (nsb + ntb > 0)
+        hipLaunchKernelGGL(k_vein_prep, dim3(nsb + ntb), dim3(256), 0, st, V, tri_box_args(tg, idx, V.X, V.nv, tdata, aabb),
+                           nsb);
 }
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st)
 {
