@@ -600,28 +600,31 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
             axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
             axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
             const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-            const uint32_t mask = neighbour_mask(A.p_map, g, cellId, nr, cX, cXY, true);
-            // One x-slab (9 cells, visit ranks (dx+1)*9 + 0..8) at a time: the
-            // table entries of its occupied cells are loaded together, then
-            // each cell's run of sorted-slot records is read 4 slots per load
-            // batch.  Contributions are added in visit order, slot by slot.
+            uint32_t mask = neighbour_mask(A.p_map, g, cellId, nr, cX, cXY, true);
+            // The occupied cells in visit order (rank r = (dx+1)*9 + (dy+1)*3 +
+            // (dz+1)), kCellBatch at a time: their table entries are loaded
+            // together, then each cell's run of sorted-slot records is read 4
+            // slots per load batch.  Contributions are added in visit order,
+            // slot by slot.
+            while (mask) {
+                uint2 ent[kCellBatch];
+                int ncb = 0;
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                uint32_t sm = (mask >> ((dx + 1) * 9)) & 0x1FFu;
-                if (!sm) continue;
-                uint2 ent[9];
-#pragma unroll
-                for (int c = 0; c < 9; ++c) {
-                    ent[c] = make_uint2(0u, 0u);
-                    if ((sm >> c) & 1u) ent[c] = A.p_table[(int64_t)cellId + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
+                for (int k = 0; k < kCellBatch; ++k) {
+                    ent[k] = make_uint2(0u, 0u);
+                    if (mask) {
+                        const int r = __builtin_ctz(mask);
+                        mask &= mask - 1u;
+                        const int dx = r / 9 - 1, c = r % 9;
+                        ent[k] = A.p_table[(int64_t)cellId + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
+                        ncb = k + 1;
+                    }
                 }
-                while (sm) {
-                    const int c = __builtin_ctz(sm);
-                    sm &= sm - 1u;
+                for (int kc = 0; kc < ncb; ++kc) {
                     uint2 e = ent[0];
 #pragma unroll
-                    for (int k = 1; k < 9; ++k)
-                        if (c == k) e = ent[k];
+                    for (int k = 1; k < kCellBatch; ++k)
+                        if (kc == k) e = ent[k];
                     const uint32_t u1 = min(e.x + e.y, (uint32_t)np);
                     for (uint32_t u = e.x; u < u1; u += kContactBatch) {
                         float4 q[kContactBatch], w[kContactBatch];
