@@ -45,15 +45,22 @@ __device__ __forceinline__ int type_of_index(const TypeTable& tt, int32_t s)
 
 // addResilientForceOnCollision, src/simulation/physics.cuh:133-145 (increment).
 // Chunk of this block.  Workgroups are dispatched round-robin over the 8
-// XCDs, so block b runs on XCD b % 8; each XCD takes one contiguous eighth of
-// the chunk list (and, with corder spatially sorted, one compact region of the
-// vein, whose grid-table lines then stay in that XCD's L2).  The grid is
-// 8 * ceil(nchunks / 8) blocks; blocks past the end return -1.
+// XCDs, so block b runs on XCD b % 8.  The blocks of the 8 * ceil(nchunks / 8)
+// grid are cut into super-groups of 8 * kXcdRun; in each, XCD x takes a run of
+// kXcdRun consecutive chunks (the last, partial super-group splits the same
+// way with shorter runs), a bijection onto [0, grid).  With the chunks in a
+// spatial (Morton) order a run is one compact region, whose grid-table lines
+// stay in that XCD's L2, while a crowded region of the vessel is shared out
+// over all XCDs.  Blocks past the last chunk return -1.
 __device__ __forceinline__ int chunk_of_block(int32_t nchunks, uint32_t block)
 {
-    const int per = (nchunks + 7) >> 3;
-    const int c = (int)(block & 7u) * per + (int)(block >> 3);
-    return c < nchunks ? c : -1;
+    constexpr uint32_t S = 8u * kXcdRun;
+    const uint32_t nb = 8u * (((uint32_t)nchunks + 7u) >> 3);
+    const uint32_t sg = block / S, w = block % S;
+    const uint32_t x = w & 7u, q = w >> 3;
+    const uint32_t run = (sg + 1u) * S <= nb ? kXcdRun : (nb - sg * S) >> 3;
+    const uint32_t c = sg * S + x * run + q;
+    return c < (uint32_t)nchunks ? (int)c : -1;
 }
 __device__ __forceinline__ int chunk_of_block(int32_t nchunks) { return chunk_of_block(nchunks, blockIdx.x); }
 
