@@ -311,6 +311,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
                 c.spring_end[k] >= c.particles_in_cell)
                 return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_create: ill-formed spring");
         tt.ppc[t] = c.particles_in_cell;
+        tt.ppc_div[t] = make_div32((uint32_t)c.particles_in_cell);
         tt.cnt[t] = c.count;
         tt.pstart[t] = (int32_t)np;
         tt.cstart[t] = (int32_t)nc;

@@ -36,6 +36,28 @@ __device__ __forceinline__ int32_t model_index(const TypeTable& tt, int t, int32
     return m;
 }
 
+// model_index for one launch type t held in registers (the contact search
+// evaluates it for every partner of a particle): same result, the C remainder
+// by multiply-shift division.
+struct ModelIx {
+    int32_t pstart, mstart, ppc, nm;
+    Div32 div;
+};
+__device__ __forceinline__ ModelIx model_ix(const TypeTable& tt, int t)
+{
+    return ModelIx{tt.pstart[t], tt.mstart[t], tt.ppc[t], tt.nm, tt.ppc_div[t]};
+}
+__device__ __forceinline__ int32_t model_index(const ModelIx& mi, int32_t id, uint32_t& fl)
+{
+    const int32_t d = id - mi.pstart;
+    const uint32_t ad = d < 0 ? (uint32_t)(-(int64_t)d) : (uint32_t)d;
+    const uint32_t r = ad - udiv(ad, mi.div) * (uint32_t)mi.ppc;
+    int32_t m = mi.mstart + (d < 0 ? -(int32_t)r : (int32_t)r);
+    if (m < 0) { fl |= BCS_FLAG_RADIUS_CLAMPED; m = 0; }
+    if (m >= mi.nm) { fl |= BCS_FLAG_RADIUS_CLAMPED; m = mi.nm - 1; }
+    return m;
+}
+
 __device__ __forceinline__ int type_of_index(const TypeTable& tt, int32_t s)
 {
     for (int t = 0; t < tt.nt; ++t)
@@ -470,6 +492,8 @@ template <bool FUSED>
 __device__ __forceinline__ void respond_integrate(const StepArgs& A, bool act, int32_t i, int T, int a, int cl,
                                                   int32_t cell, f3 x1, f3 v1, f3 F, uint32_t fl, int* s_tele);
 
+static_assert(sizeof(StepArgs) <= 4096, "kernel arguments exceed 4 KiB");
+
 template <bool FUSED>
 __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
 {
@@ -595,7 +619,8 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
             const GridGeom& g = A.pg;
             int Tl = T;
             if (tt.nt > 1) Tl = type_of_index(tt, slot_i);
-            const float r1 = radius(model_index(tt, Tl, i, fl));
+            const ModelIx mi = model_ix(tt, Tl);
+            const float r1 = radius(model_index(mi, i, fl));
             // the cell the grid build gave i (gridCellIds of its slot); the face
             // clipping below uses the current position, as the reference does
             const uint32_t cellId = key_of;
@@ -646,7 +671,7 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
                             if (u + k >= u1) break;
                             const int32_t j = __float_as_int(q[k].w);
                             if (j == i) continue;
-                            const float r2 = radius(model_index(tt, Tl, j, fl));
+                            const float r2 = radius(model_index(mi, j, fl));
                             const f3 rel = sub(x1, mk(q[k].x, q[k].y, q[k].z));
                             const float d2 = dot(rel, rel);
                             const float minD = r1 + r2;

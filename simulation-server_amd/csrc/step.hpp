@@ -21,6 +21,7 @@ struct TypeTable {
     int32_t np, nc, nm;
     int32_t ppc[kMaxTypes], cnt[kMaxTypes], pstart[kMaxTypes], cstart[kMaxTypes], mstart[kMaxTypes];
     int32_t end_spheres[kMaxTypes];  // vein-end variant with ending spheres (block-sync variant)
+    Div32 ppc_div[kMaxTypes];        // division by ppc[t] (model index of a partner)
 };
 
 // Physics constants in the form the kernels use them.
