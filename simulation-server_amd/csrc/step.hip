@@ -639,8 +639,7 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
             // slots per load batch.  Contributions are added in visit order,
             // slot by slot.
             while (mask) {
-                uint2 ent[kCellBatch];
-                int ncb = 0;
+                uint2 ent[kCellBatch];  // {first slot, end slot} per occupied cell (empty: {0, 0})
 #pragma unroll
                 for (int k = 0; k < kCellBatch; ++k) {
                     ent[k] = make_uint2(0u, 0u);
@@ -648,37 +647,51 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
                         const int r = __builtin_ctz(mask);
                         mask &= mask - 1u;
                         const int dx = r / 9 - 1, c = r % 9;
-                        ent[k] = A.p_table[(int64_t)cellId + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
-                        ncb = k + 1;
+                        const uint2 e = A.p_table[(int64_t)cellId + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
+                        ent[k] = make_uint2(e.x, min(e.x + e.y, (uint32_t)np));
                     }
                 }
-                for (int kc = 0; kc < ncb; ++kc) {
-                    uint2 e = ent[0];
+                // One stream of candidate slots over these cells in visit order
+                // (stream position p lies in cell k when cum[k] <= p < cum[k+1]);
+                // each load batch takes the next kContactBatch of them, across
+                // cell boundaries (a cell of one or two objects costs no round
+                // trip of its own).
+                uint32_t cum[kCellBatch + 1];
+                cum[0] = 0u;
 #pragma unroll
-                    for (int k = 1; k < kCellBatch; ++k)
-                        if (kc == k) e = ent[k];
-                    const uint32_t u1 = min(e.x + e.y, (uint32_t)np);
-                    for (uint32_t u = e.x; u < u1; u += kContactBatch) {
-                        float4 q[kContactBatch], w[kContactBatch];
+                for (int k = 0; k < kCellBatch; ++k) cum[k + 1] = cum[k] + (ent[k].y > ent[k].x ? ent[k].y - ent[k].x : 0u);
+                const uint32_t total = cum[kCellBatch];
+                for (uint32_t p0 = 0; p0 < total; p0 += kContactBatch) {
+                    uint32_t slot[kContactBatch];
+                    const int nval = (int)min((uint32_t)kContactBatch, total - p0);
 #pragma unroll
-                        for (int k = 0; k < kContactBatch; ++k)
-                            if (u + k < u1) {
-                                q[k] = A.pvs[2 * (size_t)(u + k)];
-                                w[k] = A.pvs[2 * (size_t)(u + k) + 1];
-                            }
+                    for (int t = 0; t < kContactBatch; ++t) {
+                        const uint32_t pp = p0 + (uint32_t)t;
+                        uint32_t sl = 0u;
 #pragma unroll
-                        for (int k = 0; k < kContactBatch; ++k) {
-                            if (u + k >= u1) break;
-                            const int32_t j = __float_as_int(q[k].w);
-                            if (j == i) continue;
-                            const float r2 = radius(model_index(mi, j, fl));
-                            const f3 rel = sub(x1, mk(q[k].x, q[k].y, q[k].z));
-                            const float d2 = dot(rel, rel);
-                            const float minD = r1 + r2;
-                            if (d2 <= minD * minD && d2 >= 0.0001f) {
-                                const f3 rv = sub(v1, mk(w[k].x, w[k].y, w[k].z));
-                                F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
-                            }
+                        for (int k = 0; k < kCellBatch; ++k)
+                            if (pp >= cum[k] && pp < cum[k + 1]) sl = ent[k].x + (pp - cum[k]);
+                        slot[t] = sl;
+                    }
+                    float4 q[kContactBatch], w[kContactBatch];
+#pragma unroll
+                    for (int t = 0; t < kContactBatch; ++t)
+                        if (t < nval) {
+                            q[t] = A.pvs[2 * (size_t)slot[t]];
+                            w[t] = A.pvs[2 * (size_t)slot[t] + 1];
+                        }
+#pragma unroll
+                    for (int t = 0; t < kContactBatch; ++t) {
+                        if (t >= nval) break;
+                        const int32_t j = __float_as_int(q[t].w);
+                        if (j == i) continue;
+                        const float r2 = radius(model_index(mi, j, fl));
+                        const f3 rel = sub(x1, mk(q[t].x, q[t].y, q[t].z));
+                        const float d2 = dot(rel, rel);
+                        const float minD = r1 + r2;
+                        if (d2 <= minD * minD && d2 >= 0.0001f) {
+                            const f3 rv = sub(v1, mk(w[t].x, w[t].y, w[t].z));
+                            F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
                         }
                     }
                 }
