@@ -96,8 +96,9 @@ struct bcs_sim {
     uint64_t* step = nullptr;
     hipGraphExec_t graph_exec = nullptr;    // frame without the leading vein update ...
     hipGraphExec_t graph_exec_b = nullptr;  // ... and with it (the previous frame's k_vein)
+    hipGraphExec_t graph_exec_m = nullptr;  // kGraphFrames frames with it, in one launch
     bool use_graph = true;
-    hipGraph_t graph = nullptr, graph_b = nullptr;
+    hipGraph_t graph = nullptr, graph_b = nullptr, graph_m = nullptr;
     std::vector<float> host_bs;
     std::vector<float> host_smallest;
 
@@ -107,6 +108,8 @@ struct bcs_sim {
         if (graph) (void)hipGraphDestroy(graph);
         if (graph_exec_b) (void)hipGraphExecDestroy(graph_exec_b);
         if (graph_b) (void)hipGraphDestroy(graph_b);
+        if (graph_exec_m) (void)hipGraphExecDestroy(graph_exec_m);
+        if (graph_m) (void)hipGraphDestroy(graph_m);
         grid_free(pg);
         grid_free(tg);
         for (void* p : allocs) (void)hipFree(p);
@@ -264,6 +267,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
 const char* kKernelNames =
     "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_prep;k_raycand;k_ray;k_cells;k_vein";
 constexpr int kFusedKernels = kGridKernels + 5;
+constexpr int32_t kGraphFrames = 16;  // frames per multi-frame graph launch
 
 }  // namespace
 
@@ -684,8 +688,10 @@ static void drop_graph(bcs_sim* s)
     if (s->graph) (void)hipGraphDestroy(s->graph);
     if (s->graph_exec_b) (void)hipGraphExecDestroy(s->graph_exec_b);
     if (s->graph_b) (void)hipGraphDestroy(s->graph_b);
-    s->graph_exec = s->graph_exec_b = nullptr;
-    s->graph = s->graph_b = nullptr;
+    if (s->graph_exec_m) (void)hipGraphExecDestroy(s->graph_exec_m);
+    if (s->graph_m) (void)hipGraphDestroy(s->graph_m);
+    s->graph_exec = s->graph_exec_b = s->graph_exec_m = nullptr;
+    s->graph = s->graph_b = s->graph_m = nullptr;
 }
 
 // The vein update of frame f overlaps the particle grid build of frame f+1
@@ -693,10 +699,13 @@ static void drop_graph(bcs_sim* s)
 // precedes the branches and needs the updated vein).
 static bool defer_vein(const bcs_sim* s) { return !s->phys.dynamic_triangle_grid; }
 
-static int capture_frame(bcs_sim* s, bool lead_vein, hipGraph_t* graph, hipGraphExec_t* exec)
+// nframes consecutive frames in one graph (the same kernels as nframes
+// launches of the one-frame graph; fewer graph launches per step call).
+static int capture_frame(bcs_sim* s, bool lead_vein, hipGraph_t* graph, hipGraphExec_t* exec, int nframes = 1)
 {
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_fused_frame(s, nullptr, lead_vein, !defer_vein(s));
+    int rc = BCS_OK;
+    for (int f = 0; f < nframes && !rc; ++f) rc = enqueue_fused_frame(s, nullptr, lead_vein, !defer_vein(s));
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (rc) {
@@ -713,6 +722,8 @@ static int ensure_graph(bcs_sim* s)
 {
     if (!s->graph_exec) RC(capture_frame(s, false, &s->graph, &s->graph_exec));
     if (!s->graph_exec_b && defer_vein(s)) RC(capture_frame(s, true, &s->graph_b, &s->graph_exec_b));
+    if (!s->graph_exec_m && defer_vein(s))
+        RC(capture_frame(s, true, &s->graph_m, &s->graph_exec_m, kGraphFrames));
     return BCS_OK;
 }
 
@@ -722,7 +733,7 @@ extern "C" int bcs_sim_step(bcs_sim* s, int32_t n_steps)
     if (n_steps < 0) return fail(BCS_ERR_INVALID_ARGUMENT, "negative step count");
     if (n_steps == 0) return BCS_OK;
     if (s->use_graph) RC(ensure_graph(s));
-    for (int32_t k = 0; k < n_steps; ++k) {
+    for (int32_t k = 0; k < n_steps;) {
         // Refresh the spatial processing order of the cells from the last
         // frame's centres (performance only: results do not depend on it).
         if (s->frames_since_order >= kReorderPeriod) {
@@ -732,9 +743,17 @@ extern "C" int bcs_sim_step(bcs_sim* s, int32_t n_steps)
         }
         // frames after the first start with the previous frame's vein update
         const bool lead = defer_vein(s) && k > 0;
-        if (s->use_graph) HIPCHK(hipGraphLaunch(lead ? s->graph_exec_b : s->graph_exec, s->stream));
-        else RC(enqueue_fused_frame(s, nullptr, lead, !defer_vein(s)));
-        ++s->frames_since_order;
+        int32_t nf = 1;
+        if (s->use_graph && lead && s->graph_exec_m && n_steps - k >= kGraphFrames) {
+            HIPCHK(hipGraphLaunch(s->graph_exec_m, s->stream));
+            nf = kGraphFrames;
+        } else if (s->use_graph) {
+            HIPCHK(hipGraphLaunch(lead ? s->graph_exec_b : s->graph_exec, s->stream));
+        } else {
+            RC(enqueue_fused_frame(s, nullptr, lead, !defer_vein(s)));
+        }
+        k += nf;
+        s->frames_since_order += nf;
     }
     if (defer_vein(s)) launch_vein(s->vein_args(), 7, s->stream);  // the last frame's vein update
     s->copies_current = false;  // the frames moved the particles after their grid builds
