@@ -271,6 +271,21 @@ def test_graph_replay_equals_direct_launch(bcs):
     assert a.get_info().step_index == b.get_info().step_index == 12
 
 
+def test_multi_frame_graph_equals_direct_launch(bcs):
+    """step(21) replays the first-frame graph, one 16-frame graph and four
+    one-frame graphs; direct launches of the same frames give the same bits."""
+    sc = _scene(bcs, "c1_small")
+    a = bcs.Simulation.from_scene(sc)
+    b = bcs.Simulation.from_scene(sc)
+    b.use_graph(False)
+    a.step(21)
+    b.step(21)
+    for x, y, n in zip(a.get_particles(), b.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "16-frame graph vs direct " + n)
+    assert_bits_equal(a.get_vein()[0], b.get_vein()[0], "16-frame graph vs direct vein")
+    assert a.get_info().step_index == b.get_info().step_index == 21
+
+
 def test_determinism_two_runs(bcs):
     sc = _scene(bcs, "c1_small")
     a = bcs.Simulation.from_scene(sc)
