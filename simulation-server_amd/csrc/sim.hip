@@ -37,6 +37,7 @@ void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st);
 void launch_export(const float* soa, int64_t n, int64_t first, int64_t count, int stride, float* out, hipStream_t st);
 void launch_refresh_copies(const float* pos, const float* vel, int32_t n, const int32_t* perm, float4* pvs,
                            hipStream_t st);
+void launch_chunk_map(const StepArgs& A, int2* out, hipStream_t st);
 int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
                  uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
                  uint32_t* flags, hipStream_t st);
@@ -68,6 +69,7 @@ struct bcs_sim {
     int32_t *sp_off = nullptr, *sp_b = nullptr, *chunk_type = nullptr, *chunk_first = nullptr, *chunk_n = nullptr;
     // cell processing order (spatial; see k_order_key)
     int32_t* corder = nullptr;
+    int2* chunk_pi = nullptr;  // per chunk and lane: {particle, blood cell} (follows corder)
     uint32_t *okey = nullptr, *ocount = nullptr, *ostart = nullptr, *octrl = nullptr;
     uint64_t* ostatus = nullptr;
     int32_t otiles = 0;
@@ -136,7 +138,7 @@ struct bcs_sim {
         A.pos = pos; A.vel = vel; A.frc = frc; A.centers = centers;
         A.bs = bs; A.r0 = r0; A.model = model; A.sp_off = sp_off; A.sp_b = sp_b; A.sp_L = sp_L;
         A.chunk_type = chunk_type; A.chunk_first = chunk_first; A.chunk_n = chunk_n;
-        A.corder = corder; A.nchunks = nchunks;
+        A.corder = corder; A.chunk_pi = chunk_pi; A.nchunks = nchunks;
         A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table; A.p_map = pg.bm; A.p_key_of = pg.keys;
         A.pvs = pg.pvs;
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
@@ -425,6 +427,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(s->dalloc(&s->bs, nm)); RC(s->dalloc(&s->r0, nm)); RC(s->dalloc(&s->model, 3 * nm));
     RC(s->dalloc(&s->sp_off, nm + 1)); RC(s->dalloc(&s->sp_b, hb.size())); RC(s->dalloc(&s->sp_L, hL.size()));
     RC(s->dalloc(&s->chunk_type, ct.size())); RC(s->dalloc(&s->chunk_first, cf.size())); RC(s->dalloc(&s->chunk_n, cn.size()));
+    RC(s->dalloc(&s->chunk_pi, ct.size() * kBlock));
     {
         const int64_t nbins = (int64_t)n_types << kOrderBits;
         s->otiles = scan_tiles(nbins);
@@ -476,6 +479,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
         HIPCHK(hipMemcpyAsync(s->chunk_first, cf.data(), 4 * cf.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync(s->chunk_n, cn.data(), 4 * cn.size(), hipMemcpyHostToDevice, s->stream));
     }
+    launch_chunk_map(s->step_args(), s->chunk_pi, s->stream);  // after corder and the chunk tables
     {
         std::vector<float> soa(3 * NV);
         for (size_t v = 0; v < NV; ++v)
@@ -739,6 +743,7 @@ extern "C" int bcs_sim_step(bcs_sim* s, int32_t n_steps)
         if (s->frames_since_order >= kReorderPeriod) {
             RC(launch_order(s->tt, s->pg.g, s->centers, s->okey, s->ocount, s->ostart, s->ostatus, s->otiles,
                             s->octrl, s->corder, s->flags, s->stream));
+            launch_chunk_map(s->step_args(), s->chunk_pi, s->stream);
             s->frames_since_order = 0;
         }
         // frames after the first start with the previous frame's vein update

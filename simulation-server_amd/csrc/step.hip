@@ -393,16 +393,11 @@ __global__ void __launch_bounds__(kBlock) k_raycand(StepArgs A)
         if (l == 0) A.wcount[blockIdx.x] = 0u;
         return;
     }
-    const int T = A.chunk_type[chunk];
-    const int first = A.chunk_first[chunk];
-    const int ncell = A.chunk_n[chunk];
-    const int p = tt.ppc[T];
-    const bool act = l < ncell * p;
-    const int a = act ? l % p : 0;
-    const int cl = act ? l / p : 0;
+    // the lane's particle (-1: no slot), independent of the chunk's tables
+    const int2 pcl = A.chunk_pi[(size_t)chunk * kBlock + l];
+    const bool act = pcl.x >= 0;
     const int64_t np = tt.np;
-    int32_t cell = 0;
-    const int32_t i = act ? chunk_particle(A, T, first, cl, a, p, cell) : 0;
+    const int32_t i = act ? pcl.x : 0;
     uint32_t fl = 0;
     uint32_t nitems = 0, pmask = 0;
     f3 dir = mk(0.f, 0.f, 0.f), x1 = dir;
@@ -518,16 +513,16 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
         for (int t = l; t < tt.nm; t += kBlock) s_bs[t] = A.bs[t];
     auto radius = [&](int32_t m) -> float { return bs_lds ? s_bs[m] : A.bs[m]; };
     const int T = A.chunk_type[chunk];
-    const int first = A.chunk_first[chunk];
     const int ncell = A.chunk_n[chunk];
     const int p = tt.ppc[T];
-    const int nloc = ncell * p;
-    const bool act = l < nloc;
+    // the lane's {particle, blood cell} (particle -1: no slot, l >= ncell * p),
+    // a load independent of the chunk's tables
+    const int2 pcl = A.chunk_pi[(size_t)chunk * kBlock + l];
+    const bool act = pcl.x >= 0;
     const int a = act ? l % p : 0;
     const int cl = act ? l / p : 0;
     const int64_t np = tt.np;
-    int32_t cell = 0;
-    const int32_t i = act ? chunk_particle(A, T, first, cl, a, p, cell) : 0;
+    const int32_t i = act ? pcl.x : 0, cell = act ? pcl.y : 0;
     uint32_t key_of = 0;
     int32_t slot_i = 0;
     if (act) {
@@ -1296,7 +1291,25 @@ void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st)
                        nmask);
 }
 
+// {particle, blood cell} of every lane of every chunk, from the processing
+// order (run whenever corder changes).
+__global__ void __launch_bounds__(kBlock) k_chunk_map(StepArgs A, int2* __restrict__ out)
+{
+    const int chunk = blockIdx.x, l = threadIdx.x;
+    const int T = A.chunk_type[chunk];
+    const int p = A.tt.ppc[T];
+    const bool act = l < A.chunk_n[chunk] * p;
+    int32_t cell = 0;
+    const int32_t i = act ? chunk_particle(A, T, A.chunk_first[chunk], l / p, l % p, p, cell) : -1;
+    out[(size_t)chunk * kBlock + l] = make_int2(i, act ? cell : 0);
+}
+
 static dim3 chunk_grid(int nchunks) { return dim3(8 * ((nchunks + 7) / 8)); }
+
+void launch_chunk_map(const StepArgs& A, int2* out, hipStream_t st)
+{
+    if (A.nchunks > 0) hipLaunchKernelGGL(k_chunk_map, dim3(A.nchunks), dim3(kBlock), 0, st, A, out);
+}
 
 void launch_raycand(const StepArgs& A, int nchunks, hipStream_t st)
 {

@@ -60,6 +60,7 @@ struct StepArgs {
     const int32_t* chunk_first;
     const int32_t* chunk_n;
     const int32_t* corder;  // processing order of the cells (a permutation within each type)
+    const int2* chunk_pi;   // per chunk c, lane l at [c * kBlock + l]: {particle, blood cell}, from corder
     int32_t nchunks;
     // particle grid
     GridGeom pg;
