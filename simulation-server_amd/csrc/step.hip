@@ -6,7 +6,8 @@
 //              neighbourhood whose box the ray can reach, as work items of
 //              <= 16 sorted triangles (needs only the frame's starting x, v
 //              and the triangle data: runs beside the particle grid build)
-//   k_ray      one thread per work item: first Moller-Trumbore hit of the item,
+//   k_ray      eight lanes per work item, two triangles each: first
+//              Moller-Trumbore hit of the item,
 //              combined per particle with atomicMin over (cell-visit rank,
 //              sorted slot) = the reference's first hit in x->y->z order
 //              (src/simulation/vein_collisions.cuh:60-93)
@@ -112,8 +113,7 @@ __device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVe
 //   sign(d) != sign(a), d != 0 and |d| >= |a| 2^-100  =>  u < 0 (no underflow)
 //   same signs and |d| > 1.001 |a|                   =>  u > 1 (rounding < 2^-22)
 // so exactly the candidates the reference rejects at this test are skipped.
-__device__ __forceinline__ bool ray_tri_pre(f3 v0, f3 edge1, f3 edge2, f3 orig, f3 dir, float& t_out, f3& normal,
-                                            f3& refl, uint32_t& fl)
+__device__ __forceinline__ bool ray_tri_t(f3 v0, f3 edge1, f3 edge2, f3 orig, f3 dir, float& t_out)
 {
     const float EPS = 0.000001f;
     f3 h = cross(dir, edge2);
@@ -132,13 +132,20 @@ __device__ __forceinline__ bool ray_tri_pre(f3 v0, f3 edge1, f3 edge2, f3 orig, 
     float v = f * dot(dir, q);
     if (v < 0.0f || u + v > 1.0f) return false;
     float t = f * dot(edge2, q);
-    if (t > EPS) {
-        t_out = t;
-        normal = normalize(cross(edge2, edge1), fl);
-        refl = sub(dir, smul(2.0f * dot(dir, normal), normal));
-        return true;
-    }
-    return false;
+    t_out = t;
+    return t > EPS;
+}
+
+// The hit of ray_tri_t with the reference's normal and reflected direction.
+__device__ __forceinline__ bool ray_tri_pre(f3 v0, f3 edge1, f3 edge2, f3 orig, f3 dir, float& t_out, f3& normal,
+                                            f3& refl, uint32_t& fl)
+{
+    float t;
+    if (!ray_tri_t(v0, edge1, edge2, orig, dir, t)) return false;
+    t_out = t;
+    normal = normalize(cross(edge2, edge1), fl);
+    refl = sub(dir, smul(2.0f * dot(dir, normal), normal));
+    return true;
 }
 
 // calculateBaricentric (edges v1-v0, v2-v1), src/simulation/vein_collisions.cu:47-61.
@@ -209,7 +216,10 @@ constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kScanSerial = 0xFFFFFFFEu;  // work list full: k_cells scans serially
 constexpr uint32_t kSlotBits = 27;             // best = rank << 27 | sorted triangle slot
 constexpr uint32_t kItemTris = 16;
-constexpr uint32_t kRaySplit = 4;  // k_ray blocks per work region
+constexpr uint32_t kRayLanes = 8;  // k_ray lanes per work item
+// k_ray blocks per work region: about kRayBlocks blocks in all, 1 .. 8 per
+// region (few regions: more blocks share one; many: one block each)
+constexpr uint32_t kRayBlocks = 8192, kRaySplitMax = 8;
 
 // Triangle-grid neighbourhood of a particle (face clipped, unsigned ids as
 // in src/simulation/vein_collisions.cu:80-232).
@@ -697,54 +707,68 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
     respond_integrate<FUSED>(A, act, i, T, a, cl, cell, x1, v1, F, fl, s_tele);  // raises fl
 }
 
-// One thread per work item: first hit among its <= 16 triangles, folded into
-// the particle's minimum (cell rank, sorted slot).  One block per work region.
-// The items of work region r taken by block `sub` of its kRaySplit blocks.
-__device__ __forceinline__ void ray_region(const StepArgs& A, uint32_t r, uint32_t sub)
+// kRayLanes lanes per work item, each testing kItemTris / kRayLanes
+// consecutive triangles with their loads issued together: the item's first
+// hit in sorted order is the first hit of its lowest hitting lane (the
+// reference's scan stops there), folded into the particle's minimum (cell
+// rank, sorted slot).  The hit test alone decides; the normal of the winning
+// triangle is formed by k_cells.  The items of work region r taken by block
+// `sub` of its `split` blocks.
+__device__ __forceinline__ void ray_region(const StepArgs& A, uint32_t r, uint32_t sub, uint32_t split)
 {
+    constexpr uint32_t G = kRayLanes, K = kItemTris / kRayLanes;
+    static_assert(G * K == kItemTris && G <= 64 && (G & (G - 1)) == 0, "lanes per item divide the item");
     const uint32_t n = min(A.wcount[r], A.region_cap);
     const uint4* items = A.work + (size_t)r * A.region_cap;
     const int64_t np = A.tt.np;
+    const uint32_t lane = threadIdx.x & 63u, g = threadIdx.x % G;
+    const uint32_t per_block = blockDim.x / G;
     uint32_t fl = 0;
-    for (uint32_t w = sub * blockDim.x + threadIdx.x; w < n; w += kRaySplit * blockDim.x) {
-        const uint4 it = items[w];
-        const int32_t i = (int32_t)it.x;
-        const f3 x1 = load3(A.pos, np, i);
-        const f3 dir = normalize(load3(A.vel, np, i), fl);
-        const uint32_t u1 = min(it.y + it.w, (uint32_t)A.ntri);
-        // triangles in sorted order, their data loaded four at a time
-        bool found = false;
-        for (uint32_t u = it.y; u < u1 && !found; u += 4) {
-            float4 a[4], e1[4], e2[4];
+    // base is block-uniform, so every lane reaches the ballot
+    for (uint32_t base = sub * per_block; base < n; base += split * per_block) {
+        const uint32_t w = base + threadIdx.x / G;
+        uint32_t first = K;  // the lane's first hitting triangle
+        uint4 it = make_uint4(0u, 0u, 0u, 0u);
+        if (w < n) {
+            it = items[w];
+            const int32_t i = (int32_t)it.x;
+            const uint32_t u0 = it.y + g * K;
+            const uint32_t u1 = min(it.y + it.w, (uint32_t)A.ntri);
+            const f3 x1 = load3(A.pos, np, i);
+            const f3 dir = normalize(load3(A.vel, np, i), fl);
+            float4 ta[K], te1[K], te2[K];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (u + k < u1) {
-                    a[k] = A.tdata[3 * (size_t)(u + k)];
-                    e1[k] = A.tdata[3 * (size_t)(u + k) + 1];
-                    e2[k] = A.tdata[3 * (size_t)(u + k) + 2];
+            for (uint32_t k = 0; k < K; ++k) {
+                if (u0 + k < u1) {
+                    ta[k] = A.tdata[3 * (size_t)(u0 + k)];
+                    te1[k] = A.tdata[3 * (size_t)(u0 + k) + 1];
+                    te2[k] = A.tdata[3 * (size_t)(u0 + k) + 2];
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (found || u + k >= u1) continue;
+            for (uint32_t k = 0; k < K; ++k) {
                 float rt;
-                f3 nrm, refl;
-                if (ray_tri_pre(mk(a[k].x, a[k].y, a[k].z), mk(e1[k].x, e1[k].y, e1[k].z),
-                                mk(e2[k].x, e2[k].y, e2[k].z), x1, dir, rt, nrm, refl, fl)) {
-                    atomicMin(&A.best[i], (it.z << kSlotBits) | (u + k));
-                    found = true;
-                }
+                if (first == K && u0 + k < u1 &&
+                    ray_tri_t(mk(ta[k].x, ta[k].y, ta[k].z), mk(te1[k].x, te1[k].y, te1[k].z),
+                              mk(te2[k].x, te2[k].y, te2[k].z), x1, dir, rt))
+                    first = k;
             }
         }
+        const bool hit = first < K;
+        const uint64_t m = __ballot(hit);
+        const uint64_t gm = (m >> (lane & ~(G - 1u))) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
+        if (hit && g == (uint32_t)__builtin_ctzll(gm))
+            atomicMin(&A.best[it.x], (it.z << kSlotBits) | (it.y + g * K + first));
     }
+    if (fl) atomicOr(A.flags, fl);
 }
 
-__global__ void __launch_bounds__(256) k_ray(StepArgs A)
+__global__ void __launch_bounds__(256) k_ray(StepArgs A, uint32_t split)
 {
-    // kRaySplit blocks share the work region of k_raycand block r; block
+    // `split` blocks share the work region of k_raycand block r; block
     // b = s * nregions + r runs on the XCD (b % 8 == r % 8) whose L2 holds it
-    const uint32_t nreg = gridDim.x / kRaySplit;
-    ray_region(A, blockIdx.x % nreg, blockIdx.x / nreg);
+    const uint32_t nreg = gridDim.x / split;
+    ray_region(A, blockIdx.x % nreg, blockIdx.x / nreg, split);
 }
 
 // Vein hit response of particle i (detectVeinCollisions response,
@@ -1317,7 +1341,10 @@ void launch_raycand(const StepArgs& A, int nchunks, hipStream_t st)
 }
 void launch_ray(const StepArgs& A, hipStream_t st)
 {
-    if (A.nchunks > 0) hipLaunchKernelGGL(k_ray, dim3(chunk_grid(A.nchunks).x * kRaySplit), dim3(256), 0, st, A);
+    if (A.nchunks <= 0) return;
+    const uint32_t nreg = chunk_grid(A.nchunks).x;
+    const uint32_t split = std::max(1u, std::min(kRaySplitMax, kRayBlocks / nreg));
+    hipLaunchKernelGGL(k_ray, dim3(nreg * split), dim3(256), 0, st, A, split);
 }
 void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
 {
