@@ -65,9 +65,11 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
                    and 2-bit code written (<= 2 bytes) per occupied cell
                                                                    8 / particle + 10 / cell
       k_raycand    x, v read 24, first-hit key written 4; work items written
-                   16 each (triangle-cell boxes counted in k_tri_box)
+                   16 each (triangle-cell boxes counted in k_vein_prep)
                                                                    28 / particle + 16 / item
-      k_ray        work item 16 + x, v 24 per item (triangle data counted in k_tri_box)
+      k_ray        work item 16 + x, v 24 per item (triangle data counted in k_vein_prep)
+      k_raycand_ray  k_raycand and k_ray in one launch, the work items in LDS:
+                   x, v read 24, first-hit key written 4          28 / particle
       k_cells      x, v, F read 36, own key 4, sorted-slot records of the
                    contact candidates read 32, first-hit key read 4, x, v, F
                    written 36; centre written 12 per blood cell; table entry
@@ -87,6 +89,7 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
         "k_occ": 8 * np_ + 10 * n_occ,
         "k_raycand": 28 * np_ + 16 * n_items,
         "k_ray": 40 * n_items,
+        "k_raycand_ray": 28 * np_,
         "k_cells": 112 * np_ + 12 * nc + 10 * n_occ,
         "k_vein_prep": 120 * nv + 76 * nt,
         "k_vein": 76 * nv,

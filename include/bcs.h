@@ -248,6 +248,18 @@ int bcs_sim_step(bcs_sim* sim, int32_t n_steps);
  * counterpart in the reference (its frames are direct launches). */
 int bcs_sim_use_graph(bcs_sim* sim, int32_t enabled);
 
+/* How a frame runs the vein ray test (detectVeinCollisions' triangle scan,
+ * src/simulation/vein_collisions.cu:181-233): BCS_RAY_SPLIT = candidates, then
+ * the ray test as a second launch spread over up to 8 blocks per cell block;
+ * BCS_RAY_FUSED = one launch in which each cell block tests its own
+ * candidates from LDS; BCS_RAY_AUTO (the default) fuses when the split launch
+ * would get one block per cell block (more than 4,096 cell blocks).  Results
+ * are identical in every mode.  No counterpart in the reference. */
+#define BCS_RAY_AUTO 0
+#define BCS_RAY_SPLIT 1
+#define BCS_RAY_FUSED 2
+int bcs_sim_set_ray_mode(bcs_sim* sim, int32_t mode);
+
 int bcs_sim_synchronize(bcs_sim* sim);
 
 /* Time n fused steps with HIP events on the simulation's stream (ms total). */
@@ -255,7 +267,9 @@ int bcs_sim_time_steps(bcs_sim* sim, int32_t n_steps, float* total_ms);
 
 /* Per-kernel average durations (ms) of the fused step over n steps, measured
  * with HIP events around each launch on the simulation's stream.  names_out
- * receives a ';'-separated list of kernel names matching ms_out order. */
+ * receives a ';'-separated list of kernel names matching ms_out order.  In
+ * fused ray mode the candidate-and-test launch is named k_raycand_ray and
+ * the k_ray entry reads 0 (no launch). */
 int bcs_sim_profile_kernels(bcs_sim* sim, int32_t n_steps, float* ms_out, int32_t capacity,
                             int32_t* n_kernels, char* names_out, int32_t names_capacity);
 

@@ -34,6 +34,8 @@ FLAG_SCAN_TIMEOUT = 0x4
 FLAG_RADIUS_CLAMPED = 0x8
 FLAG_RAY_SERIAL = 0x10
 
+RAY_AUTO, RAY_SPLIT, RAY_FUSED = 0, 1, 2
+
 VEIN_CYLINDER, VEIN_Y = 0, 1
 CELL_ICOSAHEDRON, CELL_DISC42 = 0, 1
 PLACE_RANDOM, PLACE_LATTICE = 0, 1
@@ -215,6 +217,7 @@ def _sig(lib):
     lib.bcs_sim_propagate_all.argtypes = [vp]
     lib.bcs_sim_step.argtypes = [vp, C.c_int32]
     lib.bcs_sim_use_graph.argtypes = [vp, C.c_int32]
+    lib.bcs_sim_set_ray_mode.argtypes = [vp, C.c_int32]
     lib.bcs_sim_synchronize.argtypes = [vp]
     lib.bcs_sim_time_steps.argtypes = [vp, C.c_int32, f32p]
     lib.bcs_sim_profile_kernels.argtypes = [vp, C.c_int32, f32p, C.c_int32, i32p, C.c_char_p, C.c_int32]
@@ -476,6 +479,11 @@ class Simulation:
 
     def use_graph(self, enabled: bool = True):
         check(self.lib.bcs_sim_use_graph(self.h, 1 if enabled else 0), "use_graph")
+
+    def set_ray_mode(self, mode: int):
+        """RAY_AUTO, RAY_SPLIT or RAY_FUSED: how frames run the vein ray test
+        (identical results)."""
+        check(self.lib.bcs_sim_set_ray_mode(self.h, int(mode)), "set_ray_mode")
 
     def synchronize(self):
         check(self.lib.bcs_sim_synchronize(self.h), "synchronize")

@@ -18,6 +18,8 @@
 
 namespace bcs {
 void launch_raycand(const StepArgs& A, int nchunks, hipStream_t st);
+void launch_raycand_ray(const StepArgs& A, int nchunks, hipStream_t st);
+bool ray_fuse_auto(int nchunks);
 void launch_ray(const StepArgs& A, hipStream_t st);
 void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st);
 void launch_integrate(const StepArgs& A, hipStream_t st);
@@ -100,6 +102,11 @@ struct bcs_sim {
     hipGraphExec_t graph_exec_b = nullptr;  // ... and with it (the previous frame's k_vein)
     hipGraphExec_t graph_exec_m = nullptr;  // kGraphFrames frames with it, in one launch
     bool use_graph = true;
+    int32_t ray_mode = BCS_RAY_AUTO;
+    bool ray_fused() const
+    {
+        return ray_mode == BCS_RAY_FUSED || (ray_mode == BCS_RAY_AUTO && ray_fuse_auto(nchunks));
+    }
     hipGraph_t graph = nullptr, graph_b = nullptr, graph_m = nullptr;
     std::vector<float> host_bs;
     std::vector<float> host_smallest;
@@ -238,8 +245,12 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
         if (lead_vein) launch_vein(Va, 7, s->side);
         launch_vein_prep(Va, s->tg, s->tidx, s->tdata, s->t_aabb, s->side);
-        launch_raycand(A, s->nchunks, s->side);
-        launch_ray(A, s->side);
+        if (s->ray_fused()) {
+            launch_raycand_ray(A, s->nchunks, s->side);
+        } else {
+            launch_raycand(A, s->nchunks, s->side);
+            launch_ray(A, s->side);
+        }
         HIPCHK(hipEventRecord(s->ev_join, s->side));
         RC(enqueue_particle_grid(s));
         HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
@@ -249,11 +260,14 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         tb(kGridKernels + 0);
         launch_vein_prep(Va, s->tg, s->tidx, s->tdata, s->t_aabb, st);
         te(kGridKernels + 0);
+        // fused: the one launch is timed in the k_raycand slot (named
+        // k_raycand_ray), the k_ray slot times no launch
         tb(kGridKernels + 1);
-        launch_raycand(A, s->nchunks, st);
+        if (s->ray_fused()) launch_raycand_ray(A, s->nchunks, st);
+        else launch_raycand(A, s->nchunks, st);
         te(kGridKernels + 1);
         tb(kGridKernels + 2);
-        launch_ray(A, st);
+        if (!s->ray_fused()) launch_ray(A, st);
         te(kGridKernels + 2);
     }
     tb(kGridKernels + 3);
@@ -773,6 +787,19 @@ extern "C" int bcs_sim_use_graph(bcs_sim* s, int32_t enabled)
     return BCS_OK;
 }
 
+extern "C" int bcs_sim_set_ray_mode(bcs_sim* s, int32_t mode)
+{
+    if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
+    if (mode != BCS_RAY_AUTO && mode != BCS_RAY_SPLIT && mode != BCS_RAY_FUSED)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_set_ray_mode: mode is BCS_RAY_AUTO, _SPLIT or _FUSED");
+    if (mode == s->ray_mode) return BCS_OK;
+    RC(set_dev(s));
+    HIPCHK(hipStreamSynchronize(s->stream));  // the graphs hold the old launches
+    drop_graph(s);
+    s->ray_mode = mode;
+    return BCS_OK;
+}
+
 extern "C" int bcs_sim_synchronize(bcs_sim* s)
 {
     RC(set_dev(s));
@@ -823,6 +850,7 @@ extern "C" int bcs_sim_profile_kernels(bcs_sim* s, int32_t n_steps, float* ms_ou
             if (hipEventElapsedTime(&ms, eb[k], ee[k]) == hipSuccess) acc[k] += ms;
         }
     }
+    if (s->ray_fused()) acc[kGridKernels + 2] = 0.0;  // no k_ray launch
     for (int k = 0; k < kFusedKernels; ++k) {
         ms_out[k] = (float)(acc[k] / n_steps);
         (void)hipEventDestroy(eb[k]);
@@ -831,7 +859,9 @@ extern "C" int bcs_sim_profile_kernels(bcs_sim* s, int32_t n_steps, float* ms_ou
     *n_kernels = kFusedKernels;
     s->copies_current = false;  // the profiled frames moved the particles
     if (names_out && names_capacity > 0) {
-        std::strncpy(names_out, kKernelNames, (size_t)names_capacity - 1);
+        std::string names = kKernelNames;
+        if (s->ray_fused()) names.replace(names.find("k_raycand;"), 10, "k_raycand_ray;");
+        std::strncpy(names_out, names.c_str(), (size_t)names_capacity - 1);
         names_out[names_capacity - 1] = 0;
     }
     return rc;

@@ -220,6 +220,8 @@ constexpr uint32_t kRayLanes = 8;  // k_ray lanes per work item
 // k_ray blocks per work region: about kRayBlocks blocks in all, 1 .. 8 per
 // region (few regions: more blocks share one; many: one block each)
 constexpr uint32_t kRayBlocks = 8192, kRaySplitMax = 8;
+constexpr uint32_t kRayLdsItems = 2048;  // work items a k_raycand_ray block holds (8 bytes each)
+static_assert(kBlock <= 256 && kItemTris < 32, "an LDS ray item packs lane (8 bits), rank (5), count (5)");
 
 // Triangle-grid neighbourhood of a particle (face clipped, unsigned ids as
 // in src/simulation/vein_collisions.cu:80-232).
@@ -386,13 +388,55 @@ __device__ bool ray_first_hit_serial(const StepArgs& A, f3 x1, f3 dir, float& rt
 }
 
 
+// Lane g of the kRayLanes lanes of work item `it` tests the item's triangles
+// it.y + g*K .. + K-1 (K = kItemTris / kRayLanes), their loads issued
+// together; returns the first hitting k, or K.
+__device__ __forceinline__ uint32_t item_lane_first_hit(const StepArgs& A, uint4 it, uint32_t g, f3 x1, f3 dir)
+{
+    constexpr uint32_t K = kItemTris / kRayLanes;
+    const uint32_t u0 = it.y + g * K;
+    const uint32_t u1 = min(it.y + it.w, (uint32_t)A.ntri);
+    float4 ta[K], te1[K], te2[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        if (u0 + k < u1) {
+            ta[k] = A.tdata[3 * (size_t)(u0 + k)];
+            te1[k] = A.tdata[3 * (size_t)(u0 + k) + 1];
+            te2[k] = A.tdata[3 * (size_t)(u0 + k) + 2];
+        }
+    }
+    uint32_t first = K;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        float rt;
+        if (first == K && u0 + k < u1 &&
+            ray_tri_t(mk(ta[k].x, ta[k].y, ta[k].z), mk(te1[k].x, te1[k].y, te1[k].z), mk(te2[k].x, te2[k].y, te2[k].z),
+                      x1, dir, rt))
+            first = k;
+    }
+    return first;
+}
+
+// The item's first hit in sorted order is the first hit of its lowest hitting
+// lane (the reference's scan stops there).  Every lane of the wave calls this;
+// returns whether this lane holds the item's first hit.
+__device__ __forceinline__ bool item_first_lane(bool hit, uint32_t g)
+{
+    constexpr uint32_t G = kRayLanes;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t m = __ballot(hit);
+    const uint64_t gm = (m >> (lane & ~(G - 1u))) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
+    return hit && g == (uint32_t)__builtin_ctzll(gm);
+}
+
 // Vein ray candidates (part of detectVeinCollisions, src/simulation/vein_collisions.cu:181-233):
 // per particle, the occupied triangle cells of its 27-neighbourhood whose box
 // the ray from x along v/|v| can reach, split into work items of <= 16 sorted
 // triangles written to this block's own region of the work list.  Needs only
 // the frame's starting x, v and the triangle data, so it runs beside the
 // particle grid build.
-__global__ void __launch_bounds__(kBlock) k_raycand(StepArgs A)
+template <bool FUSED>
+__device__ __forceinline__ void raycand_block(const StepArgs& A)
 {
     __shared__ uint32_t s_wsum[kBlock / 64];
     __shared__ uint32_t s_end;
@@ -468,15 +512,42 @@ __global__ void __launch_bounds__(kBlock) k_raycand(StepArgs A)
     const uint32_t off = wave_off + incl - nitems;  // within the region
     // particles whose items fit form a prefix of the scan order; the region's
     // count ends with the last of them (no unwritten slots below it)
-    const bool fits = (uint64_t)off + nitems <= (uint64_t)A.region_cap;
+    const uint32_t cap = FUSED ? min(A.region_cap, kRayLdsItems) : A.region_cap;
+    const bool fits = (uint64_t)off + nitems <= (uint64_t)cap;
     if (act && nitems > 0 && fits) atomicMax(&s_end, off + nitems);
-    __syncthreads();
-    if (l == 0) A.wcount[blockIdx.x] = s_end;
-    if (act) {
-        A.best[i] = (nitems > 0 && !fits) ? kScanSerial : kNoHit;
-        if (nitems > 0 && !fits) fl |= BCS_FLAG_RAY_SERIAL;
-        if (nitems > 0 && fits) {
-            uint4* out = A.work + (size_t)blockIdx.x * A.region_cap;
+    if (!FUSED) {
+        __syncthreads();
+        if (l == 0) A.wcount[blockIdx.x] = s_end;
+        if (act) {
+            A.best[i] = (nitems > 0 && !fits) ? kScanSerial : kNoHit;
+            if (nitems > 0 && !fits) fl |= BCS_FLAG_RAY_SERIAL;
+            if (nitems > 0 && fits) {
+                uint4* out = A.work + (size_t)blockIdx.x * A.region_cap;
+                uint32_t w = off;
+                uint32_t m = pmask;
+                while (m) {
+                    const int r = __builtin_ctz(m);
+                    m &= m - 1u;
+                    const int dx = r / 9 - 1, c = r % 9;
+                    const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
+                    const uint2 ent = A.t_table[nk];
+                    for (uint32_t u = ent.x; u < ent.x + ent.y && w < off + nitems; u += kItemTris)
+                        out[w++] = make_uint4((uint32_t)i, u, (uint32_t)r, min(kItemTris, ent.x + ent.y - u));
+                }
+            }
+        }
+    } else {
+        // the block's items, rays and per-particle minima stay in LDS; the
+        // block tests its own items (kRayLanes lanes per item)
+        // item: {lane | rank << 8 | count << 13, first sorted slot}
+        __shared__ uint2 s_items[kRayLdsItems];
+        __shared__ float s_ray[6][kBlock];
+        __shared__ uint32_t s_best[kBlock];
+        s_best[l] = (act && nitems > 0 && !fits) ? kScanSerial : kNoHit;
+        if (act && nitems > 0 && !fits) fl |= BCS_FLAG_RAY_SERIAL;
+        s_ray[0][l] = x1.x; s_ray[1][l] = x1.y; s_ray[2][l] = x1.z;
+        s_ray[3][l] = dir.x; s_ray[4][l] = dir.y; s_ray[5][l] = dir.z;
+        if (act && nitems > 0 && fits) {
             uint32_t w = off;
             uint32_t m = pmask;
             while (m) {
@@ -486,12 +557,39 @@ __global__ void __launch_bounds__(kBlock) k_raycand(StepArgs A)
                 const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
                 const uint2 ent = A.t_table[nk];
                 for (uint32_t u = ent.x; u < ent.x + ent.y && w < off + nitems; u += kItemTris)
-                    out[w++] = make_uint4((uint32_t)i, u, (uint32_t)r, min(kItemTris, ent.x + ent.y - u));
+                    s_items[w++] = make_uint2((uint32_t)l | (uint32_t)r << 8 | min(kItemTris, ent.x + ent.y - u) << 13, u);
             }
         }
+        __syncthreads();
+        const uint32_t total = s_end;
+        if (l == 0) A.wcount[blockIdx.x] = total;
+        constexpr uint32_t G = kRayLanes, K = kItemTris / kRayLanes;
+        const uint32_t g = l % G;
+        // base is block-uniform, so every lane reaches the ballot
+        for (uint32_t base = 0; base < total; base += kBlock / G) {
+            const uint32_t w = base + l / G;
+            uint32_t first = K;
+            uint4 it = make_uint4(0u, 0u, 0u, 0u);
+            if (w < total) {
+                const uint2 q = s_items[w];
+                it = make_uint4(q.x & 0xFFu, q.y, (q.x >> 8) & 31u, q.x >> 13);
+                const uint32_t pl = it.x;
+                first = item_lane_first_hit(A, it, g, mk(s_ray[0][pl], s_ray[1][pl], s_ray[2][pl]),
+                                            mk(s_ray[3][pl], s_ray[4][pl], s_ray[5][pl]));
+            }
+            if (item_first_lane(first < K, g)) atomicMin(&s_best[it.x], (it.z << kSlotBits) | (it.y + g * K + first));
+        }
+        __syncthreads();
+        if (act) A.best[i] = s_best[l];
     }
     if (fl) atomicOr(A.flags, fl);
 }
+
+__global__ void __launch_bounds__(kBlock) k_raycand(StepArgs A) { raycand_block<false>(A); }
+
+// k_raycand and k_ray in one launch: each block keeps its work items in LDS
+// (at most kRayLdsItems; rays that do not fit are scanned serially by k_cells).
+__global__ void __launch_bounds__(kBlock) k_raycand_ray(StepArgs A) { raycand_block<true>(A); }
 
 template <bool FUSED>
 __device__ __forceinline__ void respond_integrate(const StepArgs& A, bool act, int32_t i, int T, int a, int cl,
@@ -707,13 +805,10 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
     respond_integrate<FUSED>(A, act, i, T, a, cl, cell, x1, v1, F, fl, s_tele);  // raises fl
 }
 
-// kRayLanes lanes per work item, each testing kItemTris / kRayLanes
-// consecutive triangles with their loads issued together: the item's first
-// hit in sorted order is the first hit of its lowest hitting lane (the
-// reference's scan stops there), folded into the particle's minimum (cell
-// rank, sorted slot).  The hit test alone decides; the normal of the winning
-// triangle is formed by k_cells.  The items of work region r taken by block
-// `sub` of its `split` blocks.
+// kRayLanes lanes per work item; the item's first hit is folded into the
+// particle's minimum (cell rank, sorted slot).  The hit test alone decides;
+// the normal of the winning triangle is formed by k_cells.  The items of work
+// region r taken by block `sub` of its `split` blocks.
 __device__ __forceinline__ void ray_region(const StepArgs& A, uint32_t r, uint32_t sub, uint32_t split)
 {
     constexpr uint32_t G = kRayLanes, K = kItemTris / kRayLanes;
@@ -721,44 +816,22 @@ __device__ __forceinline__ void ray_region(const StepArgs& A, uint32_t r, uint32
     const uint32_t n = min(A.wcount[r], A.region_cap);
     const uint4* items = A.work + (size_t)r * A.region_cap;
     const int64_t np = A.tt.np;
-    const uint32_t lane = threadIdx.x & 63u, g = threadIdx.x % G;
+    const uint32_t g = threadIdx.x % G;
     const uint32_t per_block = blockDim.x / G;
     uint32_t fl = 0;
     // base is block-uniform, so every lane reaches the ballot
     for (uint32_t base = sub * per_block; base < n; base += split * per_block) {
         const uint32_t w = base + threadIdx.x / G;
-        uint32_t first = K;  // the lane's first hitting triangle
+        uint32_t first = K;
         uint4 it = make_uint4(0u, 0u, 0u, 0u);
         if (w < n) {
             it = items[w];
             const int32_t i = (int32_t)it.x;
-            const uint32_t u0 = it.y + g * K;
-            const uint32_t u1 = min(it.y + it.w, (uint32_t)A.ntri);
             const f3 x1 = load3(A.pos, np, i);
             const f3 dir = normalize(load3(A.vel, np, i), fl);
-            float4 ta[K], te1[K], te2[K];
-#pragma unroll
-            for (uint32_t k = 0; k < K; ++k) {
-                if (u0 + k < u1) {
-                    ta[k] = A.tdata[3 * (size_t)(u0 + k)];
-                    te1[k] = A.tdata[3 * (size_t)(u0 + k) + 1];
-                    te2[k] = A.tdata[3 * (size_t)(u0 + k) + 2];
-                }
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < K; ++k) {
-                float rt;
-                if (first == K && u0 + k < u1 &&
-                    ray_tri_t(mk(ta[k].x, ta[k].y, ta[k].z), mk(te1[k].x, te1[k].y, te1[k].z),
-                              mk(te2[k].x, te2[k].y, te2[k].z), x1, dir, rt))
-                    first = k;
-            }
+            first = item_lane_first_hit(A, it, g, x1, dir);
         }
-        const bool hit = first < K;
-        const uint64_t m = __ballot(hit);
-        const uint64_t gm = (m >> (lane & ~(G - 1u))) & (G == 64 ? ~0ull : ((1ull << G) - 1ull));
-        if (hit && g == (uint32_t)__builtin_ctzll(gm))
-            atomicMin(&A.best[it.x], (it.z << kSlotBits) | (it.y + g * K + first));
+        if (item_first_lane(first < K, g)) atomicMin(&A.best[it.x], (it.z << kSlotBits) | (it.y + g * K + first));
     }
     if (fl) atomicOr(A.flags, fl);
 }
@@ -1339,12 +1412,23 @@ void launch_raycand(const StepArgs& A, int nchunks, hipStream_t st)
 {
     if (nchunks > 0) hipLaunchKernelGGL(k_raycand, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
 }
+uint32_t ray_split(int nchunks)
+{
+    const uint32_t nreg = chunk_grid(nchunks).x;
+    return nreg ? std::max(1u, std::min(kRaySplitMax, kRayBlocks / nreg)) : 1u;
+}
+// The fused launch pays off once k_ray would run one block per region: the
+// cell blocks then fill the device by themselves.
+bool ray_fuse_auto(int nchunks) { return ray_split(nchunks) == 1u; }
+void launch_raycand_ray(const StepArgs& A, int nchunks, hipStream_t st)
+{
+    if (nchunks > 0) hipLaunchKernelGGL(k_raycand_ray, chunk_grid(nchunks), dim3(kBlock), 0, st, A);
+}
 void launch_ray(const StepArgs& A, hipStream_t st)
 {
     if (A.nchunks <= 0) return;
-    const uint32_t nreg = chunk_grid(A.nchunks).x;
-    const uint32_t split = std::max(1u, std::min(kRaySplitMax, kRayBlocks / nreg));
-    hipLaunchKernelGGL(k_ray, dim3(nreg * split), dim3(256), 0, st, A, split);
+    const uint32_t split = ray_split(A.nchunks);
+    hipLaunchKernelGGL(k_ray, dim3(chunk_grid(A.nchunks).x * split), dim3(256), 0, st, A, split);
 }
 void launch_cells(const StepArgs& A, int nchunks, bool fused, hipStream_t st)
 {

@@ -231,6 +231,30 @@ def test_ray_work_overflow_falls_back_to_serial_scan(bcs, orc):
     assert sim.error_flags() & bcs.FLAG_RAY_SERIAL
 
 
+@pytest.mark.parametrize("mode", ["split", "fused"])
+def test_ray_modes_match_oracle(bcs, orc, mode):
+    """The ray test as two launches (candidates, then 8 lanes per item over
+    several blocks per cell block) or as one launch (each cell block tests its
+    own candidates from LDS): 40 graph frames against the oracle, with the
+    serial fallback exercised by a short work list in a second run."""
+    sc = _scene(bcs, "c0")
+    for items in (0, 240):
+        ph = sc.physics
+        ph.ray_work_items = items
+        sim = bcs.Simulation.from_scene(sc, physics=ph)
+        sim.set_ray_mode(bcs.RAY_SPLIT if mode == "split" else bcs.RAY_FUSED)
+        o = orc.Oracle(ph, sc.types, sc.mesh)
+        o.set_particles(sc.positions, sc.velocities, sc.forces)
+        sim.step(40)
+        o.step(40)
+        for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+            assert_bits_equal(x, y, f"{mode} ray test, work list {items}: {n}")
+        assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], f"{mode} ray test, work list {items}: vein")
+        assert sim.frame_stats()["ray_work_items"] > 0, "the last frame should have ray candidates"
+        if items:
+            assert sim.error_flags() & bcs.FLAG_RAY_SERIAL
+
+
 def test_fused_40_steps_mixed_types(bcs, orc):
     """Two cell types (discs and icosahedra): 40 fused frames, across two
     refreshes of the spatial processing order of the cells."""
