@@ -220,6 +220,7 @@ constexpr uint32_t kRayLanes = 8;  // k_ray lanes per work item
 // k_ray blocks per work region: about kRayBlocks blocks in all, 1 .. 8 per
 // region (few regions: more blocks share one; many: one block each)
 constexpr uint32_t kRayBlocks = 8192, kRaySplitMax = 8;
+constexpr uint32_t kBoxBatch = 8;  // triangle-cell boxes k_raycand loads at once
 constexpr uint32_t kRayLdsItems = 2048;  // work items a k_raycand_ray block holds (8 bytes each)
 static_assert(kBlock <= 256 && kItemTris < 32, "an LDS ray item packs lane (8 bits), rank (5), count (5)");
 
@@ -468,26 +469,30 @@ __device__ __forceinline__ void raycand_block(const StepArgs& A)
             if (om) {
                 const f3 inv = inv_dir(dir);
                 const float bm = box_margin(A);
-                // boxes of one x-slab's occupied cells loaded together, then
-                // tested; a passing cell's triangle count is in hi.w
+                // boxes of up to kBoxBatch occupied cells loaded together,
+                // then tested; a passing cell's triangle count is in hi.w
+                uint32_t m = om;
+                while (m) {
+                    float4 lo[kBoxBatch], hi[kBoxBatch];
+                    int rr[kBoxBatch];
 #pragma unroll
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const uint32_t sm = (om >> ((dx + 1) * 9)) & 0x1FFu;
-                    if (!sm) continue;
-                    float4 lo[9], hi[9];
-#pragma unroll
-                    for (int c = 0; c < 9; ++c) {
-                        if ((sm >> c) & 1u) {
+                    for (int k = 0; k < (int)kBoxBatch; ++k) {
+                        rr[k] = -1;
+                        if (m) {
+                            const int r = __builtin_ctz(m);
+                            m &= m - 1u;
+                            rr[k] = r;
+                            const int dx = r / 9 - 1, c = r % 9;
                             const int64_t nk = (int64_t)rc.cellId + (c % 3 - 1) * rc.cXY + (c / 3 - 1) * rc.cX + dx;
-                            lo[c] = A.t_aabb[2 * nk];
-                            hi[c] = A.t_aabb[2 * nk + 1];
+                            lo[k] = A.t_aabb[2 * nk];
+                            hi[k] = A.t_aabb[2 * nk + 1];
                         }
                     }
 #pragma unroll
-                    for (int c = 0; c < 9; ++c)
-                        if (((sm >> c) & 1u) && ray_box(x1, inv, dir, lo[c], hi[c], bm)) {
-                            pmask |= 1u << ((dx + 1) * 9 + c);
-                            nitems += (min(__float_as_uint(hi[c].w), (uint32_t)A.ntri) + kItemTris - 1) / kItemTris;
+                    for (int k = 0; k < (int)kBoxBatch; ++k)
+                        if (rr[k] >= 0 && ray_box(x1, inv, dir, lo[k], hi[k], bm)) {
+                            pmask |= 1u << rr[k];
+                            nitems += (min(__float_as_uint(hi[k].w), (uint32_t)A.ntri) + kItemTris - 1) / kItemTris;
                         }
                 }
             }
