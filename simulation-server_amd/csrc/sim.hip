@@ -31,10 +31,9 @@ void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t
 void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st);
 void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
-void launch_vein_prep(const VeinArgs& V, const DeviceGrid& tg, const uint32_t* idx, float4* tdata, float4* aabb,
-                      hipStream_t st);
-void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
-                    float4* aabb, hipStream_t st);
+void launch_vein_prep(const VeinArgs& V, int32_t ntri, const uint4* trun, const uint4* trec, float4* tdata,
+                      float4* aabb, hipStream_t st);
+void launch_tri_records(const DeviceGrid& tg, const uint32_t* idx, uint4* trun, uint4* trec, hipStream_t st);
 void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st);
 void launch_export(const float* soa, int64_t n, int64_t first, int64_t count, int stride, float* out, hipStream_t st);
 void launch_refresh_copies(const float* pos, const float* vel, int32_t n, const int32_t* perm, float4* pvs,
@@ -85,6 +84,8 @@ struct bcs_sim {
     int32_t *vhead = nullptr, *vnext = nullptr;  // per-vertex lists of hit entries
     float4* t_aabb = nullptr;  // triangle-cell boxes by cell key (per frame)
     uint32_t* t_nmask = nullptr;  // per triangle cell: occupied neighbourhood mask
+    uint4* trun = nullptr;  // per occupied triangle cell: run record (k_tri_records)
+    uint4* trec = nullptr;  // per sorted triangle: vertex indices and triangle
     float4* tdata = nullptr;   // per sorted triangle: (v0 | id), e1, e2 (per frame)
     uint32_t* best = nullptr;  // per particle first-hit key
     uint4* work = nullptr;     // ray work items
@@ -201,13 +202,14 @@ int set_dev(bcs_sim* s)
 
 // Triangle grid on the triangle centres: frozen after creation (the
 // reference's single-GPU behaviour) unless dynamic_triangle_grid is set.  The
-// per-triangle data and cell boxes are rebuilt every frame (k_tri_box).
+// per-triangle data and cell boxes are rebuilt every frame (k_vein_prep).
 int build_triangle_grid(bcs_sim* s)
 {
     const bool dyn = s->phys.dynamic_triangle_grid != 0;
     if (dyn || !s->tg_built) launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
     RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, nullptr, s->flags, s->stream));
     launch_tri_nmask(s->tg, s->t_nmask, s->stream);
+    launch_tri_records(s->tg, s->tidx, s->trun, s->trec, s->stream);
     s->tg_built = true;
     return BCS_OK;
 }
@@ -244,7 +246,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         HIPCHK(hipEventRecord(s->ev_fork, st));
         HIPCHK(hipStreamWaitEvent(s->side, s->ev_fork, 0));
         if (lead_vein) launch_vein(Va, 7, s->side);
-        launch_vein_prep(Va, s->tg, s->tidx, s->tdata, s->t_aabb, s->side);
+        launch_vein_prep(Va, s->ntri, s->trun, s->trec, s->tdata, s->t_aabb, s->side);
         if (s->ray_fused()) {
             launch_raycand_ray(A, s->nchunks, s->side);
         } else {
@@ -258,7 +260,7 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         if (lead_vein) launch_vein(Va, 7, st);
         RC(enqueue_particle_grid(s, timer));
         tb(kGridKernels + 0);
-        launch_vein_prep(Va, s->tg, s->tidx, s->tdata, s->t_aabb, st);
+        launch_vein_prep(Va, s->ntri, s->trun, s->trec, s->tdata, s->t_aabb, st);
         te(kGridKernels + 0);
         // fused: the one launch is timed in the k_raycand slot (named
         // k_raycand_ray), the k_ray slot times no launch
@@ -522,6 +524,8 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(grid_alloc(s->tg, tgeom, s->ntri, false, true, false, false, s->stream));
     RC(s->dalloc(&s->t_nmask, (size_t)tgeom.cell_count));
     RC(s->dalloc(&s->t_aabb, 2 * (size_t)tgeom.cell_count));
+    RC(s->dalloc(&s->trun, (size_t)std::max(s->ntri, 1)));
+    RC(s->dalloc(&s->trec, (size_t)std::max(s->ntri, 1)));
     RC(build_triangle_grid(s.get()));
     HIPCHK(hipStreamSynchronize(s->stream));
     *out = s.release();
@@ -675,7 +679,7 @@ extern "C" int bcs_sim_calculate_next_frame(bcs_sim* s)
     }
     const StepArgs A = s->step_args();
     const VeinArgs Va = s->vein_args();
-    launch_vein_prep(Va, s->tg, s->tidx, s->tdata, s->t_aabb, s->stream);
+    launch_vein_prep(Va, s->ntri, s->trun, s->trec, s->tdata, s->t_aabb, s->stream);
     launch_raycand(A, s->nchunks, s->stream);
     launch_ray(A, s->stream);
     launch_cells(A, s->nchunks, false, s->stream);

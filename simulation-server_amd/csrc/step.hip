@@ -108,7 +108,7 @@ __device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVe
 
 // realCollisionDetection, src/simulation/vein_collisions.cu:11-45, with
 // edge1 = v1 - v0 and edge2 = v2 - v0 precomputed (the identical float
-// subtractions, done once per frame by k_tri_box).  Before the correctly
+// subtractions, done once per frame by k_vein_prep).  Before the correctly
 // rounded 1/a, two exact rejections of u = RN(RN(1/a) * d), d = dot(s, h):
 //   sign(d) != sign(a), d != 0 and |d| >= |a| 2^-100  =>  u < 0 (no underflow)
 //   same signs and |d| > 1.001 |a|                   =>  u > 1 (rounding < 2^-22)
@@ -1094,13 +1094,9 @@ __global__ void __launch_bounds__(256) k_tri_centers(const float* __restrict__ X
 // reduces the cell's box (exact vertex bounds), stored under the cell key.
 // Non-finite coordinates widen the box to everything.
 struct TriBoxArgs {
-    const uint32_t* runs;
-    const uint32_t* ctrl;
-    const uint2* table;
-    const uint32_t* keys_sorted;
-    const int32_t* perm;
+    const uint4* trun;  // per occupied cell r: {run start, cell key, count, 1}; w = 0 ends the list
+    const uint4* trec;  // per sorted slot u: {i0, i1, i2, triangle}
     int32_t ntri;
-    const uint32_t* idx;
     const float* X;
     int32_t nv;
     float4* tdata;
@@ -1108,15 +1104,13 @@ struct TriBoxArgs {
 };
 
 // Blocks [0, nblocks) of 256 threads share the occupied triangle cells, one
-// wave per cell at a time.
+// wave per cell at a time: run record, then the sorted triangles' records,
+// then their vertices (three dependent loads).
 __device__ __forceinline__ void tri_box_blocks(const TriBoxArgs& B, uint32_t block, uint32_t nblocks)
 {
-    const uint32_t* __restrict__ runs = B.runs;
-    const uint2* __restrict__ table = B.table;
-    const uint32_t* __restrict__ keys_sorted = B.keys_sorted;
-    const int32_t* __restrict__ perm = B.perm;
+    const uint4* __restrict__ trun = B.trun;
+    const uint4* __restrict__ trec = B.trec;
     const int32_t ntri = B.ntri;
-    const uint32_t* __restrict__ idx = B.idx;
     const float* __restrict__ X = B.X;
     const int32_t nv = B.nv;
     float4* __restrict__ tdata = B.tdata;
@@ -1124,18 +1118,16 @@ __device__ __forceinline__ void tri_box_blocks(const TriBoxArgs& B, uint32_t blo
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (block * 256u + threadIdx.x) >> 6;
     const uint32_t nwaves = (nblocks * 256u) >> 6;
-    const uint32_t* __restrict__ ctrl = B.ctrl;
-    const uint32_t nr = ctrl[2];
-    for (uint32_t r = wave; r < nr; r += nwaves) {
-        const uint32_t s0 = runs[r];
-        const uint32_t key = keys_sorted[s0];
-        const uint32_t cnt = table[key].y;
+    for (uint32_t r = wave; r < (uint32_t)ntri; r += nwaves) {
+        const uint4 ru = trun[r];
+        if (ru.w == 0u) break;  // wave-uniform: the occupied cells are a prefix
+        const uint32_t s0 = ru.x, key = ru.y, cnt = ru.z;
         float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
         for (uint32_t t = lane; t < cnt && s0 + t < (uint32_t)ntri; t += 64) {
             const uint32_t u = s0 + t;
-            const int32_t tri = perm[u];
-            const f3 v0 = load3(X, nv, idx[3 * tri]), v1 = load3(X, nv, idx[3 * tri + 1]),
-                     v2 = load3(X, nv, idx[3 * tri + 2]);
+            const uint4 tr = trec[u];
+            const int32_t tri = (int32_t)tr.w;
+            const f3 v0 = load3(X, nv, tr.x), v1 = load3(X, nv, tr.y), v2 = load3(X, nv, tr.z);
             const f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
             tdata[3 * (size_t)u] = make_float4(v0.x, v0.y, v0.z, __int_as_float(tri));
             tdata[3 * (size_t)u + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
@@ -1171,7 +1163,29 @@ __device__ __forceinline__ void tri_box_blocks(const TriBoxArgs& B, uint32_t blo
     }
 }
 
-__global__ void __launch_bounds__(256) k_tri_box(TriBoxArgs B) { tri_box_blocks(B, blockIdx.x, gridDim.x); }
+// Run and sorted-triangle records of the triangle grid for tri_box_blocks,
+// rebuilt with the grid (one thread per sorted slot u < ntri): trun[u] is the
+// record of occupied cell u, or zero past the last one.
+__global__ void __launch_bounds__(256) k_tri_records(const uint32_t* __restrict__ runs, const uint32_t* __restrict__ ctrl,
+                                                     const uint2* __restrict__ table,
+                                                     const uint32_t* __restrict__ keys_sorted,
+                                                     const int32_t* __restrict__ perm, int32_t ntri,
+                                                     const uint32_t* __restrict__ idx, uint4* __restrict__ trun,
+                                                     uint4* __restrict__ trec)
+{
+    const int32_t u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= ntri) return;
+    const uint32_t nr = ctrl[2];
+    uint4 ru = make_uint4(0u, 0u, 0u, 0u);
+    if ((uint32_t)u < nr) {
+        const uint32_t s0 = runs[u];
+        const uint32_t key = keys_sorted[s0];
+        ru = make_uint4(s0, key, table[key].y, 1u);
+    }
+    trun[u] = ru;
+    const int32_t tri = perm[u];
+    trec[u] = make_uint4(idx[3 * tri], idx[3 * tri + 1], idx[3 * tri + 2], (uint32_t)tri);
+}
 
 // Vein springs and triangle data in one launch (both need only the updated
 // vein): the first nsb blocks take the vertices, the rest the triangle cells.
@@ -1474,26 +1488,26 @@ void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t
     if (A.tt.np > 0)
         hipLaunchKernelGGL(k_contacts, dim3((A.tt.np + 255) / 256), dim3(256), 0, st, A, pass, counts, offs, pairs);
 }
-static TriBoxArgs tri_box_args(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv,
+static TriBoxArgs tri_box_args(int32_t ntri, const uint4* trun, const uint4* trec, const float* X, int32_t nv,
                                float4* tdata, float4* aabb)
 {
-    return TriBoxArgs{tg.runs, tg.ctrl, tg.table, tg.keys_sorted, tg.perm, tg.n, idx, X, nv, tdata, aabb};
+    return TriBoxArgs{trun, trec, ntri, X, nv, tdata, aabb};
 }
 constexpr uint32_t kTriBoxBlocks = 1024;
-void launch_tri_box(const DeviceGrid& tg, const uint32_t* idx, const float* X, int32_t nv, float4* tdata,
-                    float4* aabb, hipStream_t st)
+void launch_tri_records(const DeviceGrid& tg, const uint32_t* idx, uint4* trun, uint4* trec, hipStream_t st)
 {
     if (tg.n > 0)
-        hipLaunchKernelGGL(k_tri_box, dim3(kTriBoxBlocks), dim3(256), 0, st, tri_box_args(tg, idx, X, nv, tdata, aabb));
+        hipLaunchKernelGGL(k_tri_records, dim3((tg.n + 255) / 256), dim3(256), 0, st, tg.runs, tg.ctrl, tg.table,
+                           tg.keys_sorted, tg.perm, tg.n, idx, trun, trec);
 }
-void launch_vein_prep(const VeinArgs& V, const DeviceGrid& tg, const uint32_t* idx, float4* tdata, float4* aabb,
-                      hipStream_t st)
+void launch_vein_prep(const VeinArgs& V, int32_t ntri, const uint4* trun, const uint4* trec, float4* tdata,
+                      float4* aabb, hipStream_t st)
 {
     const uint32_t nsb = (uint32_t)((std::max(V.nv, 0) + 255) / 256);
-    const uint32_t ntb = tg.n > 0 ? kTriBoxBlocks : 0u;
+    const uint32_t ntb = ntri > 0 ? kTriBoxBlocks : 0u;
     if (nsb + ntb > 0)
-        hipLaunchKernelGGL(k_vein_prep, dim3(nsb + ntb), dim3(256), 0, st, V, tri_box_args(tg, idx, V.X, V.nv, tdata, aabb),
-                           nsb);
+        hipLaunchKernelGGL(k_vein_prep, dim3(nsb + ntb), dim3(256), 0, st, V,
+                           tri_box_args(ntri, trun, trec, V.X, V.nv, tdata, aabb), nsb);
 }
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st)
 {
