@@ -638,13 +638,45 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
     const int32_t i = act ? pcl.x : 0, cell = act ? pcl.y : 0;
     uint32_t key_of = 0;
     int32_t slot_i = 0;
+    // occupied neighbour cells of the contact search (rank r = (dx+1)*9 +
+    // (dy+1)*3 + (dz+1)), from the occupancy map: its words are loaded before
+    // the barrier, so that they arrive during the centre and the springs
+    const GridGeom& g = A.pg;
+    const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
+    uint32_t mask = 0u;
     if (act) {
-        sx[l] = A.pos[i]; sy[l] = A.pos[np + i]; sz[l] = A.pos[2 * np + i];
+        const float px = A.pos[i], py = A.pos[np + i], pz = A.pos[2 * np + i];
+        sx[l] = px; sy[l] = py; sz[l] = pz;
         svx[l] = A.vel[i]; svy[l] = A.vel[np + i]; svz[l] = A.vel[2 * np + i];
         sfx[l] = A.frc[i]; sfy[l] = A.frc[np + i]; sfz[l] = A.frc[2 * np + i];
         key_of = A.p_key_of[i];
         if (tt.nt > 1) slot_i = A.p_slot[i];
+        // the cell the grid build gave i (gridCellIds of its slot); the face
+        // clipping uses the current position, as the reference does
+        const int32_t xId = sat_i32((px - g.minv[0]) / g.cell[0]);
+        const int32_t yId = sat_i32((py - g.minv[1]) / g.cell[1]);
+        const int32_t zId = sat_i32((pz - g.minv[2]) / g.cell[2]);
+        NbrRange nr;
+        axis_range_i(xId, g.cnt[0], nr.xl, nr.xh);
+        axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
+        axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
+        mask = neighbour_mask(A.p_map, g, key_of, nr, cX, cXY, true);
     }
+    // table entries {first slot, end slot} of the next kCellBatch occupied
+    // cells in visit order (empty: {0, 0}), taken off mask
+    auto load_cells = [&](uint2 (&ent)[kCellBatch]) {
+#pragma unroll
+        for (int k = 0; k < kCellBatch; ++k) {
+            ent[k] = make_uint2(0u, 0u);
+            if (mask) {
+                const int r = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                const int dx = r / 9 - 1, c = r % 9;
+                const uint2 e = A.p_table[(int64_t)key_of + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
+                ent[k] = make_uint2(e.x, min(e.x + e.y, (uint32_t)np));
+            }
+        }
+    };
     // spring lists of this type's model vertices (one type per block), from
     // LDS when they fit
     const int m0 = tt.mstart[T];
@@ -678,6 +710,7 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
             A.centers[tt.nc + cell] = cen.y;
             A.centers[2 * (int64_t)tt.nc + cell] = cen.z;
         }
+        bool more = mask != 0u;
         // ---- springs + environment (F of neighbours from the LDS snapshot)
         const int ma = m0 + a;
         f3 newF = mk(0.f, 0.f, 0.f);
@@ -724,41 +757,17 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
         // (from the occupancy map) are looked up in the cell table
         // (calculateParticleCollisions, src/simulation/particle_collisions.cuh:104-269).
         {
-            const GridGeom& g = A.pg;
             int Tl = T;
             if (tt.nt > 1) Tl = type_of_index(tt, slot_i);
             const ModelIx mi = model_ix(tt, Tl);
             const float r1 = radius(model_index(mi, i, fl));
-            // the cell the grid build gave i (gridCellIds of its slot); the face
-            // clipping below uses the current position, as the reference does
-            const uint32_t cellId = key_of;
-            const int32_t xId = sat_i32((x1.x - g.minv[0]) / g.cell[0]);
-            const int32_t yId = sat_i32((x1.y - g.minv[1]) / g.cell[1]);
-            const int32_t zId = sat_i32((x1.z - g.minv[2]) / g.cell[2]);
-            NbrRange nr;
-            axis_range_i(xId, g.cnt[0], nr.xl, nr.xh);
-            axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
-            axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
-            const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-            uint32_t mask = neighbour_mask(A.p_map, g, cellId, nr, cX, cXY, true);
-            // The occupied cells in visit order (rank r = (dx+1)*9 + (dy+1)*3 +
-            // (dz+1)), kCellBatch at a time: their table entries are loaded
-            // together, then each cell's run of sorted-slot records is read 4
-            // slots per load batch.  Contributions are added in visit order,
-            // slot by slot.
-            while (mask) {
-                uint2 ent[kCellBatch];  // {first slot, end slot} per occupied cell (empty: {0, 0})
-#pragma unroll
-                for (int k = 0; k < kCellBatch; ++k) {
-                    ent[k] = make_uint2(0u, 0u);
-                    if (mask) {
-                        const int r = __builtin_ctz(mask);
-                        mask &= mask - 1u;
-                        const int dx = r / 9 - 1, c = r % 9;
-                        const uint2 e = A.p_table[(int64_t)cellId + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
-                        ent[k] = make_uint2(e.x, min(e.x + e.y, (uint32_t)np));
-                    }
-                }
+            // The occupied cells in visit order, kCellBatch at a time: their
+            // table entries are loaded together, then each cell's run of
+            // sorted-slot records is read 4 slots per load batch.
+            // Contributions are added in visit order, slot by slot.
+            while (more) {
+                uint2 ent[kCellBatch];
+                load_cells(ent);
                 // One stream of candidate slots over these cells in visit order
                 // (stream position p lies in cell k when cum[k] <= p < cum[k+1]);
                 // each load batch takes the next kContactBatch of them, across
@@ -803,6 +812,7 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
                         }
                     }
                 }
+                more = mask != 0u;
             }
         }
 
