@@ -255,6 +255,26 @@ def test_ray_modes_match_oracle(bcs, orc, mode):
             assert sim.error_flags() & bcs.FLAG_RAY_SERIAL
 
 
+def test_ray_mode_switch_mid_run(bcs, orc):
+    """Switching the ray mode between step calls drops the captured frame
+    graphs; the run stays bit-identical to the oracle.  An unknown mode is
+    rejected and leaves the simulation usable."""
+    sc = _scene(bcs, "mixed")
+    sim, o = _pair(bcs, orc, sc)
+    sim.step(20)
+    sim.set_ray_mode(bcs.RAY_FUSED)
+    sim.step(20)
+    sim.set_ray_mode(bcs.RAY_AUTO)
+    sim.step(5)
+    with pytest.raises(bcs.BcsError):
+        sim.set_ray_mode(7)
+    sim.step(3)
+    o.step(48)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "ray mode switch " + n)
+    assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "ray mode switch vein")
+
+
 def test_fused_40_steps_mixed_types(bcs, orc):
     """Two cell types (discs and icosahedra): 40 fused frames, across two
     refreshes of the spatial processing order of the cells."""
