@@ -12,8 +12,8 @@ region.  `value` is particle-steps/s.
 
 Roofline (dominant kernel, HBM-bound): algorithmic bytes per launch / its
 average duration measured with HIP events on the library's stream.  The
-algorithmic bytes are stated in DESIGN.md; `traffic` comes from the committed
-rocprofv3 PMC summary when present.
+algorithmic bytes per launch of each kernel are listed in kernel_bytes();
+`traffic` comes from the committed rocprofv3 PMC summary when present.
 
 cpu_baseline: the CPU oracle (the C restatement of the reference frame,
 OpenMP) on a bounded sample of the same workload, on this host's cores.
