@@ -32,7 +32,7 @@ def test_no_device_is_reported_not_fatal(bcs):
     n = C.c_int32(-1)
     assert lib.bcs_device_count(C.byref(n)) == bcs.BCS_OK
     if n.value > 0:
-        return  # a GPU box: nothing to check here
+        return  # a GPU is present: nothing to check here
     sc = bcs.Scene(0, 0.05, seed=3)
     types = bcs.cell_type_array(sc.types)
     meshc = sc.mesh.c()
