@@ -721,12 +721,25 @@ static void drop_graph(bcs_sim* s)
 // precedes the branches and needs the updated vein).
 static bool defer_vein(const bcs_sim* s) { return !s->phys.dynamic_triangle_grid; }
 
+// Refresh the spatial processing order of the cells from the last frame's
+// centres, and the chunk lanes' particles with it (performance only: results
+// do not depend on the order).
+static int enqueue_order(bcs_sim* s)
+{
+    RC(launch_order(s->tt, s->pg.g, s->centers, s->okey, s->ocount, s->ostart, s->ostatus, s->otiles, s->octrl,
+                    s->corder, s->flags, s->stream));
+    launch_chunk_map(s->step_args(), s->chunk_pi, s->stream);
+    return BCS_OK;
+}
+
 // nframes consecutive frames in one graph (the same kernels as nframes
-// launches of the one-frame graph; fewer graph launches per step call).
-static int capture_frame(bcs_sim* s, bool lead_vein, hipGraph_t* graph, hipGraphExec_t* exec, int nframes = 1)
+// launches of the one-frame graph; fewer graph launches per step call),
+// preceded by the order refresh when with_order is set.
+static int capture_frame(bcs_sim* s, bool lead_vein, hipGraph_t* graph, hipGraphExec_t* exec, int nframes = 1,
+                         bool with_order = false)
 {
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int rc = BCS_OK;
+    int rc = with_order ? enqueue_order(s) : BCS_OK;
     for (int f = 0; f < nframes && !rc; ++f) rc = enqueue_fused_frame(s, nullptr, lead_vein, !defer_vein(s));
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
@@ -745,7 +758,7 @@ static int ensure_graph(bcs_sim* s)
     if (!s->graph_exec) RC(capture_frame(s, false, &s->graph, &s->graph_exec));
     if (!s->graph_exec_b && defer_vein(s)) RC(capture_frame(s, true, &s->graph_b, &s->graph_exec_b));
     if (!s->graph_exec_m && defer_vein(s))
-        RC(capture_frame(s, true, &s->graph_m, &s->graph_exec_m, kGraphFrames));
+        RC(capture_frame(s, true, &s->graph_m, &s->graph_exec_m, kGraphFrames, true));
     return BCS_OK;
 }
 
@@ -756,24 +769,21 @@ extern "C" int bcs_sim_step(bcs_sim* s, int32_t n_steps)
     if (n_steps == 0) return BCS_OK;
     if (s->use_graph) RC(ensure_graph(s));
     for (int32_t k = 0; k < n_steps;) {
-        // Refresh the spatial processing order of the cells from the last
-        // frame's centres (performance only: results do not depend on it).
-        if (s->frames_since_order >= kReorderPeriod) {
-            RC(launch_order(s->tt, s->pg.g, s->centers, s->okey, s->ocount, s->ostart, s->ostatus, s->otiles,
-                            s->octrl, s->corder, s->flags, s->stream));
-            launch_chunk_map(s->step_args(), s->chunk_pi, s->stream);
-            s->frames_since_order = 0;
-        }
         // frames after the first start with the previous frame's vein update
         const bool lead = defer_vein(s) && k > 0;
         int32_t nf = 1;
         if (s->use_graph && lead && s->graph_exec_m && n_steps - k >= kGraphFrames) {
+            // kGraphFrames frames, after the order refresh the graph starts with
             HIPCHK(hipGraphLaunch(s->graph_exec_m, s->stream));
             nf = kGraphFrames;
-        } else if (s->use_graph) {
-            HIPCHK(hipGraphLaunch(lead ? s->graph_exec_b : s->graph_exec, s->stream));
+            s->frames_since_order = 0;
         } else {
-            RC(enqueue_fused_frame(s, nullptr, lead, !defer_vein(s)));
+            if (s->frames_since_order >= kReorderPeriod) {
+                RC(enqueue_order(s));
+                s->frames_since_order = 0;
+            }
+            if (s->use_graph) HIPCHK(hipGraphLaunch(lead ? s->graph_exec_b : s->graph_exec, s->stream));
+            else RC(enqueue_fused_frame(s, nullptr, lead, !defer_vein(s)));
         }
         k += nf;
         s->frames_since_order += nf;
