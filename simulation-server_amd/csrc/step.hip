@@ -1014,8 +1014,21 @@ __device__ __forceinline__ void vein_springs_at(const VeinArgs& A, int32_t id)
             const float L = A.nbl[s * nv + id];
             const f3 NP = load3(A.X, nv, nb);
             const f3 NV = load3(A.V, nv, nb);
-            const float sf = (len(sub(P, NP)) - L) * A.vk + dot(normalize(sub(P, NP), fl), sub(Vv, NV)) * A.vd;
-            force = add(force, smul(sf, normalize(sub(NP, P), fl)));
+            // len(P - NP), normalize(P - NP) and normalize(NP - P) from one
+            // square root and three divisions: NP - P = -(P - NP) exactly and
+            // negation commutes with every rounding here; a NaN quotient gives
+            // (0,0,0) for both, as normalize() does.
+            const f3 dP = sub(P, NP);
+            const float sl = sqrtf(dot(dP, dP));
+            f3 nrm = sdiv(dP, sl);
+            f3 back = smul(-1.0f, nrm);
+            if (__builtin_isnan(nrm.x) || __builtin_isnan(nrm.y) || __builtin_isnan(nrm.z)) {
+                fl |= BCS_FLAG_NAN_NORMALIZE;
+                nrm = mk(0.f, 0.f, 0.f);
+                back = nrm;
+            }
+            const float sf = (sl - L) * A.vk + dot(nrm, sub(Vv, NV)) * A.vd;
+            force = add(force, smul(sf, back));
         }
     }
     const f3 F0 = load3(A.Fv, nv, id);
