@@ -170,17 +170,16 @@ __device__ __forceinline__ bool ray_box(f3 o, f3 inv, f3 dir, float4 lo, float4 
     float tmin = 0.0f, tmax = 3.0e38f;
     const float os[3] = {o.x, o.y, o.z}, is[3] = {inv.x, inv.y, inv.z}, ds[3] = {dir.x, dir.y, dir.z};
     const float ls[3] = {lo.x - m, lo.y - m, lo.z - m}, hs[3] = {hi.x + m, hi.y + m, hi.z + m};
+    bool out = false;  // branch-free: selects instead of a branch per axis
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        if (ds[d] == 0.0f) {
-            if (os[d] < ls[d] || os[d] > hs[d]) return false;
-        } else {
-            float t1 = (ls[d] - os[d]) * is[d], t2 = (hs[d] - os[d]) * is[d];
-            tmin = fmaxf(tmin, fminf(t1, t2));
-            tmax = fminf(tmax, fmaxf(t1, t2));
-        }
+        const bool flat = ds[d] == 0.0f;  // ray parallel to the slab: inside it or a miss
+        out = out || (flat && (os[d] < ls[d] || os[d] > hs[d]));
+        const float t1 = (ls[d] - os[d]) * is[d], t2 = (hs[d] - os[d]) * is[d];
+        tmin = flat ? tmin : fmaxf(tmin, fminf(t1, t2));
+        tmax = flat ? tmax : fminf(tmax, fmaxf(t1, t2));
     }
-    return tmax >= tmin;
+    return !out && tmax >= tmin;
 }
 
 // Vein-end condition of one particle (src/simulation/vein_end.cu:57-90,
@@ -489,11 +488,11 @@ __device__ __forceinline__ void raycand_block(const StepArgs& A)
                         }
                     }
 #pragma unroll
-                    for (int k = 0; k < (int)kBoxBatch; ++k)
-                        if (rr[k] >= 0 && ray_box(x1, inv, dir, lo[k], hi[k], bm)) {
-                            pmask |= 1u << rr[k];
-                            nitems += (min(__float_as_uint(hi[k].w), (uint32_t)A.ntri) + kItemTris - 1) / kItemTris;
-                        }
+                    for (int k = 0; k < (int)kBoxBatch; ++k) {
+                        const bool pass = rr[k] >= 0 && ray_box(x1, inv, dir, lo[k], hi[k], bm);
+                        pmask |= pass ? 1u << (rr[k] & 31) : 0u;
+                        nitems += pass ? (min(__float_as_uint(hi[k].w), (uint32_t)A.ntri) + kItemTris - 1) / kItemTris : 0u;
+                    }
                 }
             }
         }
