@@ -352,6 +352,21 @@ def test_c1_full_one_frame(bcs, orc):
     assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "c1 vein")
 
 
+def test_c2_full_two_frames(bcs, orc):
+    """Full config-2 sizes (6.6M particles of two cell types, ~1.8M triangles,
+    ~26k cell blocks): two frames.  At this size the default ray mode runs
+    candidates and ray test as one launch, and the particle grid uses small
+    buckets with crowded big ones."""
+    sc = bcs.Scene(2, 1.0, seed=0)
+    sim, o = _pair(bcs, orc, sc)
+    sim.step(2)
+    o.step(2)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, "c2 " + n)
+    assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "c2 vein")
+    assert "k_raycand_ray" in sim.profile_kernels(1), "config 2 should run the fused ray launch"
+
+
 def test_displaced_vein_keeps_parity(bcs, orc):
     """Vein moved away from its creation position (the frozen triangle grid
     keeps its creation-time cells): the per-frame triangle-cell boxes follow
