@@ -199,11 +199,14 @@ def test_fused_100_steps_c0(bcs, orc):
     assert sim.get_info().step_index == 100
 
 
-def test_fused_1000_steps_c0_bit_exact(bcs, orc):
+@pytest.mark.parametrize("mode", ["auto", "fused"])
+def test_fused_1000_steps_c0_bit_exact(bcs, orc, mode):
     """The whole config-0 run (1,000 frames, teleports included) stays bit
-    for bit on the oracle's trajectory."""
+    for bit on the oracle's trajectory, with either ray launch."""
     sc = _scene(bcs, "c0")
     sim, o = _pair(bcs, orc, sc)
+    if mode == "fused":
+        sim.set_ray_mode(bcs.RAY_FUSED)
     for chunk in range(4):
         sim.step(250)
         o.step(250)
@@ -388,14 +391,19 @@ def test_displaced_vein_keeps_parity(bcs, orc):
     assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "displaced vein X")
 
 
-def test_dynamic_triangle_grid_parity(bcs, orc):
+@pytest.mark.parametrize("mode", ["auto", "fused"])
+def test_dynamic_triangle_grid_parity(bcs, orc, mode):
+    """Triangle grid and its records rebuilt from the vein every frame (the
+    vein update then ends each frame), with either ray launch."""
     sc = _scene(bcs, "c0")
     ph = sc.physics
     ph.dynamic_triangle_grid = 1
     sim = bcs.Simulation.from_scene(sc, physics=ph)
+    if mode == "fused":
+        sim.set_ray_mode(bcs.RAY_FUSED)
     o = orc.Oracle(ph, sc.types, sc.mesh)
     o.set_particles(sc.positions, sc.velocities, sc.forces)
     sim.step(20)
     o.step(20)
     for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
-        assert_bits_equal(x, y, "dynamic triangle grid " + n)
+        assert_bits_equal(x, y, f"dynamic triangle grid ({mode}) " + n)
