@@ -64,6 +64,8 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
       k_occ        sorted key read 4, map-word slot written 4; count read 8
                    and 2-bit code written (<= 2 bytes) per occupied cell
                                                                    8 / particle + 10 / cell
+                   (with buckets of 2^8 cells or more k_rank writes the map
+                   instead, and these bytes belong to it)
       k_raycand    x, v read 24, first-hit key written 4; work items written
                    16 each (triangle-cell boxes counted in k_vein_prep)
                                                                    28 / particle + 16 / item

@@ -260,6 +260,14 @@ int bcs_sim_use_graph(bcs_sim* sim, int32_t enabled);
 #define BCS_RAY_FUSED 2
 int bcs_sim_set_ray_mode(bcs_sim* sim, int32_t mode);
 
+/* Bucket size of the particle sort: 2^shift grid cells per bucket (0..24).
+ * bcs_sim_set_particles chooses it from the particles' positions; this call
+ * overrides that choice until the next bcs_sim_set_particles.  Buckets of 2^8
+ * cells or more have the sort write the occupancy map itself; smaller ones
+ * build it in a pass of its own.  Results are identical for every size.  No
+ * counterpart in the reference (thrust's sort has no such parameter). */
+int bcs_sim_set_bucket_shift(bcs_sim* sim, int32_t shift);
+
 int bcs_sim_synchronize(bcs_sim* sim);
 
 /* Time n fused steps with HIP events on the simulation's stream (ms total). */

@@ -218,6 +218,7 @@ def _sig(lib):
     lib.bcs_sim_step.argtypes = [vp, C.c_int32]
     lib.bcs_sim_use_graph.argtypes = [vp, C.c_int32]
     lib.bcs_sim_set_ray_mode.argtypes = [vp, C.c_int32]
+    lib.bcs_sim_set_bucket_shift.argtypes = [vp, C.c_int32]
     lib.bcs_sim_synchronize.argtypes = [vp]
     lib.bcs_sim_time_steps.argtypes = [vp, C.c_int32, f32p]
     lib.bcs_sim_profile_kernels.argtypes = [vp, C.c_int32, f32p, C.c_int32, i32p, C.c_char_p, C.c_int32]
@@ -484,6 +485,11 @@ class Simulation:
         """RAY_AUTO, RAY_SPLIT or RAY_FUSED: how frames run the vein ray test
         (identical results)."""
         check(self.lib.bcs_sim_set_ray_mode(self.h, int(mode)), "set_ray_mode")
+
+    def set_bucket_shift(self, shift: int):
+        """2^shift grid cells per bucket of the particle sort (identical
+        results; the automatic choice returns with the next set_particles)."""
+        check(self.lib.bcs_sim_set_bucket_shift(self.h, int(shift)), "set_bucket_shift")
 
     def synchronize(self):
         check(self.lib.bcs_sim_synchronize(self.h), "synchronize")

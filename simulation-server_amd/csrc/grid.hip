@@ -15,8 +15,11 @@
 //              buckets larger than the threshold are then shared out over the
 //              blocks: bitonic sort of the (key, index) pairs in LDS (up to
 //              kSortLds), else ranking by counting over LDS tiles
-//   k_occ      (optional) the 2-bit brick occupancy map from the sorted keys:
-//              the first slot of every 8-cell x-group writes the group's word
+// The optional 2-bit brick occupancy map (one word per 8-cell x-group):
+//   buckets of 2^8 cells or more: written by k_rank (the first run of every
+//              group writes the group's word; a group straddling two buckets
+//              ORs its codes in, rare at this bucket size)
+//   smaller buckets (many straddling groups): k_occ from the sorted keys
 // Cell table: dense {start, count} per cell id (8 bytes, one load per lookup),
 // written for occupied cells only.  With clear_table (the triangle grid) k_key
 // first zeroes the entries of the previous build's keys, so every entry is
@@ -256,7 +259,71 @@ struct RankOut {
     uint32_t* nruns;
     const float4* brec;    // per-item records (nullable) ...
     float4* pvs;           // ... copied to their sorted slots
+    BrickMap bm;           // occupancy map (bm.words nullable) ...
+    uint32_t* gpos;        // ... and the word each sorted slot wrote (kNoWord: none)
+    int shift;             // bucket = key >> shift
 };
+
+// The 8-cell x-group of `key` (cells x&~7 .. x|7 of its row): keys [gs, ge)
+// and the map word holding their 2-bit codes.
+struct OccGroup {
+    uint32_t gs, ge, word, bit;  // bit: position of key's code in the word
+};
+__device__ __forceinline__ OccGroup occ_group(const BrickMap& bm, uint32_t key)
+{
+    uint32_t x, y, z;
+    key_coords(bm, key, x, y, z);
+    OccGroup q;
+    q.gs = key - (x & 7u);
+    q.ge = q.gs + min(8u, (uint32_t)bm.cnt[0] - (x & ~7u));
+    q.word = brick_word(bm, x, y, z);
+    q.bit = 2u * (x & 7u);
+    return q;
+}
+// Record key k2 (one object) in the group masks: seen1 = keys present,
+// seen2 = keys with two or more objects.
+__device__ __forceinline__ void occ_note(uint32_t k2, uint32_t gs, uint32_t ge, uint32_t& seen1, uint32_t& seen2)
+{
+    if (k2 >= gs && k2 < ge) {
+        const uint32_t b = 1u << (k2 - gs);
+        seen2 |= seen1 & b;
+        seen1 |= b;
+    }
+}
+// The group's word: code 2 for keys of two or more objects, 1 for one, 0 else.
+__device__ __forceinline__ uint32_t occ_word(uint32_t seen1, uint32_t seen2)
+{
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w |= (((seen2 >> k) & 1u) ? 2u : ((seen1 >> k) & 1u)) << (2 * k);
+    return w;
+}
+
+// Occupancy map entry of sorted slot pos.  A run start whose group lies in
+// one bucket is written whole by the group's first run (owner, with the
+// group's masks); a group that straddles two buckets, or one whose masks are
+// not known (known = false), gets each run's code by an atomic OR.  Every
+// slot records the word it wrote (or kNoWord), so that the next build clears
+// exactly these words.
+__device__ __forceinline__ void map_write(const RankOut& o, uint32_t key, uint32_t pos, bool first, uint32_t count,
+                                          bool known, bool owner, uint32_t seen1, uint32_t seen2)
+{
+    if (!o.gpos) return;
+    uint32_t w = kNoWord;
+    if (first) {
+        const OccGroup q = occ_group(o.bm, key);
+        const bool straddle = (q.gs >> o.shift) != ((q.ge - 1u) >> o.shift);
+        if (straddle || !known) {
+            uint32_t* cont = reinterpret_cast<uint32_t*>(o.bm.words) + (q.word >> 1);
+            atomicOr(cont, (count > 1u ? 2u : 1u) << (q.bit + 16u * (q.word & 1u)));
+            w = q.word;
+        } else if (owner) {
+            o.bm.words[q.word] = (uint16_t)occ_word(seen1, seen2);
+            w = q.word;
+        }
+    }
+    o.gpos[pos] = w;
+}
 
 // Object idx goes to sorted slot pos; (ra, rb) is its record (when o.pvs).
 __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32_t idx, uint32_t pos, bool first,
@@ -340,6 +407,32 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint16_t* sq, uint32_t
                     while (t + count < m && (uint32_t)(sv[t + count] >> 32) == key) ++count;
                 }
                 rank_write(o, key, (int32_t)(uint32_t)v, s0 + t, first, count, s0 + sq[t]);
+                if (first) sq[t] = (uint16_t)count;  // run length at each run start (sq[t] was read above)
+            }
+            if (o.gpos) {
+                // the group's first run builds its word, run by run, from the
+                // sorted keys and the run lengths
+                __syncthreads();
+                for (uint32_t t = tid; t < m; t += blockDim.x) {
+                    const uint32_t key = (uint32_t)(sv[t] >> 32);
+                    const bool first = t == 0 || (uint32_t)(sv[t - 1] >> 32) != key;
+                    bool owner = false;
+                    uint32_t seen1 = 0, seen2 = 0, count = 0;
+                    if (first) {
+                        count = sq[t];
+                        const OccGroup q = occ_group(o.bm, key);
+                        owner = t == 0 || (uint32_t)(sv[t - 1] >> 32) < q.gs;
+                        for (uint32_t u = t; owner && u < m;) {
+                            const uint32_t k2 = (uint32_t)(sv[u] >> 32);
+                            if (k2 >= q.ge) break;
+                            const uint32_t c = sq[u];
+                            seen1 |= 1u << (k2 - q.gs);
+                            seen2 |= c > 1u ? 1u << (k2 - q.gs) : 0u;
+                            u += max(c, 1u);  // c >= 1 at run starts; the loop always ends
+                        }
+                    }
+                    map_write(o, key, s0 + t, first, count, true, owner, seen1, seen2);
+                }
             }
             __syncthreads();  // sv is reused by the next bucket
             continue;
@@ -376,7 +469,10 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint16_t* sq, uint32_t
                     }
                 }
             }
-            if (own) rank_write(o, key, idx, s0 + rank, first, same, q);
+            if (own) {
+                rank_write(o, key, idx, s0 + rank, first, same, q);
+                map_write(o, key, s0 + rank, first, same, false, false, 0u, 0u);
+            }
         }
         __syncthreads();
     }
@@ -443,11 +539,25 @@ __global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items, i
         if (s1 >= s0 && s1 <= (uint32_t)n && s1 - s0 <= kRankThreshold) {
             uint32_t rank = 0, same = 0;
             bool first = true;
-            if (s0 >= (uint32_t)w0 && s1 <= (uint32_t)w1)
-                count_rank([&](uint32_t u) { return win[u - w0]; }, s0, s1, key, idx, rank, same, first);
-            else  // inconsistent offsets (flagged by k_scatter): read the items directly
-                count_rank([&](uint32_t u) { return items[u]; }, s0, s1, key, idx, rank, same, first);
+            const bool in_win = s0 >= (uint32_t)w0 && s1 <= (uint32_t)w1;
+            auto get = [&](uint32_t u) { return in_win ? win[u - w0] : items[u]; };
+            // (inconsistent offsets, flagged by k_scatter, read the items directly)
+            count_rank(get, s0, s1, key, idx, rank, same, first);
             rank_write(o, key, idx, s0 + rank, first, same, ra, rb);
+            // the group's first run (no smaller key of its group in the
+            // bucket) builds the group's word from the bucket's items
+            bool owner = false;
+            uint32_t seen1 = 0, seen2 = 0;
+            if (o.gpos && first) {
+                const OccGroup g8 = occ_group(o.bm, key);
+                owner = true;
+                for (uint32_t u = s0; u < s1; ++u) {
+                    const uint32_t k2 = get(u).x;
+                    owner = owner && !(k2 >= g8.gs && k2 < key);
+                    occ_note(k2, g8.gs, g8.ge, seen1, seen2);
+                }
+            }
+            map_write(o, key, s0 + rank, first, same, true, owner, seen1, seen2);
         }
     }
     __syncthreads();  // the window's LDS is reused by the big-bucket sort
@@ -681,13 +791,15 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.arrival, G.items,
                        flags, pos, vel, stride, brec);
     te(2);
-    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr};
+    const bool rank_map = grid_map_in_rank(G);
+    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr,
+              G.bm, rank_map ? G.gpos : nullptr, G.shift};
     tb(3);
     hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items, n, G.shift, G.bstart, G.biglist, G.ctrl,
                        G.big_cap, o);
     te(3);
     tb(4);
-    if (G.gpos)
+    if (G.gpos && !rank_map)
         hipLaunchKernelGGL(k_occ, dim3((std::max(G.cap, 1) + 255) / 256), dim3(256), 0, st, G.keys_sorted, n, G.cap,
                            G.table, G.bm, G.gpos);
     te(4);

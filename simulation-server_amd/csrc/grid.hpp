@@ -60,6 +60,11 @@ void grid_free(DeviceGrid& G);
 int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
                hipStream_t st, const KernelTimer* timer = nullptr);
 constexpr int kGridKernels = 5;
+// The occupancy map is written by k_rank when buckets span at least
+// 2^kRankMapShift cells (few 8-cell groups straddle two buckets), else by
+// k_occ after it.
+constexpr int kRankMapShift = 8;
+inline bool grid_map_in_rank(const DeviceGrid& G) { return G.gpos && G.shift >= kRankMapShift; }
 
 // Bucket size from a host sample of the objects (SoA, stride apart), and the
 // switch to another bucket size (waits for the stream; buckets start empty).

@@ -801,6 +801,19 @@ extern "C" int bcs_sim_use_graph(bcs_sim* s, int32_t enabled)
     return BCS_OK;
 }
 
+extern "C" int bcs_sim_set_bucket_shift(bcs_sim* s, int32_t shift)
+{
+    if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
+    if (shift < 0 || shift > 24) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_set_bucket_shift: shift in 0..24");
+    if (shift == s->pg.shift) return BCS_OK;
+    RC(set_dev(s));
+    HIPCHK(hipStreamSynchronize(s->stream));  // the graphs hold the old bucket arrays
+    drop_graph(s);
+    RC(grid_set_shift(s->pg, shift, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
 extern "C" int bcs_sim_set_ray_mode(bcs_sim* s, int32_t mode)
 {
     if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
@@ -865,6 +878,7 @@ extern "C" int bcs_sim_profile_kernels(bcs_sim* s, int32_t n_steps, float* ms_ou
         }
     }
     if (s->ray_fused()) acc[kGridKernels + 2] = 0.0;  // no k_ray launch
+    if (grid_map_in_rank(s->pg)) acc[4] = 0.0;         // no k_occ launch (k_rank writes the map)
     for (int k = 0; k < kFusedKernels; ++k) {
         ms_out[k] = (float)(acc[k] / n_steps);
         (void)hipEventDestroy(eb[k]);

@@ -50,6 +50,7 @@ def test_null_and_invalid_arguments(bcs):
     assert lib.bcs_sim_destroy(None) in (bcs.BCS_OK, bcs.BCS_ERR_INVALID_ARGUMENT)
     assert lib.bcs_sim_use_graph(None, 0) == bcs.BCS_ERR_INVALID_ARGUMENT
     assert lib.bcs_sim_set_ray_mode(None, 0) == bcs.BCS_ERR_INVALID_ARGUMENT
+    assert lib.bcs_sim_set_bucket_shift(None, 8) == bcs.BCS_ERR_INVALID_ARGUMENT
     ms = C.c_float()
     assert lib.bcs_calibration_copy(C.c_int64(8), 1, C.byref(ms)) == bcs.BCS_ERR_INVALID_ARGUMENT
     assert lib.bcs_scene_preset(99, C.c_double(1.0), C.c_uint64(0), C.byref(C.c_void_p())) == bcs.BCS_ERR_INVALID_ARGUMENT

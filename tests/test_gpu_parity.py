@@ -44,16 +44,21 @@ def test_particle_grid_matches_oracle(bcs, orc, kind):
         assert_bits_equal(a, b, name)
 
 
+@pytest.mark.parametrize("shift", [None, 4, 8, 12])
 @pytest.mark.parametrize("squeeze", [0.25, 0.03, 0.002])
-def test_particle_grid_crowded_matches_oracle(bcs, orc, squeeze):
+def test_particle_grid_crowded_matches_oracle(bcs, orc, squeeze, shift):
     """Particles pulled together so that sort buckets hold hundreds to all
-    6,000 objects (LDS bitonic path and the tiled fallback)."""
+    6,000 objects (LDS bitonic path and the tiled fallback), with the
+    automatic bucket size and with buckets of 2^4 cells (occupancy map from
+    its own pass), 2^8 and 2^12 cells (map written by the sort)."""
     sc = _scene(bcs, "c0")
     p = sc.positions.reshape(3, -1).copy()
     c = p.mean(axis=1, keepdims=True)
     p = (c + (p - c) * np.float32(squeeze)).astype(np.float32)
     sc.positions = p.reshape(-1)
     sim, o = _pair(bcs, orc, sc)
+    if shift is not None:
+        sim.set_bucket_shift(shift)
     sim.calculate_grid(0)
     o.calculate_grid(0)
     g, r = sim.get_grid(0), o.get_grid(0)
@@ -276,6 +281,22 @@ def test_ray_mode_switch_mid_run(bcs, orc):
     for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
         assert_bits_equal(x, y, "ray mode switch " + n)
     assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "ray mode switch vein")
+
+
+@pytest.mark.parametrize("shift", [4, 8, 11])
+def test_bucket_sizes_keep_parity(bcs, orc, shift):
+    """40 graph frames with buckets of 2^4 cells (occupancy map from its own
+    pass), 2^8 and 2^11 cells (map written by the sort, groups straddling two
+    buckets ORed in): the same trajectory as the oracle."""
+    sc = _scene(bcs, "mixed")
+    sim, o = _pair(bcs, orc, sc)
+    sim.set_bucket_shift(shift)
+    sim.step(40)
+    o.step(40)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, f"buckets of 2^{shift} cells: {n}")
+    with pytest.raises(bcs.BcsError):
+        sim.set_bucket_shift(25)
 
 
 def test_fused_40_steps_mixed_types(bcs, orc):
