@@ -479,11 +479,17 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint16_t* sq, uint32_t
 }
 
 // Rank of item (key, idx) among the items get(s0 .. s1-1) of its bucket.
-template <class Get>
+// With OCC, the same pass notes the keys of the item's 8-cell group [gs, ge)
+// (seen1, seen2) and whether a smaller key of the group is present (earlier).
+template <bool OCC, class Get>
 __device__ __forceinline__ void count_rank(Get get, uint32_t s0, uint32_t s1, uint32_t key, int32_t idx,
-                                           uint32_t& rank, uint32_t& same, bool& first)
+                                           uint32_t& rank, uint32_t& same, bool& first, uint32_t gs = 0u,
+                                           uint32_t ge = 0u, uint32_t* seen1 = nullptr, uint32_t* seen2 = nullptr,
+                                           bool* earlier = nullptr)
 {
     uint32_t before = 0;  // same key, lower index
+    uint32_t m1 = 0, m2 = 0;
+    bool e = false;
 #pragma unroll 4
     for (uint32_t u = s0; u < s1; ++u) {
         const uint2 it = get(u);
@@ -491,9 +497,18 @@ __device__ __forceinline__ void count_rank(Get get, uint32_t s0, uint32_t s1, ui
         rank += it.x < key ? 1u : 0u;
         same += eq;
         before += eq & ((int32_t)it.y < idx ? 1u : 0u);
+        if (OCC) {
+            e = e || (it.x >= gs && it.x < key);
+            occ_note(it.x, gs, ge, m1, m2);
+        }
     }
     rank += before;
     first = before == 0u;
+    if (OCC) {
+        *seen1 = m1;
+        *seen2 = m2;
+        *earlier = e;
+    }
 }
 
 // In-bucket ranking: every thread first ranks its item if the bucket holds at
@@ -542,22 +557,20 @@ __global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items, i
             const bool in_win = s0 >= (uint32_t)w0 && s1 <= (uint32_t)w1;
             auto get = [&](uint32_t u) { return in_win ? win[u - w0] : items[u]; };
             // (inconsistent offsets, flagged by k_scatter, read the items directly)
-            count_rank(get, s0, s1, key, idx, rank, same, first);
-            rank_write(o, key, idx, s0 + rank, first, same, ra, rb);
-            // the group's first run (no smaller key of its group in the
-            // bucket) builds the group's word from the bucket's items
-            bool owner = false;
-            uint32_t seen1 = 0, seen2 = 0;
-            if (o.gpos && first) {
+            if (o.gpos) {
+                // the group's first run (no smaller key of its group in the
+                // bucket) builds the group's word from the bucket's items,
+                // noted in the ranking pass
                 const OccGroup g8 = occ_group(o.bm, key);
-                owner = true;
-                for (uint32_t u = s0; u < s1; ++u) {
-                    const uint32_t k2 = get(u).x;
-                    owner = owner && !(k2 >= g8.gs && k2 < key);
-                    occ_note(k2, g8.gs, g8.ge, seen1, seen2);
-                }
+                uint32_t seen1 = 0, seen2 = 0;
+                bool earlier = false;
+                count_rank<true>(get, s0, s1, key, idx, rank, same, first, g8.gs, g8.ge, &seen1, &seen2, &earlier);
+                rank_write(o, key, idx, s0 + rank, first, same, ra, rb);
+                map_write(o, key, s0 + rank, first, same, true, first && !earlier, seen1, seen2);
+            } else {
+                count_rank<false>(get, s0, s1, key, idx, rank, same, first);
+                rank_write(o, key, idx, s0 + rank, first, same, ra, rb);
             }
-            map_write(o, key, s0 + rank, first, same, true, owner, seen1, seen2);
         }
     }
     __syncthreads();  // the window's LDS is reused by the big-bucket sort
