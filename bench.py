@@ -51,7 +51,7 @@ def step_bytes(np_, n_occ, nc, nv, nt):
     return 88 * np_ + 16 * n_occ + 24 * nc + 144 * nv + 16 * nt
 
 
-def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
+def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0, map_in_rank=False):
     """Algorithmic HBM bytes of one launch of each frame kernel (every datum
     counted once, as a perfect cache would move it):
       k_key        x read 12, key written 4, previous map-word slot read 4 (the
@@ -64,8 +64,9 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
       k_occ        sorted key read 4, map-word slot written 4; count read 8
                    and 2-bit code written (<= 2 bytes) per occupied cell
                                                                    8 / particle + 10 / cell
-                   (with buckets of 2^8 cells or more k_rank writes the map
-                   instead, and these bytes belong to it)
+                   (map_in_rank: with buckets of 2^8 cells or more k_rank
+                   writes the map instead -- map-word slot 4 / particle and
+                   code 2 / cell, its keys and counts already in registers)
       k_raycand    x, v read 24, first-hit key written 4; work items written
                    16 each (triangle-cell boxes counted in k_vein_prep)
                                                                    28 / particle + 16 / item
@@ -87,7 +88,7 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0):
     table = {
         "k_key": 20 * np_,
         "k_scatter": 68 * np_,
-        "k_rank": 84 * np_ + 8 * n_occ,
+        "k_rank": 84 * np_ + 8 * n_occ + ((4 * np_ + 2 * n_occ) if map_in_rank else 0),
         "k_occ": 8 * np_ + 10 * n_occ,
         "k_raycand": 28 * np_ + 16 * n_items,
         "k_ray": 40 * n_items,
