@@ -721,10 +721,12 @@ int grid_shift_for(const GridGeom& g, const float* pos, int64_t stride, int32_t 
     // bucket size w = sum(c^2)/sum(c) -- the bucket an object typically sorts
     // in -- with the sampling bias of sum(c^2) removed (E[c_s^2] = f^2 c^2 +
     // f(1 - f) c for a fraction f).  The bucket size minimising
-    // 18 * buckets + n * w is taken among those with w <= 32 and at most 2^27
-    // buckets (the finest allowed one if none has w <= 32).  The weights are
+    // 18 * buckets + n * w is taken among those with w <= 128 and at most 2^27
+    // buckets (the finest allowed one if none has w <= 128).  The weights are
     // fitted to MI355X timings of the whole grid build (scan, histogram and
-    // ranking kernels) at 2.4e5, 6.6e6 and 4.2e7 objects.
+    // ranking kernels) at 2.4e5, 6.6e6 and 4.2e7 objects; with the cap at 128
+    // the 6.6e6-object crowded scene takes 2^7-cell buckets (2^5 under a cap
+    // of 32 was 4.6% slower per frame).
     if (n <= 0 || !pos) return choose_shift(g.cell_count, n);
     const int64_t step = std::max<int64_t>(1, (int64_t)n >> 20);
     std::vector<uint32_t> keys;
@@ -759,7 +761,7 @@ int grid_shift_for(const GridGeom& g, const float* pos, int64_t stride, int32_t 
             a = b;
         }
         const double weighted = (s2 - (1.0 - f) * s1) / std::max(f * s1, 1e-30);
-        if (weighted > 32.0) break;  // coarser buckets only grow
+        if (weighted > 128.0) break;  // coarser buckets only grow
         const double cost = 18.0 * (double)nb + (double)n * weighted;
         if (best < 0 || cost < best_cost) {
             best = sh;
