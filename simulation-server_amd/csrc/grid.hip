@@ -47,7 +47,8 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
                                              const uint32_t* __restrict__ gpos, uint32_t* __restrict__ bcount,
                                              uint64_t* __restrict__ status, int32_t ntiles,
                                              uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags,
-                                             uint32_t* __restrict__ arrival)
+                                             uint32_t* __restrict__ arrival, const int2* __restrict__ order,
+                                             int32_t norder)
 {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -64,20 +65,26 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
         if (clear_table) old = keys[gid];
         if (gpos) w = gpos[gid];  // map word written by the previous build (indexed by its sorted slots)
     }
-    if (gid < n) {
-        x = pos[gid];
-        y = pos[stride + gid];
-        z = pos[2 * stride + gid];
+    // the object of this thread: gid, or with `order` the object of lane gid
+    // (objects in spatial order, so that neighbouring lanes count into
+    // neighbouring buckets; -1: no object)
+    int64_t i = gid < n ? gid : -1;
+    if (order) i = gid < norder ? (int64_t)order[gid].x : -1;
+    const bool have = i >= 0 && i < n;
+    if (have) {
+        x = pos[i];
+        y = pos[stride + i];
+        z = pos[2 * stride + i];
     }
     if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
     if (w != kNoWord) bm.words[w] = 0;
     uint32_t fl = 0;
-    if (gid < n) {
+    if (have) {
         uint32_t k = cell_key(g, x, y, z, fl);
-        keys[gid] = k;
+        keys[i] = k;
         // the count before this object: its slot within the bucket (the
         // scatter needs no atomics; slot order within a bucket is free)
-        arrival[gid] = atomicAdd(&bcount[k >> shift], 1u);
+        arrival[i] = atomicAdd(&bcount[k >> shift], 1u);
     }
     if (fl) atomicOr(flags, fl);
 }
@@ -781,7 +788,7 @@ void grid_free(DeviceGrid& G)
 }
 
 int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
-               hipStream_t st, const KernelTimer* timer)
+               hipStream_t st, const KernelTimer* timer, const int2* order, int32_t norder)
 {
     if (n < 0 || n > G.cap) return fail(BCS_ERR_INVALID_ARGUMENT, "grid: object count exceeds capacity");
     // Every slot below the capacity clears its previous cell (slots never
@@ -790,12 +797,13 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     const int32_t n_prev = G.cap;
     G.n = n;
     const int blocks = (std::max(std::max(n, n_prev), 1) + 255) / 256;
+    const int key_blocks = std::max(blocks, order ? (norder + 255) / 256 : 0);
     auto tb = [&](int id) { if (timer) timer->begin(id, st); };
     auto te = [&](int id) { if (timer) timer->end(id, st); };
     tb(0);
-    hipLaunchKernelGGL(k_key, dim3(std::max(blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride, n,
-                       n_prev, G.shift, G.keys, G.table, G.clear_table ? 1 : 0, G.bm, G.gpos, G.bcount, G.status,
-                       G.ntiles, G.ctrl, flags, G.arrival);
+    hipLaunchKernelGGL(k_key, dim3(std::max(key_blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride,
+                       n, n_prev, G.shift, G.keys, G.table, G.clear_table ? 1 : 0, G.bm, G.gpos, G.bcount, G.status,
+                       G.ntiles, G.ctrl, flags, G.arrival, order, norder);
     te(0);
     tb(1);
     hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,

@@ -56,9 +56,12 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
 void grid_free(DeviceGrid& G);
 // pos: SoA base pointer (x, y, z at +0, +stride, +2*stride).  vel (same
 // layout, nullable) with a grid allocated with copies also writes the
-// sorted-slot records pvs.
+// sorted-slot records pvs.  order (nullable): norder lanes naming every
+// object in [0, n) exactly once (.x, -1 for an empty lane), in spatial order;
+// the key kernel visits the objects in that order, so that its bucket
+// counters are hit in address order.  The result does not depend on it.
 int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
-               hipStream_t st, const KernelTimer* timer = nullptr);
+               hipStream_t st, const KernelTimer* timer = nullptr, const int2* order = nullptr, int32_t norder = 0);
 constexpr int kGridKernels = 5;
 // The occupancy map is written by k_rank when buckets span at least
 // 2^kRankMapShift cells (few 8-cell groups straddle two buckets), else by

@@ -216,7 +216,8 @@ int build_triangle_grid(bcs_sim* s)
 
 int enqueue_particle_grid(bcs_sim* s, const KernelTimer* timer = nullptr)
 {
-    RC(grid_build(s->pg, s->pos, s->np, s->np, s->vel, s->flags, s->stream, timer));
+    RC(grid_build(s->pg, s->pos, s->np, s->np, s->vel, s->flags, s->stream, timer, s->chunk_pi,
+                  s->nchunks * kBlock));
     s->copies_current = true;
     return BCS_OK;
 }
