@@ -9,7 +9,7 @@ constexpr int kMaxEndings = 16;
 constexpr int kBlock = 256;  // threads per cell-chunk block
 constexpr int kBsLds = 1024;       // model-vertex radii held in LDS by the contact search (more: read globally)
 constexpr int kContactBatch = 4;   // sorted-slot records loaded together by the contact search
-constexpr int kCellBatch = 8;      // neighbour-cell table entries loaded together by the contact search
+constexpr int kCellBatch = 6;      // neighbour-cell table entries loaded together by the contact search
 constexpr uint32_t kXcdRun = 16;   // consecutive chunks per XCD run in the block -> chunk mapping
 constexpr int kSpringLds = 1024;   // springs of one cell type held in LDS by k_cells (more: read globally)
 constexpr int kOrderBits = 15;      // Morton bits of the cell processing order
