@@ -370,3 +370,252 @@ def test_fp64_rerun_drift_is_small_over_100_steps(bcs, orc):
     drift = np.abs(pa.astype(np.float64) - pb)
     assert np.isfinite(drift).all()
     assert drift.max() < 1e-2, drift.max()
+
+
+# ------------------------------------- first hit, launch type, teleport --
+#
+# Scenes whose outcome depends on the cell visit order of the ray search, on
+# the launch type's model table in the contact search and on which cell types
+# test the ending spheres.  The expected values are worked out by hand in the
+# comments; a nearest-hit search, a z-outer loop, a radius from the
+# particle's own type, a floor (not truncating) remainder or ending spheres
+# tested for every type fails one of them.
+
+def _anchored_mesh(tris, lo=0.0, hi=100.0):
+    """Separate-vertex triangles plus two anchor vertices (in no triangle) at
+    (lo, lo, lo) and (hi, hi, hi), so that with zero margins the grid bounds
+    are exactly [lo, hi]^3."""
+    verts = [v for t in tris for v in t] + [[lo, lo, lo], [hi, hi, hi]]
+    idx = [[3 * k, 3 * k + 1, 3 * k + 2] for k in range(len(tris))]
+    return np.asarray(verts, F32), np.asarray(idx, np.uint32)
+
+
+# Three horizontal triangles on the ray x = z = 15, y = 15 + t (particle at
+# (15, 15, 15) moving +y; triangle grid cell 10 on [0, 100]^3, so the particle
+# is in cell (1, 1, 1)).  Centroids:
+#   A  y = 17 (t = 2)  centroid (15, 17, 15)    -> cell (1, 1, 1)  dx=0,  dz=0
+#   C  y = 18 (t = 3)  centroid (15, 18, 9.33)  -> cell (1, 1, 0)  dx=0,  dz=-1
+#   B  y = 19 (t = 4)  centroid (8, 19, 16.67)  -> cell (0, 1, 1)  dx=-1, dz=0
+# The x-outer, y, z-inner visit order reaches B's cell (dx = -1) first, so B
+# is the first hit although A is nearer (nearest hit: A; z-outer order: C;
+# triangle-index order: A).
+TRI_A = [[14, 17, 14], [17, 17, 14], [14, 17, 17]]
+TRI_C = [[12, 18, 4], [18, 18, 4], [15, 18, 20]]
+TRI_B = [[2, 19, 14], [20, 19, 14], [2, 19, 22]]
+
+
+def first_hit_scene(bcs, tris):
+    verts, idx = _anchored_mesh(tris)
+    mesh = bcs.MeshSpec(verts, idx)
+    ph = _phys(bcs, dt=0.01, particle_cell=(5.0, 5.0, 5.0), triangle_cell=(10.0, 10.0, 10.0),
+               grid_xz_margin=0.0, grid_y_margin=0.0)
+    types = [_point_type(bcs, 1)]
+    pos = np.array([15, 15, 15], F32)
+    vel = np.array([0, 10, 0], F32)
+    return ph, types, mesh, pos, vel
+
+
+def _one_frame(orc, ph, types, mesh, pos, vel, force=None):
+    o = orc.Oracle(ph, types, mesh)
+    o.set_particles(pos, vel, np.zeros_like(pos) if force is None else force)
+    o.calculate_grid(0)
+    o.calculate_grid(1)
+    o.calculate_next_frame()
+    return o
+
+
+def test_first_hit_follows_x_outer_cell_order(bcs, orc):
+    ph, types, mesh, pos, vel = first_hit_scene(bcs, [TRI_A, TRI_C, TRI_B])
+    o = _one_frame(orc, ph, types, mesh, pos, vel)
+    _, v, f = o.get_particles()
+    # every candidate is a horizontal plane: reflection (0, -1, 0),
+    # v = v + (0.96 * 10 * (0, -1, 0) - v) = (0, -9.6, 0); reaction force
+    # removes all of F = (G - 0.39 v) / 2, which has only a y component
+    np.testing.assert_allclose(v, [0, -9.6, 0], rtol=1e-6)
+    np.testing.assert_allclose(f, [0, 0, 0], atol=1e-6)
+    Fv = soa(o.get_vein()[2], mesh.positions.size // 3)
+    # ds = 0.005 * (0, 10, 0) = (0, 0.05, 0) on B's corners (vertices 6, 7, 8)
+    # with the reference's barycentric of (15, 19, 15) in B:
+    #   e0 = (18, 0, 0), e1 = v2 - v1 = (-18, 0, 8), e2 = (13, 0, 1)
+    #   d00 = 324, d01 = -324, d11 = 388, d20 = 234, d21 = -226, den = 20736
+    #   x = (388*234 - 324*226) / 20736 = 17568/20736, y = 2592/20736, z = 1 - x - y
+    x, y = 17568 / 20736, 2592 / 20736
+    np.testing.assert_allclose(Fv[1, 6:9], [0.05 * x, 0.05 * y, 0.05 * (1 - x - y)], rtol=1e-5)
+    np.testing.assert_array_equal(Fv[:, 0:6], 0)  # A and C untouched
+    np.testing.assert_array_equal(Fv[[0, 2], 6:9], 0)
+
+
+def test_first_hit_without_the_x_neighbour_is_the_z_neighbour(bcs, orc):
+    """With B removed the dx = 0 column is next: C's cell (dz = -1) comes
+    before A's (dz = 0), so the farther C still beats the nearer A."""
+    ph, types, mesh, pos, vel = first_hit_scene(bcs, [TRI_A, TRI_C])
+    o = _one_frame(orc, ph, types, mesh, pos, vel)
+    Fv = soa(o.get_vein()[2], mesh.positions.size // 3)
+    # barycentric of (15, 18, 15) in C: e0 = (6, 0, 0), e1 = (-3, 0, 16),
+    # e2 = (3, 0, 11): d00 = 36, d01 = -18, d11 = 265, d20 = 18, d21 = 167,
+    # den = 36*265 - 324 = 9216; x = (265*18 + 18*167)/9216 = 7776/9216,
+    # y = (36*167 + 18*18)/9216 = 6336/9216
+    x, y = 7776 / 9216, 6336 / 9216
+    np.testing.assert_allclose(Fv[1, 3:6], [0.05 * x, 0.05 * y, 0.05 * (1 - x - y)], rtol=1e-5)
+    np.testing.assert_array_equal(Fv[:, 0:3], 0)
+
+
+# Two triangles in the particle's own cell (1, 1, 1): D (index 0, y = 19,
+# t = 4) and E (index 1, y = 17, t = 2).  Within a cell the stable sort keeps
+# index order, so D is tested first and wins although E is nearer.
+TRI_D = [[13, 19, 13], [19, 19, 13], [13, 19, 19]]
+TRI_E = [[14, 17, 14], [17, 17, 14], [14, 17, 17]]
+
+
+def test_first_hit_within_a_cell_follows_sorted_slots(bcs, orc):
+    ph, types, mesh, pos, vel = first_hit_scene(bcs, [TRI_D, TRI_E])
+    o = _one_frame(orc, ph, types, mesh, pos, vel)
+    Fv = soa(o.get_vein()[2], mesh.positions.size // 3)
+    # barycentric of (15, 19, 15) in D: e0 = (6, 0, 0), e1 = (-6, 0, 6),
+    # e2 = (2, 0, 2): d00 = 36, d01 = -36, d11 = 72, d20 = 12, d21 = 0,
+    # den = 1296; x = 864/1296, y = 432/1296, z = 0
+    np.testing.assert_allclose(Fv[1, 0:2], [0.05 * 864 / 1296, 0.05 * 432 / 1296], rtol=1e-5)
+    assert abs(Fv[1, 2]) < 1e-8
+    np.testing.assert_array_equal(Fv[:, 3:6], 0)
+
+
+# Launch-type radius.  Type 0: three particles in one cell, model vertices at
+# x = 0, 1, 3 -> derived radii min distance / (2*3) = [1/6, 1/6, 1/3].
+# Type 1: two particles, model vertices at x = 0, 2.4 -> [0.4, 0.4].
+# Model table bs = [1/6, 1/6, 1/3, 0.4, 0.4]; particle starts [0, 3].
+# Positions (grid cell 1, z cell boundary at z = 0):
+#   P3 (0, 0, -0.3)  P1 (10, 0, -0.3)  P2 (20, 0, -0.3)  -> z cell 99
+#   P0 (0, 0, 0.35)                                      -> z cell 100
+#   P4 (30, 0, 20)                                       -> z cell 120
+# Sorted slots: [P3, P1, P2, P0, P4]; slots 0-2 run with type 0's tables,
+# slots 3-4 with type 1's (src/simulation/particle_collisions.cuh:124,76).
+#   P0 in slot 3 (type 1): own radius bs[3 + (0-3) % 2] = bs[3 - 1] = 1/3
+#       (C remainder -3 % 2 = -1), partner P3: bs[3 + 0 % 2] = 0.4;
+#       0.65 <= 1/3 + 0.4 -> contact, spring (2/3 - 0.65) * -6 * 0.5 = -0.05 in z
+#   P3 in slot 0 (type 0): bs[0 + 3 % 3] = 1/6, partner P0 bs[0] = 1/6;
+#       0.65 > 1/3 -> no contact
+# Own-type radii (1/6 + 0.4 = 0.567) give no contact at all; a floor
+# remainder gives P0 radius 0.4 and a force of -0.45.
+
+def launch_type_scene(bcs):
+    mesh = _tri_mesh(bcs, [[500, 500, 500], [510, 500, 500], [500, 510, 510]])
+    ph = _phys(bcs, dt=0.01, particle_cell=(1.0, 1.0, 1.0), grid_xz_margin=600.0, grid_y_margin=600.0)
+    t0 = bcs.CellTypeSpec(1, [0, 0, 0, 1, 0, 0, 3, 0, 0], [], contact_radius=0.0)
+    t1 = bcs.CellTypeSpec(1, [0, 0, 0, 2.4, 0, 0], [], contact_radius=0.0)
+    P = np.array([[0, 0, 0.35], [10, 0, -0.3], [20, 0, -0.3], [0, 0, -0.3], [30, 0, 20]], F32)
+    return ph, [t0, t1], mesh, np.ascontiguousarray(P.T).reshape(-1)
+
+
+def test_contact_radius_comes_from_the_launch_type(bcs, orc):
+    ph, types, mesh, pos = launch_type_scene(bcs)
+    o = orc.Oracle(ph, types, mesh)
+    bs, _, _ = o.tables()
+    np.testing.assert_allclose(bs, [1 / 6, 1 / 6, 1 / 3, 0.4, 0.4], rtol=1e-6)
+    o.set_particles(pos, np.zeros_like(pos), np.zeros_like(pos))
+    o.calculate_grid(0)
+    keys, perm, *_ = o.get_grid(0)
+    np.testing.assert_array_equal(perm, [3, 1, 2, 0, 4])
+    counts, pairs = o.contacts()
+    np.testing.assert_array_equal(counts, [1, 0, 0, 0, 0])
+    np.testing.assert_array_equal(pairs, [3])
+    o.calculate_grid(1)
+    o.calculate_next_frame()
+    F = soa(o.get_particles()[2], 5)
+    # cell springs (no springs, v = 0): F = G / 2; then the contact of P0
+    np.testing.assert_allclose(F[:, 0], [0, -12.5, -0.05], rtol=1e-3, atol=1e-7)
+    np.testing.assert_array_equal(F[:, 3], np.array([0, -12.5, 0], F32))
+
+
+# Teleport selection (src/simulation/vein_end.cu:23-30): a type whose
+# particles fill whole 32-lane groups (32 % ppc == 0) or which has at most 32
+# particles checks only the thresholds; the others also check the ending
+# spheres.  One cell of each type sits inside the ending sphere at (50, 0, 0),
+# well inside the thresholds; only the ppc-12 type with 10 cells teleports.
+
+def teleport_scene(bcs):
+    mesh = bcs.MeshSpec(np.array([[-100, -100, -100], [100, -100, -100], [0, 100, 100]], F32),
+                        np.array([[0, 1, 2]], np.uint32), [[50.0, 0.0, 0.0]], [5.0])
+    ph = _phys(bcs, dt=0.01)
+    ph.use_blood_flow = 1
+
+    def ring(ppc):
+        a = np.arange(ppc) * (2 * np.pi / ppc)
+        return np.stack([np.cos(a), np.sin(a), np.zeros(ppc)], 1).astype(F32).reshape(-1)
+
+    types = [bcs.CellTypeSpec(10, ring(8), [], 0.5),    # 32 % 8 == 0: thresholds only
+             bcs.CellTypeSpec(10, ring(12), [], 0.5),   # 120 particles, 32 % 12 != 0: spheres too
+             bcs.CellTypeSpec(2, ring(12), [], 0.5)]    # 24 <= 32 particles: thresholds only
+    pos = []
+    for t in types:
+        for c in range(t.count):
+            centre = np.array([50.0, 0.0, 0.0]) if c == 0 else np.array([-60.0 + 4 * c, 30.0 * len(pos) / 100, 0.0])
+            pos.append(t.vertices.reshape(-1, 3) + centre)
+    P = np.concatenate(pos).astype(F32)
+    return ph, types, mesh, np.ascontiguousarray(P.T).reshape(-1)
+
+
+def test_teleport_ending_spheres_only_for_block_synchronised_types(bcs, orc):
+    ph, types, mesh, pos = teleport_scene(bcs)
+    o = orc.Oracle(ph, types, mesh)
+    n = pos.size // 3
+    o.set_particles(pos, np.zeros_like(pos), np.zeros_like(pos))
+    o.propagate_all()
+    X, V = soa(o.get_particles()[0], n), soa(o.get_particles()[1], n)
+    P0 = soa(pos, n)
+    moved = np.flatnonzero(np.any(X != P0, axis=0))
+    # type 1 starts at particle 80; its cell 0 is particles 80..91
+    np.testing.assert_array_equal(moved, np.arange(80, 92))
+    np.testing.assert_array_equal(X[1, 80:92], np.float32(ph.min_spawn_y) + (P0[1, 80:92] - P0[1, 80]))
+    np.testing.assert_array_equal(V[:, 80:92], np.repeat(np.array(ph.init_velocity[:], F32)[:, None], 12, 1))
+
+
+# Runtime switches (src/config/simulation.hpp:6-8).
+
+def test_reaction_force_off_adds_the_resilient_force(bcs, orc):
+    """enableReactionForce = 0 (src/simulation/vein_collisions.cu:245-249):
+    the wall pushes with addResilientForceOnCollision on relativePosition =
+    pos - hit = (0, 5, 0): dir (0, 1, 0), spring -6 (2*0.5 - 5) = 24,
+    damping 6 * (0, -10, 0), no shear: 0.5 * (0, -36, 0) = (0, -18, 0) added
+    to F = (G - 0.39 v) / 2 = (0, -10.55, 0)."""
+    verts = [[-10, -5, -10], [10, -5, -10], [0, -5, 10]]
+    mesh = _tri_mesh(bcs, verts)
+    ph = _phys(bcs, dt=0.01, particle_cell=(2.0, 2.0, 2.0), triangle_cell=(25.0, 25.0, 25.0),
+               enable_reaction_force=0)
+    o = _one_frame(orc, ph, [_point_type(bcs, 1)], mesh, np.zeros(3, F32), np.array([0, -10, 0], F32))
+    _, v, f = o.get_particles()
+    np.testing.assert_allclose(f, [0, -28.55, 0], rtol=1e-6)
+    np.testing.assert_allclose(v, [0, 9.6, 0], rtol=1e-6)  # the reflection is unchanged
+
+
+def test_big_cells_brake_off(bcs, orc):
+    """enableBigCellsBrake = 0 (src/simulation/physics.cuh:102-120): ratio 2
+    no longer brakes: env = G - 0.39 v, F_x = (100 - 0.39*2) / 2."""
+    mesh = _tri_mesh(bcs, [[500, 500, 500], [510, 500, 500], [500, 510, 510]])
+    ph = _phys(bcs, dt=0.01, particle_k_sniff=50.0, particle_d_fact=0.5, particle_cell=(5.0, 5.0, 5.0),
+               grid_xz_margin=600.0, grid_y_margin=600.0, enable_big_cells_brake=0)
+    t = _two_particle_spring(bcs, 4.0, 0.0)
+    o = orc.Oracle(ph, [t], mesh)
+    o.set_particles(np.array([0, 4, 0, 0, 0, 0], F32), np.array([2, 2, 0, 0, 0, 0], F32), np.zeros(6, F32))
+    o.calculate_grid(0)
+    o.calculate_grid(1)
+    o.calculate_next_frame()
+    F = soa(o.get_particles()[2], 2)
+    np.testing.assert_allclose(F[:, 0], [(100.0 - 0.78) / 2, -12.5, 0.0], rtol=1e-6)
+
+
+def test_blood_flow_off_keeps_cells_beyond_the_thresholds(bcs, orc):
+    """useBloodFlow = 0 (src/simulation/simulation_controller.cu:328): no
+    teleport; the cell below the lower threshold only integrates."""
+    sc = bcs.Scene(0, 0.01, seed=0)
+    ph = sc.physics
+    ph.use_blood_flow = 0
+    o = orc.Oracle(ph, sc.types, sc.mesh)
+    n = sc.particle_count
+    p = soa(sc.positions, n).copy()
+    p[1, :12] -= F32(600.0)
+    v = soa(sc.velocities, n)
+    o.set_particles(p.reshape(-1), sc.velocities, np.zeros(3 * n, F32))
+    o.propagate_all()
+    X = soa(o.get_particles()[0], n)
+    dt = F32(ph.dt)
+    np.testing.assert_array_equal(X, p + F32(0.5) * dt * (v + v))
