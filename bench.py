@@ -1,22 +1,25 @@
 """Benchmark of the MI355X blood-cell physics step (one JSON line on stdout).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--no-cpu]
 
-Workload (default): config 1 of the plain method -- 20,000 icosahedron cells
-(240,000 particles, FP32) in a Y-bifurcating vein (trunk r 60 x 600, branches
-r 42 x 400 at +/-30 deg, ~200k triangles), dt = 0.005, seeded placement.
-A step is one full frame (particle grid, vein springs, cell springs,
-particle contacts, vein ray collisions, integration, vein-end teleport),
-replayed from a HIP graph; all state is resident in HBM during the timed
-region.  `value` is particle-steps/s.
+Workload (default): config 2 of the plain method -- 200,000 cells (70%
+42-particle discs, 30% 12-particle icosahedra; 6.6M particles, FP32) packed
+at 35% of the lumen volume in a Y-bifurcating vein scaled 3x (trunk r 180 x
+1800, branches r 126 x 1200 at +/-30 deg, ~1.8M triangles), dt = 0.005,
+seeded placement.  A step is one full frame (particle grid, vein springs,
+cell springs, particle contacts, vein ray collisions, integration, vein-end
+teleport), replayed from HIP graphs; all state is resident in HBM during the
+timed region.  `value` is particle-steps/s over frames W+1 .. W+K.
 
 Roofline (dominant kernel, HBM-bound): algorithmic bytes per launch / its
 average duration measured with HIP events on the library's stream.  The
 algorithmic bytes per launch of each kernel are listed in kernel_bytes();
-`traffic` comes from the committed rocprofv3 PMC summary when present.
+`traffic` comes from the committed rocprofv3 PMC summary of the same
+workload when present.
 
 cpu_baseline: the CPU oracle (the C restatement of the reference frame,
-OpenMP) on a bounded sample of the same workload, on this host's cores.
+OpenMP) on a bounded sample of the same workload, on all of this host's
+cores, with a one-thread sample beside it.
 """
 from __future__ import annotations
 
@@ -100,27 +103,54 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0, map_in_rank=False):
     return table.get(name)
 
 
-def cpu_baseline(scene, target_s=12.0, max_steps=400):
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle_py
+def host_cores():
+    """Cores this process may run on (the affinity mask), and the machine's."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or usable
 
-    threads = min(16, os.cpu_count() or 1)
+
+def _time_oracle(oracle_py, scene, threads, warmup, target_s, max_steps):
     oracle_py.set_threads(threads)
     o = oracle_py.Oracle(scene.physics, scene.types, scene.mesh)
     o.set_particles(scene.positions, scene.velocities, scene.forces)
-    o.step(2)  # warm-up
+    if warmup:
+        o.step(warmup)
     n = 0
     t0 = time.perf_counter()
-    while n < max_steps and time.perf_counter() - t0 < target_s:
+    while n < max_steps and (n == 0 or time.perf_counter() - t0 < target_s):
         o.step(1)
         n += 1
     dt = time.perf_counter() - t0
+    o.close()
+    return n, dt
+
+
+def cpu_baseline(scene, target_s=12.0, max_steps=400):
+    """The oracle on every usable host core (OpenMP), then on one thread, each
+    on a bounded sample: whole frames of the same workload from its initial
+    state until about target_s seconds have passed (at least one frame)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py
+
+    usable, machine = host_cores()
+    n, dt = _time_oracle(oracle_py, scene, usable, 1, target_s, max_steps)
+    n1, dt1 = _time_oracle(oracle_py, scene, 1, 0, target_s, max_steps)
     return {
         "value": scene.particle_count * n / dt,
         "unit": "particle-steps/s",
-        "cores": threads,
+        "cores": usable,
         "kind": "port",
-        "sample": f"{n} frames (after 2 warm-up) of the same workload on {threads} OpenMP threads, {dt:.1f} s",
+        "sample": (f"frames 2..{n + 1} of the same workload from its initial state (frame 1 untimed) on {usable} "
+                   f"OpenMP threads, one per usable core ({machine} logical CPUs on the host), {dt:.1f} s"),
+        "single_thread": {
+            "value": scene.particle_count * n1 / dt1,
+            "unit": "particle-steps/s",
+            "cores": 1,
+            "sample": f"frames 1..{n1} of the same workload from its initial state on one thread, {dt1:.1f} s",
+        },
     }
 
 
@@ -140,9 +170,9 @@ def pmc_traffic(kernel, config_id):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", type=int, default=1, choices=sorted(WORKLOADS))
+    ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
@@ -186,7 +216,8 @@ def main():
     kern = sim.profile_kernels(args.profile_steps)
     dominant = max(kern, key=kern.get)
     dom_ms = kern[dominant]
-    dom_bytes = kernel_bytes(dominant, np_, n_occ, nc, nv, nt, frame_stats.get("ray_work_items", 0))
+    map_in_rank = kern.get("k_occ", 0.0) == 0.0
+    dom_bytes = kernel_bytes(dominant, np_, n_occ, nc, nv, nt, frame_stats.get("ray_work_items", 0), map_in_rank)
     achieved = dom_bytes / (dom_ms * 1e-3) if dom_bytes else None
     roof = {
         "kernel": dominant,
@@ -212,6 +243,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "profile_steps": args.profile_steps,
+        "map_in_rank": map_in_rank,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -220,6 +253,7 @@ def main():
         "data": "synthetic (seeded scene generator, seed %d)" % args.seed,
         "config": {
             "workload": WORKLOADS[args.config],
+            "frames_timed": f"{args.warmup + 1}..{args.warmup + args.steps}",
             "particles": np_,
             "cells": nc,
             "vein_vertices": nv,
