@@ -167,7 +167,29 @@ typedef struct bcs_device_state {
     float* vein_forces;
     float* triangle_centers;
     void* stream;         /* hipStream_t the simulation enqueues on */
+    /* per model vertex (sum of ppc over the types, in type order) */
+    float* model_radii;   /* collision radii: cellModelsBoundingSpheres
+                             (src/simulation/simulation_controller.cuh:42) */
+    float* initial_radii; /* distance to the model centroid: BloodCells::initialRadiuses
+                             (src/objects/blood_cells.cuh:33) */
+    uint32_t* triangle_indices; /* 3 per triangle: VeinTriangles::indices
+                                   (src/objects/vein_triangles.cuh:60) */
 } bcs_device_state;
+
+/* Device view of one grid: the public members of the reference's
+ * UniformGrid (src/grids/uniform_grid.cuh:23-35) after its last build. */
+typedef struct bcs_grid_view {
+    uint32_t* grid_cell_ids;    /* gridCellIds: cell id of every sorted slot (object_count) */
+    int32_t* particle_ids;      /* particleIds: object index of every sorted slot (object_count) */
+    int32_t* grid_cell_starts;  /* gridCellStarts: first slot of every cell id (cell_count), or NULL */
+    int32_t* grid_cell_ends;    /* gridCellEnds: last slot (inclusive) of every cell id, or NULL */
+    int32_t object_count;       /* objectCount */
+    int32_t cell_counts[3];     /* cellCountX/Y/Z */
+    uint64_t cell_count;        /* cellCount */
+    float cell_size[3];         /* cellWidth/Height/Depth */
+    float bounds_min[3];        /* minX, minY, minZ */
+    float bounds_max[3];        /* maxX, maxY, maxZ */
+} bcs_grid_view;
 
 typedef struct bcs_sim bcs_sim;
 
@@ -299,6 +321,23 @@ int bcs_sim_get_grid(bcs_sim* sim, int32_t which, uint32_t* keys_sorted, int32_t
  * pairs_j may be NULL to only count. */
 int bcs_sim_get_contacts(bcs_sim* sim, int32_t* counts, int32_t* pairs_j, int64_t capacity,
                          int64_t* n_total);
+
+/* Device pointers of a grid's sorted arrays (src/grids/uniform_grid.cuh:32-35).
+ * With dense_ranges != 0 the view also holds per-cell-id start and end
+ * arrays laid out as the reference's gridCellStarts / gridCellEnds, filled
+ * from the current grid at this call (empty cells: start 0, end -1, so an
+ * inclusive loop over them runs no iteration; the reference leaves them
+ * stale).  They are owned by the simulation, allocated at the first such
+ * call (cellCount int32 each; BCS_ERR_GRID_TOO_LARGE above 2^31 cells) and
+ * refreshed only by this call, on the simulation's stream.  The frame
+ * kernels themselves use a compact table and never read them. */
+int bcs_sim_grid_view(bcs_sim* sim, int32_t which, int32_t dense_ranges, bcs_grid_view* out);
+
+/* UniformGrid::calculateCellId (src/grids/uniform_grid.cuh:60,
+ * src/grids/uniform_grid.cu:24-36,157-160) for n host positions (SoA: x[n],
+ * y[n], z[n]) on the particle or triangle grid; the same keys the grid build
+ * computes.  Host computation, no GPU work. */
+int bcs_sim_cell_ids(bcs_sim* sim, int32_t which, const float* positions, int32_t n, uint32_t* cell_ids);
 
 /* Counters of the last completed frame: ray work items (triangle runs of
  * <= 16 whose cell box the ray can reach) and particle-vein hits. */

@@ -141,6 +141,24 @@ class DeviceState(C.Structure):
         ("vein_forces", C.c_void_p),
         ("triangle_centers", C.c_void_p),
         ("stream", C.c_void_p),
+        ("model_radii", C.c_void_p),
+        ("initial_radii", C.c_void_p),
+        ("triangle_indices", C.c_void_p),
+    ]
+
+
+class GridView(C.Structure):
+    _fields_ = [
+        ("grid_cell_ids", C.c_void_p),
+        ("particle_ids", C.c_void_p),
+        ("grid_cell_starts", C.c_void_p),
+        ("grid_cell_ends", C.c_void_p),
+        ("object_count", C.c_int32),
+        ("cell_counts", C.c_int32 * 3),
+        ("cell_count", C.c_uint64),
+        ("cell_size", C.c_float * 3),
+        ("bounds_min", C.c_float * 3),
+        ("bounds_max", C.c_float * 3),
     ]
 
 
@@ -225,6 +243,8 @@ def _sig(lib):
     lib.bcs_sim_get_grid.argtypes = [vp, C.c_int32, u32p, i32p, i32p, u32p, i32p, i32p]
     lib.bcs_sim_get_contacts.argtypes = [vp, i32p, i32p, C.c_int64, P(C.c_int64)]
     lib.bcs_sim_error_flags.argtypes = [vp, C.c_int32, u32p]
+    lib.bcs_sim_grid_view.argtypes = [vp, C.c_int32, C.c_int32, P(GridView)]
+    lib.bcs_sim_cell_ids.argtypes = [vp, C.c_int32, f32p, C.c_int32, u32p]
     lib.bcs_sim_frame_stats.argtypes = [vp, u32p, u32p]
     lib.bcs_device_count.argtypes = [i32p]
     lib.bcs_calibration_copy.argtypes = [C.c_int64, C.c_int32, f32p]
@@ -528,6 +548,21 @@ class Simulation:
         check(self.lib.bcs_sim_get_contacts(self.h, iptr(counts), iptr(pairs), total.value, C.byref(total)),
               "get_contacts")
         return counts[: self.np], pairs[: total.value]
+
+    def grid_view(self, which=GRID_PARTICLES, dense_ranges=False) -> GridView:
+        """Device pointers of a grid's sorted arrays (and, with dense_ranges,
+        per-cell-id start/end arrays refreshed from the current grid)."""
+        v = GridView()
+        check(self.lib.bcs_sim_grid_view(self.h, which, 1 if dense_ranges else 0, C.byref(v)), "grid_view")
+        return v
+
+    def cell_ids(self, positions, which=GRID_PARTICLES):
+        """Cell ids of SoA host positions (x[n], y[n], z[n]) on a grid."""
+        p = np.ascontiguousarray(positions, np.float32)
+        n = p.size // 3
+        out = np.empty(max(n, 1), np.uint32)
+        check(self.lib.bcs_sim_cell_ids(self.h, which, fptr(p), n, uptr(out)), "cell_ids")
+        return out[:n]
 
     def frame_stats(self):
         w, h = C.c_uint32(0), C.c_uint32(0)

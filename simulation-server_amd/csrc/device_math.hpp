@@ -37,7 +37,7 @@ __device__ __forceinline__ f3 normalize(f3 v, uint32_t& fl)
 }
 
 // static_cast<int>(float) with saturation and NaN -> 0.
-__device__ __forceinline__ int32_t sat_i32(float q)
+__host__ __device__ __forceinline__ int32_t sat_i32(float q)
 {
     if (q != q) return 0;
     if (q >= 2147483647.0f) return 2147483647;
@@ -124,7 +124,7 @@ __device__ __forceinline__ void grid_coords(const GridGeom& g, uint32_t key, uin
 // calculateIdForCell, src/grids/uniform_grid.cu:24-36 (max/min macros of
 // :20-21, upper clamp in world units).  Keys outside [0, cellCount) are
 // clamped to cellCount-1 and flagged.
-__device__ __forceinline__ uint32_t cell_key(const GridGeom& g, float x, float y, float z, uint32_t& fl)
+__host__ __device__ __forceinline__ uint32_t cell_key(const GridGeom& g, float x, float y, float z, uint32_t& fl)
 {
     if (x < g.minv[0] || x > g.maxv[0] || y < g.minv[1] || y > g.maxv[1] || z < g.minv[2] || z > g.maxv[2])
         fl |= BCS_FLAG_OUT_OF_GRID;

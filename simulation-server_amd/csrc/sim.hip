@@ -31,6 +31,8 @@ void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t
 void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st);
 void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st);
+void launch_dense_ranges(const uint32_t* keys_sorted, int32_t n, uint64_t cell_count, int32_t* starts,
+                         int32_t* ends, hipStream_t st);
 void launch_vein_prep(const VeinArgs& V, int32_t ntri, const uint4* trun, const uint4* trec, float4* tdata,
                       float4* aabb, hipStream_t st);
 void launch_tri_records(const DeviceGrid& tg, const uint32_t* idx, uint4* trun, uint4* trec, hipStream_t st);
@@ -111,6 +113,9 @@ struct bcs_sim {
     hipGraph_t graph = nullptr, graph_b = nullptr, graph_m = nullptr;
     std::vector<float> host_bs;
     std::vector<float> host_smallest;
+    // dense gridCellStarts / gridCellEnds views (bcs_sim_grid_view), per grid
+    int32_t* dense_start[2] = {nullptr, nullptr};
+    int32_t* dense_end[2] = {nullptr, nullptr};
 
     ~bcs_sim()
     {
@@ -585,6 +590,52 @@ extern "C" int bcs_sim_device_state(bcs_sim* s, bcs_device_state* out)
     out->vein_positions = s->X; out->vein_velocities = s->V; out->vein_forces = s->Fv;
     out->triangle_centers = s->tcent;
     out->stream = (void*)s->stream;
+    out->model_radii = s->bs;
+    out->initial_radii = s->r0;
+    out->triangle_indices = s->tidx;
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_grid_view(bcs_sim* s, int32_t which, int32_t dense_ranges, bcs_grid_view* out)
+{
+    RC(set_dev(s));
+    if (!out) return fail(BCS_ERR_INVALID_ARGUMENT, "null output");
+    if (which != BCS_GRID_PARTICLES && which != BCS_GRID_TRIANGLES) return fail(BCS_ERR_INVALID_ARGUMENT, "bad grid");
+    const DeviceGrid& G = which == BCS_GRID_PARTICLES ? s->pg : s->tg;
+    std::memset(out, 0, sizeof(*out));
+    out->grid_cell_ids = G.keys_sorted;
+    out->particle_ids = G.perm;
+    out->object_count = G.n;
+    for (int d = 0; d < 3; ++d) {
+        out->cell_counts[d] = G.g.cnt[d];
+        out->cell_size[d] = G.g.cell[d];
+        out->bounds_min[d] = G.g.minv[d];
+        out->bounds_max[d] = G.g.maxv[d];
+    }
+    out->cell_count = G.g.cell_count;
+    if (!dense_ranges) return BCS_OK;
+    if (G.g.cell_count > (uint64_t)INT32_MAX)
+        return fail(BCS_ERR_GRID_TOO_LARGE, "dense cell ranges: more than 2^31 cells");
+    if (!s->dense_start[which]) {
+        RC(s->dalloc(&s->dense_start[which], G.g.cell_count));
+        RC(s->dalloc(&s->dense_end[which], G.g.cell_count));
+    }
+    launch_dense_ranges(G.keys_sorted, G.n, G.g.cell_count, s->dense_start[which], s->dense_end[which], s->stream);
+    HIPCHK(hipGetLastError());
+    out->grid_cell_starts = s->dense_start[which];
+    out->grid_cell_ends = s->dense_end[which];
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_cell_ids(bcs_sim* s, int32_t which, const float* positions, int32_t n, uint32_t* cell_ids)
+{
+    if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
+    if (which != BCS_GRID_PARTICLES && which != BCS_GRID_TRIANGLES) return fail(BCS_ERR_INVALID_ARGUMENT, "bad grid");
+    if (n < 0 || (n > 0 && (!positions || !cell_ids))) return fail(BCS_ERR_INVALID_ARGUMENT, "bad arrays");
+    const GridGeom& g = (which == BCS_GRID_PARTICLES ? s->pg : s->tg).g;
+    uint32_t fl = 0;
+    for (int32_t i = 0; i < n; ++i)
+        cell_ids[i] = cell_key(g, positions[i], positions[(size_t)n + i], positions[2 * (size_t)n + i], fl);
     return BCS_OK;
 }
 

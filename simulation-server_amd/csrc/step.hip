@@ -1386,6 +1386,28 @@ __global__ void __launch_bounds__(256) k_gather_start(const uint2* __restrict__ 
     if (i < n) out[i] = table[idx[i]].x;
 }
 
+// Dense per-cell-id ranges in the reference's gridCellStarts / gridCellEnds
+// layout (src/grids/uniform_grid.cu:51-80), from the sorted keys: every entry
+// first set to the empty range (0, -1), then each occupied cell's first and
+// last slot written by the slot where its run starts or ends.
+__global__ void __launch_bounds__(256) k_fill_ranges(int32_t* __restrict__ starts, int32_t* __restrict__ ends,
+                                                     uint64_t m)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
+        starts[i] = 0;
+        ends[i] = -1;
+    }
+}
+__global__ void __launch_bounds__(256) k_dense_ranges(const uint32_t* __restrict__ keys, int32_t n,
+                                                      int32_t* __restrict__ starts, int32_t* __restrict__ ends)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) starts[k] = i;
+    if (i == n - 1 || keys[i + 1] != k) ends[k] = i;
+}
+
 // Plain copy (HBM counter calibration).
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n)
 {
@@ -1534,6 +1556,15 @@ void launch_vein_prep(const VeinArgs& V, int32_t ntri, const uint4* trun, const 
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st)
 {
     if (n > 0) hipLaunchKernelGGL(k_gather_start, dim3((n + 255) / 256), dim3(256), 0, st, table, idx, n, out);
+}
+void launch_dense_ranges(const uint32_t* keys_sorted, int32_t n, uint64_t cell_count, int32_t* starts,
+                         int32_t* ends, hipStream_t st)
+{
+    if (cell_count > 0) {
+        const uint64_t blocks = std::min<uint64_t>((cell_count + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_fill_ranges, dim3((uint32_t)blocks), dim3(256), 0, st, starts, ends, cell_count);
+    }
+    if (n > 0) hipLaunchKernelGGL(k_dense_ranges, dim3((n + 255) / 256), dim3(256), 0, st, keys_sorted, n, starts, ends);
 }
 void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st)
 {

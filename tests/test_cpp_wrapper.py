@@ -32,6 +32,7 @@ def test_reference_loop_builds_and_runs_or_reports_no_device(loop_exe, tmp_path)
     r = subprocess.run([str(loop_exe), "0.05", "2", str(tmp_path / "o.bin")], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode in (0, 3), r.stderr
+    assert "NoGrid: unsupported" in r.stdout  # refused before any device work
     if r.returncode == 3:
         assert "no HIP device" in r.stderr
 
@@ -44,10 +45,16 @@ def test_reference_loop_matches_oracle(loop_exe, tmp_path, bcs, orc, mode):
     args = [str(loop_exe), "0.2", str(frames), str(out)] + (["fused"] if mode == "fused" else [])
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    assert "NoGrid: unsupported" in r.stdout and "device arrays set" in r.stdout, r.stdout
     data = np.fromfile(out, dtype=np.float32)
     sc = bcs.Scene(0, 0.2, seed=1)
     o = orc.Oracle(sc.physics, sc.types, sc.mesh)
     o.set_particles(sc.positions, sc.velocities, sc.forces)
+    o.calculate_grid(0)
+    keys, perm, *_ = o.get_grid(0)
+    key0 = int(keys[np.flatnonzero(perm == 0)[0]])
+    assert f"cell id of particle 0 {key0}\n" in r.stdout, r.stdout
+    assert f"vertex of triangle 5 corner 2 {int(sc.mesh.indices[17])}\n" in r.stdout
     o.step(frames)
     op, ov, of = o.get_particles()
     n3 = op.size

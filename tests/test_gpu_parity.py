@@ -428,3 +428,53 @@ def test_dynamic_triangle_grid_parity(bcs, orc, mode):
     o.step(20)
     for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
         assert_bits_equal(x, y, f"dynamic triangle grid ({mode}) " + n)
+
+
+def _hip_to_host(ptr, n, dtype):
+    """Copies n elements of a device array to a numpy array (hipMemcpy)."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    out = np.empty(n, dtype)
+    rc = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(out.nbytes),
+                       ctypes.c_int(2))  # hipMemcpyDeviceToHost
+    assert rc == 0, f"hipMemcpy failed: {rc}"
+    return out
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_grid_view_and_cell_ids_match_oracle(bcs, orc, which):
+    """The reference UniformGrid's public arrays: gridCellIds / particleIds of
+    the sorted grid, the dense gridCellStarts / gridCellEnds per cell id
+    (empty cells (0, -1)), and calculateCellId of the objects' positions."""
+    sc = _scene(bcs, "mixed")
+    sim, o = _pair(bcs, orc, sc)
+    sim.step(3)
+    o.step(3)
+    sim.calculate_grid(0)
+    o.calculate_grid(0)
+    keys, perm, okeys, ostart, oend = o.get_grid(which)
+    v = sim.grid_view(which, dense_ranges=True)
+    sim.synchronize()
+    n = v.object_count
+    assert n == keys.size
+    assert_bits_equal(_hip_to_host(v.grid_cell_ids, n, np.uint32), keys, "gridCellIds")
+    assert_bits_equal(_hip_to_host(v.particle_ids, n, np.int32), perm, "particleIds")
+    m = int(v.cell_count)
+    starts = _hip_to_host(v.grid_cell_starts, m, np.int32)
+    ends = _hip_to_host(v.grid_cell_ends, m, np.int32)
+    exp_s = np.zeros(m, np.int32)
+    exp_e = np.full(m, -1, np.int32)
+    exp_s[okeys] = ostart
+    exp_e[okeys] = oend
+    assert_bits_equal(starts, exp_s, "gridCellStarts")
+    assert_bits_equal(ends, exp_e, "gridCellEnds")
+    objs = o.get_particles()[0] if which == 0 else o.get_triangle_centers()
+    per_obj = np.empty(n, np.uint32)
+    per_obj[perm] = keys
+    if which == 0:
+        # positions moved since the build: recompute on the oracle's current grid
+        o.calculate_grid(0)
+        keys, perm, *_ = o.get_grid(0)
+        per_obj[perm] = keys
+    assert_bits_equal(sim.cell_ids(objs, which), per_obj, "calculateCellId")
