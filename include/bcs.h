@@ -314,6 +314,19 @@ int bcs_sim_get_grid(bcs_sim* sim, int32_t which, uint32_t* keys_sorted, int32_t
                      int32_t* n_occupied, uint32_t* occ_keys, int32_t* occ_start,
                      int32_t* occ_end);
 
+/* bcs_sim_get_grid for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc);
+ * no counterpart in the reference.  The same outputs, but it calls no
+ * hipMalloc or hipFree: the occupied-cell gather is staged in a static
+ * device array of the library (2^20 occupied cells, 8 MiB per GPU, held
+ * until the process ends).  keys_sorted, n_occupied and the three occ_*
+ * arrays are required (capacity n; they are the host working arrays).  More
+ * than 2^20 occupied cells: BCS_ERR_INVALID_ARGUMENT, nothing written.  All
+ * calls on one GPU share the static array: calls that overlap on one GPU
+ * (from two threads) race. */
+int bcs_sim_get_grid_static(bcs_sim* sim, int32_t which, uint32_t* keys_sorted, int32_t* perm,
+                            int32_t* n_occupied, uint32_t* occ_keys, int32_t* occ_start, int32_t* occ_end);
+
 /* Contact pairs of the particle-collision phase on the current grid and
  * state: pairs (i, j) with d^2 <= (r_i + r_j)^2 and d^2 >= 1e-4, in visit
  * order per i (src/simulation/particle_collisions.cuh:26-83).  counts[i] per

@@ -14,6 +14,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace bcs {
@@ -989,6 +991,79 @@ extern "C" int bcs_sim_get_grid(bcs_sim* s, int32_t which, uint32_t* keys_sorted
         if (occ_end) occ_end[q] = oend[q];
     }
     if (n_occupied) *n_occupied = no;
+    return BCS_OK;
+}
+
+// ---- bcs_sim_get_grid without allocation --------------------------------
+// The gather of cell-table starts above stages the occupied keys and the
+// gathered starts in two device buffers it allocates per call.  The variant
+// below stages them in this file-scope array instead: kStaticOccupied keys
+// followed by kStaticOccupied starts.  Every GPU the library runs on holds
+// its own copy until the process ends; host code reaches it only through the
+// per-device address taken with hipGetSymbolAddress (never through the
+// array's name, which in host code is a stand-in, not a device address).
+constexpr int32_t kStaticOccupied = 1 << 20;  // 8 MiB of staging per GPU
+static __device__ __align__(256) uint32_t scratch[2 * (size_t)kStaticOccupied];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[kScratchDevices] = {};  // per device, taken once
+
+// Address of `scratch` on the current device (looked up at the first call on
+// that device).
+static int scratch_ptr(int device, uint32_t** out)
+{
+    if (device < 0 || device >= kScratchDevices) return fail(BCS_ERR_UNSUPPORTED, "static staging: device index >= 64");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(BCS_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+        g_scratch_ptr[device] = (uint32_t*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_get_grid_static(bcs_sim* s, int32_t which, uint32_t* keys_sorted, int32_t* perm,
+                                       int32_t* n_occupied, uint32_t* occ_keys, int32_t* occ_start,
+                                       int32_t* occ_end)
+{
+    RC(set_dev(s));
+    if (which != BCS_GRID_PARTICLES && which != BCS_GRID_TRIANGLES) return fail(BCS_ERR_INVALID_ARGUMENT, "bad grid");
+    if (!keys_sorted || !n_occupied || !occ_keys || !occ_start || !occ_end)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_get_grid_static: keys_sorted and every occupied-cell output "
+                                              "are required (they are the working arrays)");
+    const DeviceGrid& G = which == BCS_GRID_PARTICLES ? s->pg : s->tg;
+    uint32_t* scr = nullptr;
+    RC(scratch_ptr(s->device, &scr));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int32_t n = G.n;
+    if (n) HIPCHK(hipMemcpyAsync(keys_sorted, G.keys_sorted, 4 * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+    if (perm && n) HIPCHK(hipMemcpyAsync(perm, G.perm, 4 * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // occupied cells: runs of equal sorted keys, in the caller's arrays
+    int32_t no = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (i == 0 || keys_sorted[i] != keys_sorted[i - 1]) occ_keys[no++] = keys_sorted[i];
+        occ_end[no - 1] = i;
+    }
+    if (no > kStaticOccupied) {
+        *n_occupied = 0;
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_get_grid_static: " + std::to_string(no) +
+                                                  " occupied cells exceed the static staging of " +
+                                                  std::to_string(kStaticOccupied) + "; use bcs_sim_get_grid");
+    }
+    if (no) {
+        uint32_t* dkeys = scr;
+        uint32_t* dstart = scr + kStaticOccupied;
+        HIPCHK(hipMemcpyAsync(dkeys, occ_keys, 4 * (size_t)no, hipMemcpyHostToDevice, s->stream));
+        launch_gather_start(G.table, dkeys, no, dstart, s->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(occ_start, dstart, 4 * (size_t)no, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    *n_occupied = no;
     return BCS_OK;
 }
 
