@@ -31,6 +31,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -40,6 +42,8 @@ WORKLOADS = {
     0: "config 0: 500 icosahedron cells (6,000 particles) in a straight cylinder r 40 x 400 (12,800 triangles), dt 0.01",
     1: "config 1: 20,000 icosahedron cells (240,000 particles) in a Y-bifurcating vein (trunk r 60 x 600, branches r 42 x 400 at +/-30 deg), dt 0.005",
     2: "config 2: 200,000 cells (70% 42-particle discs, 30% 12-particle icosahedra) in the Y vein scaled 3x, 35% lattice packing, dt 0.005",
+    3: "config 3: batched ensemble of 64 straight veins (radius U(20, 80), length 10 radius, 500-50,000 cells "
+       "log-uniform, 70% 42-particle discs / 30% 12-particle icosahedra), dt 0.005, one launch per frame kernel",
     4: "config 4 geometry: 1,000,000 42-particle discs in the config-2 vein, dt 0.005 (plain method)",
 }
 
@@ -167,6 +171,161 @@ def pmc_traffic(kernel, config_id):
         return None
 
 
+def bench_ensemble(bcs, args):
+    """Config 3: the 64 veins as one batched ensemble, autotuned, then timed;
+    beside it the same veins stepped one after another through their own
+    launches (bcs_sim_step per vein), the way separate processes of the
+    reference would run them on one GPU."""
+    n_veins = 64
+    sims = []
+    for k in range(n_veins):
+        sc = bcs.Scene(params=bcs.ensemble_scene_params(n_veins, k, seed=args.seed))
+        sims.append(bcs.Simulation.from_scene(sc, device=0))
+    np_total = sum(s.np for s in sims)
+    ens = bcs.Ensemble(sims)
+    tune = ens.autotune(frames_per_trial=args.tune_frames)
+    ens.step(max(args.warmup, 1))
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(args.steps, sync=False)
+    ens.synchronize()
+    wall = time.perf_counter() - t0
+    # the same veins, one solo step call each per round of frames
+    solo_frames = max(1, min(args.steps, 200))
+    for s in sims:
+        s.step(1)
+    for s in sims:
+        s.synchronize()
+    t1 = time.perf_counter()
+    for s in sims:
+        s.step(solo_frames, sync=False)
+    for s in sims:
+        s.synchronize()
+    solo_wall = time.perf_counter() - t1
+    ms_per_step = 1e3 * wall / args.steps
+    flags = 0
+    for s in sims:
+        flags |= s.error_flags()
+    return {
+        "metric": "particle-steps/s",
+        "value": np_total * args.steps / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator, seed %d)" % args.seed,
+        "config": {
+            "workload": WORKLOADS[3],
+            "frames_timed": f"{args.warmup + 1}..{args.warmup + args.steps} after {len(tune['trials'])} autotuning trials",
+            "veins": n_veins,
+            "particles": np_total,
+            "particles_per_vein_min_max": [min(s.np for s in sims), max(s.np for s in sims)],
+            "frame_launch": "hip-graph, batched kernels",
+        },
+        "autotune": tune,
+        "solo_launches": {
+            "frames": solo_frames,
+            "ms_per_frame_all_veins": 1e3 * solo_wall / solo_frames,
+            "particle_steps_per_s": np_total * solo_frames / solo_wall,
+            "note": "each vein stepped by its own bcs_sim_step call (its own graphs), veins one after another",
+        },
+        "ensemble_speedup_over_solo_launches": (1e3 * solo_wall / solo_frames) / ms_per_step,
+        "error_flags": flags,
+    }
+
+
+SURR_FLOP_PER_CELL = 2 * (294 * 256 + 256 * 256 + 256 * 256 + 256 * 126)  # 477,184
+MFMA_BF16_PEAK = 2.5e15  # MI355X dense BF16, FLOP/s
+
+
+def bench_surrogate(bcs, args):
+    """Config 4: the learned spring surrogate.  Training rows come from the
+    plain method on config 2 (the 140,000 disc cells, sampled twice after 20
+    frames: 200,000 rows kept); the network is trained on them, then deployed
+    on the config-4 geometry (1,000,000 discs, 42M particles) with the
+    per-cell check against the plain springs at tolerance 1e-3 of the cell
+    radius, for args.surr_frames frames after the plain frames."""
+    t_all = time.perf_counter()
+    src = bcs.Simulation.from_scene(bcs.Scene(2, 1.0, seed=args.seed), device=0)
+    src.step(20)
+    feats, targs = [], []
+    while sum(len(f) for f in feats) < 200000:
+        f, t = src.surrogate_sample(capacity=200000)
+        feats.append(f)
+        targs.append(t)
+    x = np.concatenate(feats)[:200000]
+    y = np.concatenate(targs)[:200000]
+    src.close()
+    net = bcs.Surrogate(seed=args.seed)
+    scales = (float(np.abs(x[:, :126]).std()), float(max(np.abs(x[:, 126:252]).std(), 1e-6)),
+              float(max(np.abs(y).std(), 1e-12)))
+    net.set_scales(*scales)
+    t0 = time.perf_counter()
+    loss = net.train(x, y, epochs=args.surr_epochs, batch=4096, lr=1e-3, seed=args.seed)
+    train_s = time.perf_counter() - t0
+    pred = net.predict(x[:20000])
+    rel_err = float(np.abs(pred - y[:20000]).max() / max(np.abs(y[:20000]).max(), 1e-30))
+
+    scene = bcs.Scene(4, 1.0, seed=args.seed)
+    ncell = scene.types[0].count
+    radius = 2.0
+    tol = 1e-3 * radius
+    out = {"training": {"rows": int(len(x)), "epochs": args.surr_epochs, "seconds": train_s,
+                        "loss_first_last": [float(loss[0]), float(loss[-1])], "scales": scales,
+                        "max_abs_error_over_max_target_on_20k_rows": rel_err}}
+    runs = {}
+    for mode in ("plain", "surrogate"):
+        sim = bcs.Simulation.from_scene(scene, device=0)
+        if mode == "surrogate":
+            sim.attach_surrogate(net, tol)
+        sim.step(max(args.warmup, 1))
+        if mode == "surrogate":
+            sim.surrogate_stats(reset=True)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step(args.surr_frames, sync=False)
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        r = {"frames": args.surr_frames, "ms_per_frame": 1e3 * wall / args.surr_frames,
+             "cell_steps_per_s": ncell * args.surr_frames / wall,
+             "particle_steps_per_s": scene.particle_count * args.surr_frames / wall}
+        if mode == "surrogate":
+            st = sim.surrogate_stats()
+            r.update(st)
+            r["tolerance"] = tol
+            r["accepted_fraction"] = st["accepted"] / max(st["accepted"] + st["fallback"], 1)
+        runs[mode] = r
+        sim.close()
+    ms = net.time_inference(ncell, reps=5)
+    inf = {"cells": ncell, "ms_per_pass": ms, "cell_steps_per_s": ncell / (ms * 1e-3),
+           "tflops": SURR_FLOP_PER_CELL * ncell / (ms * 1e-3) / 1e12,
+           "mfma_frac": SURR_FLOP_PER_CELL * ncell / (ms * 1e-3) / MFMA_BF16_PEAK}
+    out.update({
+        "metric": "cell-steps/s (42-particle cells, surrogate frames vs plain frames)",
+        "value": runs["surrogate"]["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "dtype": "bf16 network (fp32 accumulate), f32 step",
+        "data": "synthetic (seeded scene generator, seed %d)" % args.seed,
+        "config": {"workload": WORKLOADS[4] + ", learned springs", "cells": ncell,
+                   "particles": scene.particle_count},
+        "plain": runs["plain"],
+        "surrogate": runs["surrogate"],
+        "speedup_vs_plain": runs["surrogate"]["cell_steps_per_s"] / runs["plain"]["cell_steps_per_s"],
+        "inference": inf,
+        "roofline": {"bound": "mfma", "achieved": inf["tflops"], "peak": MFMA_BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": inf["mfma_frac"], "traffic": None,
+                     "flop_per_cell_step": SURR_FLOP_PER_CELL},
+        "seconds_total": time.perf_counter() - t_all,
+    })
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +337,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true",
                     help="enqueue the fused frame kernels directly instead of replaying the HIP graph")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="config 4 with the learned spring surrogate (trained on config-2 rows first)")
+    ap.add_argument("--surr-epochs", type=int, default=8)
+    ap.add_argument("--surr-frames", type=int, default=200)
+    ap.add_argument("--tune-frames", type=int, default=5,
+                    help="config 3: frames timed per autotuning trial")
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="also run the plain copy kernel (k_copy) over this many bytes, for HBM counter calibration")
     args = ap.parse_args()
@@ -190,6 +355,12 @@ def main():
     if bcs.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.config == 3:
+        print(json.dumps(bench_ensemble(bcs, args)))
+        return
+    if args.surrogate:
+        print(json.dumps(bench_surrogate(bcs, args)))
+        return
     scene = bcs.Scene(args.config, 1.0, seed=args.seed)
     sim = bcs.Simulation.from_scene(scene, device=0)
     if args.no_graph:

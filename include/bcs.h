@@ -381,6 +381,109 @@ int bcs_traj_open(bcs_sim* sim, const char* path, int32_t with_velocities, bcs_t
 int bcs_traj_append(bcs_traj* traj);
 int bcs_traj_close(bcs_traj* traj);
 
+/* ---- batched ensembles ---------------------------------------------------
+ * Many independent simulations (one vein each, of any size) stepped together:
+ * each frame runs every step kernel ONCE for all members -- a persistent grid
+ * whose blocks take the members' blocks in turn -- instead of once per
+ * member, and the frames are replayed from HIP graphs.  Every member's
+ * results are bit-identical to stepping it alone (bcs_sim_step).  No
+ * counterpart in the reference, which runs one vein per process
+ * (src/main.cu:102-138, src/simulation/simulation_controller.cu:246-331). */
+typedef struct bcs_ensemble bcs_ensemble;
+
+/* Launch shape of the batched kernels (results never depend on it). */
+typedef struct bcs_ensemble_tuning {
+    int32_t blocks_per_cu;  /* hardware blocks of each batched launch per compute unit
+                               (0: one hardware block per member block, no persistence) */
+    int32_t run;            /* consecutive member blocks a hardware block takes per turn (>= 1) */
+    int32_t ray_mode;       /* BCS_RAY_SPLIT or BCS_RAY_FUSED for every member
+                               (BCS_RAY_AUTO: fused when the members hold more than 4,096 cell blocks) */
+    int32_t largest_first;  /* 1: the blocks of larger members come first in each launch */
+} bcs_ensemble_tuning;
+
+/* The members stay owned by the caller and must outlive the ensemble; all on
+ * one device, with static triangle grids (dynamic_triangle_grid = 0).  While
+ * the ensemble exists, step the members only through it (their state is read
+ * and written as usual with bcs_sim_get_* / bcs_sim_set_* after
+ * bcs_ensemble_synchronize). */
+int bcs_ensemble_create(bcs_sim* const* members, int32_t n_members, bcs_ensemble** out);
+int bcs_ensemble_destroy(bcs_ensemble* ens);
+int bcs_ensemble_set_tuning(bcs_ensemble* ens, const bcs_ensemble_tuning* tuning);
+int bcs_ensemble_get_tuning(bcs_ensemble* ens, bcs_ensemble_tuning* tuning);
+/* n frames of every member (asynchronous; the same frames bcs_sim_step runs). */
+int bcs_ensemble_step(bcs_ensemble* ens, int32_t n_steps);
+int bcs_ensemble_synchronize(bcs_ensemble* ens);
+/* n frames timed with HIP events on the ensemble's stream (ms in all). */
+int bcs_ensemble_time_steps(bcs_ensemble* ens, int32_t n_steps, float* total_ms);
+/* Times frames_per_trial frames (after one untimed frame) under every tuning
+ * of a fixed search space -- blocks_per_cu in {0, 2, 4, 8, 16}, run in {1, 4},
+ * ray_mode in {SPLIT, FUSED}, largest_first in {0, 1} -- and keeps the
+ * fastest.  The members advance by all the frames run.  report (optional)
+ * receives one JSON object: the search space, every trial's ms per frame and
+ * the choice. */
+int bcs_ensemble_autotune(bcs_ensemble* ens, int32_t frames_per_trial, bcs_ensemble_tuning* best, char* report,
+                          int32_t report_capacity);
+
+/* ---- learned spring surrogate -------------------------------------------
+ * A per-cell MLP 294 -> 256 -> 256 -> 256 -> 126 (GELU) in BF16 on the matrix
+ * cores with FP32 accumulation, trained on frames of the plain method, that
+ * stands in for the spring sub-step (gatherForcesKernel,
+ * src/objects/blood_cells.cu:66-120; calculateParticlesSpringForceComponent,
+ * src/simulation/physics.cuh:53-78) of 42-particle cells.  No counterpart in
+ * the reference.  One row per cell:
+ *   input  (294)  centroid-relative positions (126: particle a, axis k at
+ *                 3a + k), velocities relative to the cell's mean velocity
+ *                 (126), contact flags of the previous frame (42: 1 = the
+ *                 particle touched another one)
+ *   output (126)  spring displacements d = (dt^2/4) S: the spring force S's
+ *                 share of the frame's Heun position update
+ * Rows are in physical units; the network works on them divided by the
+ * scales (positions, velocities; outputs by out_scale). */
+typedef struct bcs_surrogate bcs_surrogate;
+#define BCS_SURR_INPUTS 294
+#define BCS_SURR_HIDDEN 256
+#define BCS_SURR_OUTPUTS 126
+
+/* Weights uniform in +-1/sqrt(fan_in) from seed (SplitMix64). */
+int bcs_surrogate_create(int32_t device, uint64_t seed, bcs_surrogate** out);
+int bcs_surrogate_destroy(bcs_surrogate* net);
+/* weights[l]: [out][in] row-major fp32 (256 x 294, 256 x 256, 256 x 256,
+ * 126 x 256); biases[l]: out entries.  get_weights skips null entries. */
+int bcs_surrogate_set_weights(bcs_surrogate* net, const float* const* weights, const float* const* biases);
+int bcs_surrogate_get_weights(bcs_surrogate* net, float* const* weights, float* const* biases);
+int bcs_surrogate_set_scales(bcs_surrogate* net, float pos_scale, float vel_scale, float out_scale);
+int bcs_surrogate_get_scales(bcs_surrogate* net, float* pos_scale, float* vel_scale, float* out_scale);
+/* n rows of features (n x 294) -> displacements (n x 126), physical units. */
+int bcs_surrogate_predict(bcs_surrogate* net, const float* features, int32_t n, float* out);
+/* Mean-squared-error training with Adam (betas 0.9 / 0.999, eps 1e-8) on n
+ * samples in physical units, epochs passes in a seeded order, batches of
+ * `batch` rows (a multiple of 128).  loss_per_epoch (optional, epochs
+ * entries): mean squared error in scaled units. */
+int bcs_surrogate_train(bcs_surrogate* net, const float* features, const float* targets, int32_t n, int32_t epochs,
+                        int32_t batch, float lr, uint64_t seed, float* loss_per_epoch);
+/* ms per inference pass over n_cells rows, averaged over reps passes. */
+int bcs_surrogate_time_inference(bcs_surrogate* net, int32_t n_cells, int32_t reps, float* ms_per_pass);
+
+/* Frames of bcs_sim_step then give the first 42-particle type's springs
+ * from the network: each frame computes the network input from its starting
+ * state (and the previous frame's contact flags), runs the network, and in
+ * the cell kernel compares every cell's predicted displacements with the
+ * plain spring solve of the same state; a cell whose largest particle error
+ * is at most `tolerance` (a length) uses the prediction, any other cell the
+ * plain springs (with tolerance <= 0 every cell falls back and the frames
+ * equal plain frames bit for bit).  NULL detaches.  Attach again after
+ * changing the scales. */
+int bcs_sim_attach_surrogate(bcs_sim* sim, bcs_surrogate* net, float tolerance);
+/* Training rows from the plain method: runs two frames (the first keeps the
+ * contact flags) and returns, for up to `capacity` cells of the first
+ * 42-particle type, the second frame's network input (features, capacity x
+ * 294) and plain spring displacements (targets, capacity x 126). */
+int bcs_sim_surrogate_sample(bcs_sim* sim, float* features, float* targets, int32_t capacity, int32_t* n_rows);
+/* Cells that used the prediction / fell back since the last reset, and the
+ * largest cell error among the accepted / all predicted cells. */
+int bcs_sim_surrogate_stats(bcs_sim* sim, int32_t reset, uint64_t* accepted, uint64_t* fallback,
+                            float* max_error_accepted, float* max_error_all);
+
 /* Device error word (BCS_FLAG_*); clear != 0 resets it. */
 int bcs_sim_error_flags(bcs_sim* sim, int32_t clear, uint32_t* flags);
 

@@ -82,6 +82,14 @@ typedef struct bcs_scene_view {
 int bcs_scene_preset(int32_t preset, double cell_fraction, uint64_t seed, bcs_scene** out);
 int bcs_scene_default_params(int32_t preset, bcs_scene_params* out);
 int bcs_scene_build(const bcs_scene_params* params, bcs_scene** out);
+/* Parameters of vein `index` (0 <= index < n_veins) of a batched ensemble
+ * of straight veins: radius uniform in [20, 80], length 10 radius, cell
+ * counts log-uniform in [500, 50000] scaled by cell_fraction (70% 42-particle
+ * discs, 30% 12-particle icosahedra, 35% lattice packing), radii and counts
+ * paired by rank so that every vein holds its cells, dt 0.005; the other
+ * physics as preset 0.  Reproducible from seed. */
+int bcs_scene_ensemble_params(int32_t n_veins, int32_t index, uint64_t seed, double cell_fraction,
+                              bcs_scene_params* out);
 int bcs_scene_get_view(bcs_scene* scene, bcs_scene_view* out);
 int bcs_scene_destroy(bcs_scene* scene);
 

@@ -195,6 +195,15 @@ class SceneParams(C.Structure):
     ]
 
 
+class EnsembleTuning(C.Structure):
+    _fields_ = [
+        ("blocks_per_cu", C.c_int32),
+        ("run", C.c_int32),
+        ("ray_mode", C.c_int32),
+        ("largest_first", C.c_int32),
+    ]
+
+
 class SceneView(C.Structure):
     _fields_ = [
         ("physics", Physics),
@@ -244,6 +253,27 @@ def _sig(lib):
     lib.bcs_sim_get_contacts.argtypes = [vp, i32p, i32p, C.c_int64, P(C.c_int64)]
     lib.bcs_sim_error_flags.argtypes = [vp, C.c_int32, u32p]
     lib.bcs_sim_grid_view.argtypes = [vp, C.c_int32, C.c_int32, P(GridView)]
+    lib.bcs_ensemble_create.argtypes = [P(vp), C.c_int32, P(vp)]
+    lib.bcs_ensemble_destroy.argtypes = [vp]
+    lib.bcs_ensemble_set_tuning.argtypes = [vp, P(EnsembleTuning)]
+    lib.bcs_ensemble_get_tuning.argtypes = [vp, P(EnsembleTuning)]
+    lib.bcs_ensemble_step.argtypes = [vp, C.c_int32]
+    lib.bcs_ensemble_synchronize.argtypes = [vp]
+    lib.bcs_ensemble_time_steps.argtypes = [vp, C.c_int32, f32p]
+    lib.bcs_ensemble_autotune.argtypes = [vp, C.c_int32, P(EnsembleTuning), C.c_char_p, C.c_int32]
+    lib.bcs_scene_ensemble_params.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_double, P(SceneParams)]
+    lib.bcs_surrogate_create.argtypes = [C.c_int32, C.c_uint64, P(vp)]
+    lib.bcs_surrogate_destroy.argtypes = [vp]
+    lib.bcs_surrogate_set_weights.argtypes = [vp, P(f32p), P(f32p)]
+    lib.bcs_surrogate_get_weights.argtypes = [vp, P(f32p), P(f32p)]
+    lib.bcs_surrogate_set_scales.argtypes = [vp, C.c_float, C.c_float, C.c_float]
+    lib.bcs_surrogate_get_scales.argtypes = [vp, f32p, f32p, f32p]
+    lib.bcs_surrogate_predict.argtypes = [vp, f32p, C.c_int32, f32p]
+    lib.bcs_surrogate_train.argtypes = [vp, f32p, f32p, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64, f32p]
+    lib.bcs_surrogate_time_inference.argtypes = [vp, C.c_int32, C.c_int32, f32p]
+    lib.bcs_sim_attach_surrogate.argtypes = [vp, vp, C.c_float]
+    lib.bcs_sim_surrogate_sample.argtypes = [vp, f32p, f32p, C.c_int32, i32p]
+    lib.bcs_sim_surrogate_stats.argtypes = [vp, C.c_int32, P(C.c_uint64), P(C.c_uint64), f32p, f32p]
     lib.bcs_sim_cell_ids.argtypes = [vp, C.c_int32, f32p, C.c_int32, u32p]
     lib.bcs_sim_frame_stats.argtypes = [vp, u32p, u32p]
     lib.bcs_device_count.argtypes = [i32p]
@@ -564,6 +594,32 @@ class Simulation:
         check(self.lib.bcs_sim_cell_ids(self.h, which, fptr(p), n, uptr(out)), "cell_ids")
         return out[:n]
 
+    def attach_surrogate(self, net, tolerance):
+        """Springs of the 42-particle cells from `net` (a Surrogate) where a
+        cell's largest error against the plain solve is <= tolerance; None
+        detaches."""
+        self._surrogate = net
+        check(self.lib.bcs_sim_attach_surrogate(self.h, net.h if net is not None else None, C.c_float(tolerance)),
+              "attach_surrogate")
+
+    def surrogate_sample(self, capacity=None):
+        """Runs two frames; returns (features n x 294, targets n x 126) of the
+        second frame's 42-particle cells (network input and plain spring
+        displacements)."""
+        cap = int(capacity if capacity is not None else self.np)
+        feat = np.empty((max(cap, 1), 294), np.float32)
+        targ = np.empty((max(cap, 1), 126), np.float32)
+        n = C.c_int32(0)
+        check(self.lib.bcs_sim_surrogate_sample(self.h, fptr(feat), fptr(targ), cap, C.byref(n)), "surrogate_sample")
+        return feat[: n.value], targ[: n.value]
+
+    def surrogate_stats(self, reset=False) -> dict:
+        a, f = C.c_uint64(0), C.c_uint64(0)
+        ea, e_all = C.c_float(0), C.c_float(0)
+        check(self.lib.bcs_sim_surrogate_stats(self.h, 1 if reset else 0, C.byref(a), C.byref(f), C.byref(ea),
+                                               C.byref(e_all)), "surrogate_stats")
+        return {"accepted": a.value, "fallback": f.value, "max_error_accepted": ea.value, "max_error_all": e_all.value}
+
     def frame_stats(self):
         w, h = C.c_uint32(0), C.c_uint32(0)
         check(self.lib.bcs_sim_frame_stats(self.h, C.byref(w), C.byref(h)), "frame_stats")
@@ -583,6 +639,144 @@ class Simulation:
         f = C.c_uint32(0)
         check(self.lib.bcs_sim_error_flags(self.h, 1 if clear else 0, C.byref(f)), "error_flags")
         return f.value
+
+
+class Ensemble:
+    """Many simulations stepped together (bcs_ensemble): every frame kernel
+    runs once for all members.  The members stay usable as Simulation objects
+    (read their state after synchronize()); keep them alive meanwhile."""
+
+    def __init__(self, sims):
+        self.lib = load_library()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        self.h = C.c_void_p()
+        check(self.lib.bcs_ensemble_create(arr, len(self.sims), C.byref(self.h)), "bcs_ensemble_create")
+
+    def close(self):
+        if self.h:
+            self.lib.bcs_ensemble_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1, sync=True):
+        check(self.lib.bcs_ensemble_step(self.h, int(n)), "ensemble step")
+        if sync:
+            self.synchronize()
+
+    def synchronize(self):
+        check(self.lib.bcs_ensemble_synchronize(self.h), "ensemble synchronize")
+
+    def time_steps(self, n) -> float:
+        ms = C.c_float(0)
+        check(self.lib.bcs_ensemble_time_steps(self.h, int(n), C.byref(ms)), "ensemble time_steps")
+        return ms.value
+
+    def set_tuning(self, blocks_per_cu=0, run=1, ray_mode=RAY_AUTO, largest_first=1):
+        t = EnsembleTuning(int(blocks_per_cu), int(run), int(ray_mode), int(largest_first))
+        check(self.lib.bcs_ensemble_set_tuning(self.h, C.byref(t)), "ensemble set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = EnsembleTuning()
+        check(self.lib.bcs_ensemble_get_tuning(self.h, C.byref(t)), "ensemble get_tuning")
+        return {"blocks_per_cu": t.blocks_per_cu, "run": t.run, "ray_mode": t.ray_mode,
+                "largest_first": t.largest_first}
+
+    def autotune(self, frames_per_trial=10) -> dict:
+        """Times every tuning of the search space and keeps the fastest;
+        returns the report (search space, trials, choice)."""
+        import json
+
+        best = EnsembleTuning()
+        buf = C.create_string_buffer(16384)
+        check(self.lib.bcs_ensemble_autotune(self.h, int(frames_per_trial), C.byref(best), buf, len(buf)),
+              "ensemble autotune")
+        return json.loads(buf.value.decode())
+
+
+SURR_SHAPES = [(256, 294), (256, 256), (256, 256), (126, 256)]
+
+
+class Surrogate:
+    """The learned spring surrogate (bcs_surrogate): MLP 294-256-256-256-126
+    (GELU), BF16 on the matrix cores with FP32 accumulation."""
+
+    def __init__(self, seed=0, device=0):
+        self.lib = load_library()
+        self.h = C.c_void_p()
+        check(self.lib.bcs_surrogate_create(int(device), C.c_uint64(seed), C.byref(self.h)), "bcs_surrogate_create")
+
+    def close(self):
+        if self.h:
+            self.lib.bcs_surrogate_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self):
+        ws = [np.empty(sh, np.float32) for sh in SURR_SHAPES]
+        bs = [np.empty(sh[0], np.float32) for sh in SURR_SHAPES]
+        wp = (f32p * 4)(*[fptr(w) for w in ws])
+        bp = (f32p * 4)(*[fptr(b) for b in bs])
+        check(self.lib.bcs_surrogate_get_weights(self.h, wp, bp), "surrogate get_weights")
+        return ws, bs
+
+    def set_weights(self, ws, bs):
+        ws = [np.ascontiguousarray(w, np.float32) for w in ws]
+        bs = [np.ascontiguousarray(b, np.float32) for b in bs]
+        for w, b, sh in zip(ws, bs, SURR_SHAPES):
+            assert w.shape == sh and b.shape == (sh[0],)
+        wp = (f32p * 4)(*[fptr(w) for w in ws])
+        bp = (f32p * 4)(*[fptr(b) for b in bs])
+        check(self.lib.bcs_surrogate_set_weights(self.h, wp, bp), "surrogate set_weights")
+
+    def set_scales(self, pos_scale, vel_scale, out_scale):
+        check(self.lib.bcs_surrogate_set_scales(self.h, C.c_float(pos_scale), C.c_float(vel_scale),
+                                                C.c_float(out_scale)), "surrogate set_scales")
+
+    def get_scales(self):
+        a, b, c = C.c_float(0), C.c_float(0), C.c_float(0)
+        check(self.lib.bcs_surrogate_get_scales(self.h, C.byref(a), C.byref(b), C.byref(c)), "surrogate get_scales")
+        return a.value, b.value, c.value
+
+    def predict(self, features):
+        x = np.ascontiguousarray(features, np.float32).reshape(-1, 294)
+        out = np.empty((max(x.shape[0], 1), 126), np.float32)
+        check(self.lib.bcs_surrogate_predict(self.h, fptr(x), x.shape[0], fptr(out)), "surrogate predict")
+        return out[: x.shape[0]]
+
+    def train(self, features, targets, epochs=10, batch=4096, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(features, np.float32).reshape(-1, 294)
+        y = np.ascontiguousarray(targets, np.float32).reshape(-1, 126)
+        loss = np.zeros(max(epochs, 1), np.float32)
+        check(self.lib.bcs_surrogate_train(self.h, fptr(x), fptr(y), x.shape[0], int(epochs), int(batch),
+                                           C.c_float(lr), C.c_uint64(seed), fptr(loss)), "surrogate train")
+        return loss
+
+    def time_inference(self, n_cells, reps=10) -> float:
+        ms = C.c_float(0)
+        check(self.lib.bcs_surrogate_time_inference(self.h, int(n_cells), int(reps), C.byref(ms)),
+              "surrogate time_inference")
+        return ms.value
+
+
+def ensemble_scene_params(n_veins: int, index: int, seed: int = 0, cell_fraction: float = 1.0) -> SceneParams:
+    """Scene parameters of vein `index` of an n_veins batched ensemble
+    (bcs_scene_ensemble_params)."""
+    p = SceneParams()
+    check(load_library().bcs_scene_ensemble_params(int(n_veins), int(index), C.c_uint64(seed),
+                                                    C.c_double(cell_fraction), C.byref(p)),
+          "bcs_scene_ensemble_params")
+    return p
 
 
 class Trajectory:

@@ -35,7 +35,7 @@ DEVICE = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["host_tables.cpp", "scenes.cpp", "refconfig.cpp", "grid.hip", "step.hip", "sim.hip"]
+SOURCES = ["host_tables.cpp", "scenes.cpp", "refconfig.cpp", "grid.hip", "step.hip", "sim.hip", "surrogate.hip"]
 
 
 def _compile(src: Path, obj: Path) -> tuple[Path, str]:

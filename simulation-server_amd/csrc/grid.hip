@@ -14,7 +14,9 @@
 //              write the dense cell-start table
 //              buckets larger than the threshold are then shared out over the
 //              blocks: bitonic sort of the (key, index) pairs in LDS (up to
-//              kSortLds), else ranking by counting over LDS tiles
+//              kSortLds), else an LSD radix sort of the bucket in one block
+//              (8-bit digits of (key - bucket base) << index bits | index,
+//              stable passes through two global scratch arrays)
 // The optional 2-bit brick occupancy map (one word per 8-cell x-group):
 //   buckets of 2^8 cells or more: written by k_rank (the first run of every
 //              group writes the group's word; a group straddling two buckets
@@ -36,40 +38,42 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;  // per thread, loaded and stored as uint4
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kRankThreshold = 64;
-constexpr uint32_t kSortLds = 2048;  // bucket size sorted in LDS (larger ones: tiled counting)
+constexpr uint32_t kSortLds = 2048;  // bucket size sorted in LDS (larger ones: radix sort through global scratch)
 constexpr uint64_t kStatusAgg = 1ull << 62;
 constexpr uint64_t kStatusInc = 2ull << 62;
 constexpr uint64_t kSpinLimit = 1ull << 22;
 
-__global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict__ pos, int64_t stride, int32_t n,
-                                             int32_t n_prev, int shift, uint32_t* __restrict__ keys,
-                                             uint2* __restrict__ table, int clear_table, BrickMap bm,
-                                             const uint32_t* __restrict__ gpos, uint32_t* __restrict__ bcount,
-                                             uint64_t* __restrict__ status, int32_t ntiles,
-                                             uint32_t* __restrict__ ctrl, uint32_t* __restrict__ flags,
-                                             uint32_t* __restrict__ arrival, const int2* __restrict__ order,
-                                             int32_t norder)
+__device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
 {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t t = gid; t < ntiles; t += nthreads) status[t] = 0ull;
+    const GridGeom& g = a.g;
+    const float* __restrict__ pos = a.pos;
+    const int64_t stride = a.stride;
+    const int32_t n = a.n, n_prev = a.n_prev;
+    uint32_t* __restrict__ keys = a.keys;
+    uint2* __restrict__ table = a.table;
+    const BrickMap& bm = a.bm;
+    const uint32_t* __restrict__ gpos = a.gpos;
+    const int2* __restrict__ order = a.order;
+    const int64_t gid = (int64_t)vb.id * blockDim.x + threadIdx.x;
+    const int64_t nthreads = (int64_t)vb.count * blockDim.x;
+    for (int64_t t = gid; t < a.ntiles; t += nthreads) a.status[t] = 0ull;
     if (gid == 0) {
-        ctrl[0] = 0u;  // scan ticket
-        ctrl[1] = 0u;  // big-bucket count
-        ctrl[2] = 0u;  // occupied cells (run list)
+        a.ctrl[0] = 0u;  // scan ticket
+        a.ctrl[1] = 0u;  // big-bucket count
+        a.ctrl[2] = 0u;  // occupied cells (run list)
     }
     // every load first (one round trip), then the clears and the new key
     uint32_t old = 0xFFFFFFFFu, w = kNoWord;
     float x = 0.f, y = 0.f, z = 0.f;
     if (gid < n_prev) {  // the previous build's cell of this object
-        if (clear_table) old = keys[gid];
+        if (a.clear_table) old = keys[gid];
         if (gpos) w = gpos[gid];  // map word written by the previous build (indexed by its sorted slots)
     }
     // the object of this thread: gid, or with `order` the object of lane gid
     // (objects in spatial order, so that neighbouring lanes count into
     // neighbouring buckets; -1: no object)
     int64_t i = gid < n ? gid : -1;
-    if (order) i = gid < norder ? (int64_t)order[gid].x : -1;
+    if (order) i = gid < a.norder ? (int64_t)order[gid].x : -1;
     const bool have = i >= 0 && i < n;
     if (have) {
         x = pos[i];
@@ -84,22 +88,29 @@ __global__ void __launch_bounds__(256) k_key(GridGeom g, const float* __restrict
         keys[i] = k;
         // the count before this object: its slot within the bucket (the
         // scatter needs no atomics; slot order within a bucket is free)
-        arrival[i] = atomicAdd(&bcount[k >> shift], 1u);
+        a.arrival[i] = atomicAdd(&a.bcount[k >> a.shift], 1u);
     }
-    if (fl) atomicOr(flags, fl);
+    if (fl) atomicOr(a.flags, fl);
 }
+__global__ void __launch_bounds__(256) k_key(KeyArgs a) { key_body(a, hw_block()); }
 
 // Exclusive scan of `count[0..m)` into out[0..m] (out[m] = total), single
 // pass with decoupled look-back over tiles taken in ticket order.  Entries
 // larger than big_thr are appended to biglist (when biglist != nullptr).
 // clear_count: the histogram is zeroed as it is read (ready for the next
 // build).
-__global__ void __launch_bounds__(kScanThreads) k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ out,
-                                                       int64_t m, uint64_t* __restrict__ status, int32_t ntiles,
-                                                       uint32_t* __restrict__ ctrl, uint32_t* __restrict__ biglist,
-                                                       uint32_t big_thr, uint32_t big_cap,
-                                                       uint32_t* __restrict__ flags, int clear_count)
+__device__ __forceinline__ void scan_body(const ScanArgs& a)
 {
+    uint32_t* __restrict__ count = a.count;
+    uint32_t* __restrict__ out = a.out;
+    const int64_t m = a.m;
+    uint64_t* __restrict__ status = a.status;
+    const int32_t ntiles = a.ntiles;
+    uint32_t* __restrict__ ctrl = a.ctrl;
+    uint32_t* __restrict__ biglist = a.biglist;
+    const uint32_t big_thr = a.big_thr, big_cap = a.big_cap;
+    uint32_t* __restrict__ flags = a.flags;
+    const int clear_count = a.clear_count;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_prefix;
@@ -224,17 +235,21 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(uint32_t* __restrict__ co
             if (base + k <= m) out[base + k] = o[k];
     }
 }
+// (the tile comes from the ticket counter, not from the block index)
+__global__ void __launch_bounds__(kScanThreads) k_scan(ScanArgs a) { scan_body(a); }
 
 // brec (nullable): the object's position and velocity travel with its item
 // (read here in object order, copied to the sorted slot by k_rank).
-__global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ keys, int32_t n, int shift,
-                                                 const uint32_t* __restrict__ bstart,
-                                                 const uint32_t* __restrict__ arrival, uint2* __restrict__ items,
-                                                 uint32_t* __restrict__ flags, const float* __restrict__ xyz,
-                                                 const float* __restrict__ vel, int64_t stride,
-                                                 float4* __restrict__ brec)
+__device__ __forceinline__ void scatter_body(const ScatterArgs& a, VBlock vb)
 {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t* __restrict__ keys = a.keys;
+    const int32_t n = a.n;
+    const uint32_t* __restrict__ bstart = a.bstart;
+    const float* __restrict__ xyz = a.xyz;
+    const float* __restrict__ vel = a.vel;
+    const int64_t stride = a.stride;
+    float4* __restrict__ brec = a.brec;
+    const int32_t i = vb.id * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 rx, rv;
     if (brec) {
@@ -242,34 +257,22 @@ __global__ void __launch_bounds__(256) k_scatter(const uint32_t* __restrict__ ke
         rv = make_float4(vel[i], vel[stride + i], vel[2 * stride + i], 0.0f);
     }
     const uint32_t k = keys[i];
-    const uint32_t a = arrival[i];
-    const uint32_t b = k >> shift;
+    const uint32_t arr = a.arrival[i];
+    const uint32_t b = k >> a.shift;
     const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
-    const uint32_t pos = s0 + a;
+    const uint32_t pos = s0 + arr;
     if (pos >= s1 || s1 > (uint32_t)n) {  // inconsistent histogram: never write out of bounds
-        atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);
+        atomicOr(a.flags, BCS_FLAG_SCAN_TIMEOUT);
         return;
     }
-    items[pos] = make_uint2(k, (uint32_t)i);
+    a.items[pos] = make_uint2(k, (uint32_t)i);
     if (brec) {
         brec[2 * (size_t)pos] = rx;
         brec[2 * (size_t)pos + 1] = rv;
     }
 }
+__global__ void __launch_bounds__(256) k_scatter(ScatterArgs a) { scatter_body(a, hw_block()); }
 
-struct RankOut {
-    uint32_t* keys_sorted;
-    int32_t* perm;
-    int32_t* slot_of;      // nullable
-    uint2* table;          // {start, count} per cell
-    uint32_t* runs;        // run starts of occupied cells, in no particular order (nullable)
-    uint32_t* nruns;
-    const float4* brec;    // per-item records (nullable) ...
-    float4* pvs;           // ... copied to their sorted slots
-    BrickMap bm;           // occupancy map (bm.words nullable) ...
-    uint32_t* gpos;        // ... and the word each sorted slot wrote (kNoWord: none)
-    int shift;             // bucket = key >> shift
-};
 
 // The 8-cell x-group of `key` (cells x&~7 .. x|7 of its row): keys [gs, ge)
 // and the map word holding their 2-bit codes.
@@ -361,17 +364,105 @@ __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32
     rank_write(o, key, idx, pos, first, count, ra, rb);
 }
 
-constexpr int kBigChunk = 512;
+// Exclusive scan of one value per thread over a 256-thread block (wave64
+// shuffles, then the four wave totals through tmp[0..3]).
+__device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t* tmp)
+{
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63u) tmp[wave] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wave; ++w) add += tmp[w];
+    __syncthreads();  // tmp is reused by the next scan
+    return x - v + add;
+}
 
-__device__ __forceinline__ void big_buckets(uint64_t* sv, uint16_t* sq, uint32_t* sk, int32_t* si,
-                                            const uint2* __restrict__ items,
-                                             int32_t n, int shift, const uint32_t* __restrict__ bstart,
+// One block sorts the m items of one bucket (base = its first cell key) by
+// the composite c = (key - base) << ibits | index, which orders exactly like
+// the stable sort by key (indices are distinct): least significant 8-bit
+// digit first, each pass stable -- a digit histogram in LDS, then tiles of
+// 256 items in order, each item placed after the items of its digit in
+// earlier tiles, earlier waves of its tile (LDS counts) and lower lanes of its
+// wave (ballot match on the 8 digit bits).  Every pass reads what other
+// threads of this block wrote in the previous one; the block barriers order
+// those global accesses.  Returns the pass parity holding the result.
+__device__ __forceinline__ int radix_bucket(uint32_t* lds, const uint2* __restrict__ items, uint32_t s0, uint32_t m,
+                                            uint32_t base, const RankOut& o)
+{
+    uint32_t* off = lds;         // [256] running digit offsets
+    uint32_t* wc = lds + 256;    // [4][256] digit counts of each wave of the current tile
+    uint32_t* tmp = lds + 1280;  // [4] scan scratch
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const int ib = o.ibits;
+    const int passes = (o.shift + ib + 7) / 8;
+    for (uint32_t t = tid; t < m; t += blockDim.x) {
+        const uint2 it = items[s0 + t];
+        o.rk[0][s0 + t] = ((uint64_t)(it.x - base) << ib) | (uint64_t)it.y;
+        o.rq[0][s0 + t] = t;
+    }
+    for (uint32_t i = tid; i < 1024u; i += blockDim.x) wc[i] = 0u;
+    __syncthreads();
+    for (int p = 0; p < passes; ++p) {
+        const uint64_t* __restrict__ sk = o.rk[p & 1];
+        const uint32_t* __restrict__ sq = o.rq[p & 1];
+        uint64_t* __restrict__ dk = o.rk[(p + 1) & 1];
+        uint32_t* __restrict__ dq = o.rq[(p + 1) & 1];
+        const int sh = 8 * p;
+        off[tid] = 0u;
+        __syncthreads();
+        for (uint32_t t = tid; t < m; t += blockDim.x) atomicAdd(&off[(uint32_t)(sk[s0 + t] >> sh) & 255u], 1u);
+        __syncthreads();
+        const uint32_t start = block_scan256(off[tid], tmp);
+        off[tid] = start;
+        __syncthreads();
+        for (uint32_t tile = 0; tile < m; tile += blockDim.x) {
+            const uint32_t t = tile + tid;
+            const bool valid = t < m;
+            const uint64_t k = valid ? sk[s0 + t] : 0ull;
+            const uint32_t q = valid ? sq[s0 + t] : 0u;
+            const uint32_t d = (uint32_t)(k >> sh) & 255u;
+            uint64_t same = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint32_t bit = (d >> b) & 1u;
+                const uint64_t bb = __ballot(bit);
+                same &= bit ? bb : ~bb;
+            }
+            const uint32_t below = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+            if (valid && below == 0u) wc[wave * 256u + d] = (uint32_t)__popcll(same);
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = off[d] + below;
+                for (uint32_t w = 0; w < wave; ++w) pos += wc[w * 256u + d];
+                dk[s0 + pos] = k;
+                dq[s0 + pos] = q;
+            }
+            __syncthreads();
+            off[tid] += wc[tid] + wc[256u + tid] + wc[512u + tid] + wc[768u + tid];
+            wc[tid] = 0u;
+            wc[256u + tid] = 0u;
+            wc[512u + tid] = 0u;
+            wc[768u + tid] = 0u;
+            __syncthreads();
+        }
+    }
+    return passes & 1;
+}
+
+__device__ __forceinline__ void big_buckets(uint64_t* sv, uint16_t* sq, const uint2* __restrict__ items,
+                                             int32_t n, const uint32_t* __restrict__ bstart,
                                              const uint32_t* __restrict__ biglist, const uint32_t* __restrict__ ctrl,
-                                             uint32_t big_cap, RankOut o)
+                                             uint32_t big_cap, const RankOut& o, VBlock vb)
 {
     const uint32_t nbig = min(ctrl[1], big_cap);
     const uint32_t tid = threadIdx.x;
-    for (uint32_t e = blockIdx.x; e < nbig; e += gridDim.x) {
+    for (uint32_t e = vb.id; e < nbig; e += vb.count) {
         const uint32_t b = biglist[e];
         const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
         if (s1 < s0 || s1 > (uint32_t)n) continue;
@@ -444,44 +535,34 @@ __device__ __forceinline__ void big_buckets(uint64_t* sv, uint16_t* sq, uint32_t
             __syncthreads();  // sv is reused by the next bucket
             continue;
         }
-        for (uint32_t gbase = s0; gbase < s1; gbase += blockDim.x) {
-            const uint32_t q = gbase + tid;
-            const bool own = q < s1;
-            const uint2 me = own ? items[q] : make_uint2(0u, 0u);
-            const uint32_t key = me.x;
-            const int32_t idx = (int32_t)me.y;
-            uint32_t rank = 0, same = 0;
-            bool first = true;
-            for (uint32_t cbase = s0; cbase < s1; cbase += kBigChunk) {
-                const uint32_t cn = min((uint32_t)kBigChunk, s1 - cbase);
-                __syncthreads();
-                for (uint32_t t = tid; t < cn; t += blockDim.x) {
-                    const uint2 it = items[cbase + t];
-                    sk[t] = it.x;
-                    si[t] = (int32_t)it.y;
+        // larger buckets: radix sort through the global scratch, then every
+        // slot writes its object; a run start finds its run's end by binary
+        // search over the sorted composites
+        const uint32_t base = b << o.shift;
+        const int par = radix_bucket(reinterpret_cast<uint32_t*>(sv), items, s0, m, base, o);
+        const uint64_t* __restrict__ fk = o.rk[par];
+        const uint32_t* __restrict__ fq = o.rq[par];
+        const int ib = o.ibits;
+        for (uint32_t t = tid; t < m; t += blockDim.x) {
+            const uint64_t c = fk[s0 + t];
+            const uint32_t kl = (uint32_t)(c >> ib);
+            const uint32_t key = base + kl;
+            const int32_t idx = (int32_t)(uint32_t)(c & ((1ull << ib) - 1ull));
+            const bool first = t == 0 || (uint32_t)(fk[s0 + t - 1] >> ib) != kl;
+            uint32_t count = 0;
+            if (first) {
+                uint32_t lo = t + 1, hi = m;  // first position with a larger key
+                while (lo < hi) {
+                    const uint32_t mid = lo + ((hi - lo) >> 1);
+                    if ((uint32_t)(fk[s0 + mid] >> ib) > kl) hi = mid;
+                    else lo = mid + 1;
                 }
-                __syncthreads();
-                if (own) {
-                    for (uint32_t t = 0; t < cn; ++t) {
-                        const uint32_t k2 = sk[t];
-                        if (k2 < key) {
-                            ++rank;
-                        } else if (k2 == key) {
-                            ++same;
-                            if (si[t] < idx) {
-                                ++rank;
-                                first = false;
-                            }
-                        }
-                    }
-                }
+                count = lo - t;
             }
-            if (own) {
-                rank_write(o, key, idx, s0 + rank, first, same, q);
-                map_write(o, key, s0 + rank, first, same, false, false, 0u, 0u);
-            }
+            rank_write(o, key, idx, s0 + t, first, count, s0 + fq[s0 + t]);
+            map_write(o, key, s0 + t, first, count, false, false, 0u, 0u);
         }
-        __syncthreads();
+        __syncthreads();  // the scratch and LDS are reused by the next bucket
     }
 }
 
@@ -523,16 +604,17 @@ __device__ __forceinline__ void count_rank(Get get, uint32_t s0, uint32_t s1, ui
 // share out the larger buckets (big_buckets).  The small buckets of a block's
 // items lie within kRankThreshold items of the block's range; that window is
 // read once into LDS.
-__global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items, int32_t n, int shift,
-                                              const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ biglist,
-                                              const uint32_t* __restrict__ ctrl, uint32_t big_cap, RankOut o)
+__device__ __forceinline__ void rank_body(const RankArgs& R, VBlock vb)
 {
-    __shared__ uint64_t sv[kSortLds];  // the item window first, then the big-bucket sort
+    const uint2* __restrict__ items = R.items;
+    const int32_t n = R.n;
+    const int shift = R.shift;
+    const uint32_t* __restrict__ bstart = R.bstart;
+    const RankOut& o = R.o;
+    __shared__ uint64_t sv[kSortLds];  // the item window first, then the big-bucket sorts
     __shared__ uint16_t sq[kSortLds];
-    __shared__ uint32_t sk[kBigChunk];
-    __shared__ int32_t si[kBigChunk];
     static_assert(256 + 2 * kRankThreshold <= kSortLds, "item window exceeds the sort buffer");
-    const int32_t B = blockIdx.x * blockDim.x;
+    const int32_t B = vb.id * blockDim.x;
     const int32_t q = B + threadIdx.x;
     const int32_t w0 = max(B - (int32_t)kRankThreshold, 0);
     const int32_t w1 = min(B + (int32_t)blockDim.x + (int32_t)kRankThreshold, n);
@@ -581,19 +663,22 @@ __global__ void __launch_bounds__(256) k_rank(const uint2* __restrict__ items, i
         }
     }
     __syncthreads();  // the window's LDS is reused by the big-bucket sort
-    big_buckets(sv, sq, sk, si, items, n, shift, bstart, biglist, ctrl, big_cap, o);
+    big_buckets(sv, sq, items, n, bstart, R.biglist, R.ctrl, R.big_cap, o, vb);
 }
+__global__ void __launch_bounds__(256) k_rank(RankArgs R) { rank_body(R, hw_block()); }
 
 // Brick occupancy map: the first sorted slot of each 8-cell x-group (cells
 // x&~7 .. x|7 of one row) builds the group's 2-bit codes from the sorted keys
 // and writes its word; every slot records the word it wrote (or kNoWord) so
 // that the next build can clear exactly these words.
-__global__ void __launch_bounds__(256) k_occ(const uint32_t* __restrict__ keys_sorted, int32_t n, int32_t cap,
-                                             const uint2* __restrict__ table, BrickMap bm,
-                                             uint32_t* __restrict__ gpos)
+__device__ __forceinline__ void occ_body(const OccArgs& a, VBlock vb)
 {
-    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= cap) return;
+    const uint32_t* __restrict__ keys_sorted = a.keys_sorted;
+    const int32_t n = a.n;
+    const uint2* __restrict__ table = a.table;
+    const BrickMap& bm = a.bm;
+    const int32_t u = vb.id * blockDim.x + threadIdx.x;
+    if (u >= a.cap) return;
     uint32_t w = kNoWord;
     if (u < n) {
         const uint32_t key = keys_sorted[u];
@@ -623,7 +708,38 @@ __global__ void __launch_bounds__(256) k_occ(const uint32_t* __restrict__ keys_s
             bm.words[w] = (uint16_t)code;
         }
     }
-    gpos[u] = w;
+    a.gpos[u] = w;
+}
+__global__ void __launch_bounds__(256) k_occ(OccArgs a) { occ_body(a, hw_block()); }
+
+// ---- batched launches: one launch per kernel for many simulations ----------
+// a[k]: the arguments of simulation k; start: its virtual-block offsets.
+
+__global__ void __launch_bounds__(256) k_key_batch(const KeyArgs* __restrict__ a, const uint32_t* __restrict__ start,
+                                                   int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { key_body(a[k], vb); });
+}
+__global__ void __launch_bounds__(kScanThreads) k_scan_batch(const ScanArgs* __restrict__ a,
+                                                             const uint32_t* __restrict__ start, int32_t nsim,
+                                                             uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock) { scan_body(a[k]); });
+}
+__global__ void __launch_bounds__(256) k_scatter_batch(const ScatterArgs* __restrict__ a,
+                                                       const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { scatter_body(a[k], vb); });
+}
+__global__ void __launch_bounds__(256) k_rank_batch(const RankArgs* __restrict__ a, const uint32_t* __restrict__ start,
+                                                    int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { rank_body(a[k], vb); });
+}
+__global__ void __launch_bounds__(256) k_occ_batch(const OccArgs* __restrict__ a, const uint32_t* __restrict__ start,
+                                                   int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { occ_body(a[k], vb); });
 }
 
 // ------------------------------------------------------------------ host --
@@ -682,6 +798,12 @@ int grid_alloc(DeviceGrid& G, const GridGeom& geom, int32_t cap, bool with_slot,
     }
     GRID_HIP(hipMalloc(&G.perm, n * 4));
     if (with_slot) GRID_HIP(hipMalloc(&G.slot_of, n * 4));
+    for (int k = 0; k < 2; ++k) {
+        GRID_HIP(hipMalloc(&G.rk[k], n * 8));
+        GRID_HIP(hipMalloc(&G.rq[k], n * 4));
+    }
+    G.ibits = 1;
+    while (G.ibits < 31 && (1ull << G.ibits) < (uint64_t)n) ++G.ibits;
     GRID_HIP(hipMemsetAsync(G.keys, 0xFF, n * 4, st));
     hipError_t e = hipMalloc(&G.table, geom.cell_count * 8);
     if (e != hipSuccess)
@@ -781,16 +903,22 @@ int grid_shift_for(const GridGeom& g, const float* pos, int64_t stride, int32_t 
 void grid_free(DeviceGrid& G)
 {
     void* ptrs[] = {G.keys, G.items, G.keys_sorted, G.runs, G.perm, G.slot_of, G.brec, G.pvs, G.arrival,
-                    G.table, G.bm.words, G.gpos, G.bcount, G.bstart, G.status, G.ctrl, G.biglist};
+                    G.table, G.bm.words, G.gpos, G.bcount, G.bstart, G.status, G.ctrl, G.biglist,
+                    G.rk[0], G.rk[1], G.rq[0], G.rq[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     G = DeviceGrid{};
 }
 
-int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
-               hipStream_t st, const KernelTimer* timer, const int2* order, int32_t norder)
+int grid_launch_args(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
+                     const int2* order, int32_t norder, GridLaunch* L)
 {
     if (n < 0 || n > G.cap) return fail(BCS_ERR_INVALID_ARGUMENT, "grid: object count exceeds capacity");
+    // With clear_table, lane gid clears the cell of object gid while, with an
+    // order, it writes the key of another object: one lane could then read a
+    // key that another has already replaced.  The two are never combined.
+    if (order && G.clear_table)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "grid: a lane order needs a grid without table clearing");
     // Every slot below the capacity clears its previous cell (slots never
     // built hold the sentinel key): nothing here depends on the previous
     // build's size, so a captured graph of this sequence stays valid.
@@ -798,36 +926,67 @@ int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const
     G.n = n;
     const int blocks = (std::max(std::max(n, n_prev), 1) + 255) / 256;
     const int key_blocks = std::max(blocks, order ? (norder + 255) / 256 : 0);
+    *L = GridLaunch{};
+    L->key = KeyArgs{G.g, pos, stride, n, n_prev, G.shift, G.clear_table ? 1 : 0, G.keys, G.table, G.bm, G.gpos,
+                     G.bcount, G.status, G.ntiles, G.ctrl, flags, G.arrival, order, norder};
+    L->key_blocks = (uint32_t)std::max(key_blocks, (G.ntiles + 255) / 256);
+    L->scan = ScanArgs{G.bcount, G.bstart, (int64_t)G.nb, G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold,
+                       G.big_cap, flags, 1};
+    L->scan_blocks = (uint32_t)G.ntiles;
+    float4* brec = vel ? G.brec : nullptr;
+    L->scatter = ScatterArgs{G.keys, n, G.shift, G.bstart, G.arrival, G.items, flags, pos, vel, stride, brec};
+    L->scatter_blocks = (uint32_t)blocks;
+    const bool rank_map = grid_map_in_rank(G);
+    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr,
+              G.bm, rank_map ? G.gpos : nullptr, G.shift, G.ibits, {G.rk[0], G.rk[1]}, {G.rq[0], G.rq[1]}};
+    L->rank = RankArgs{G.items, n, G.shift, G.bstart, G.biglist, G.ctrl, G.big_cap, o};
+    L->rank_blocks = (uint32_t)blocks;
+    L->occ = OccArgs{G.keys_sorted, n, G.cap, G.table, G.bm, G.gpos};
+    L->occ_blocks = (G.gpos && !rank_map) ? (uint32_t)((std::max(G.cap, 1) + 255) / 256) : 0u;
+    return BCS_OK;
+}
+
+int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
+               hipStream_t st, const KernelTimer* timer, const int2* order, int32_t norder)
+{
+    GridLaunch L;
+    const int rc = grid_launch_args(G, pos, stride, n, vel, flags, order, norder, &L);
+    if (rc) return rc;
     auto tb = [&](int id) { if (timer) timer->begin(id, st); };
     auto te = [&](int id) { if (timer) timer->end(id, st); };
     tb(0);
-    hipLaunchKernelGGL(k_key, dim3(std::max(key_blocks, (G.ntiles + 255) / 256)), dim3(256), 0, st, G.g, pos, stride,
-                       n, n_prev, G.shift, G.keys, G.table, G.clear_table ? 1 : 0, G.bm, G.gpos, G.bcount, G.status,
-                       G.ntiles, G.ctrl, flags, G.arrival, order, norder);
+    hipLaunchKernelGGL(k_key, dim3(L.key_blocks), dim3(256), 0, st, L.key);
     te(0);
     tb(1);
-    hipLaunchKernelGGL(k_scan, dim3(G.ntiles), dim3(kScanThreads), 0, st, G.bcount, G.bstart, (int64_t)G.nb,
-                       G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold, G.big_cap, flags, 1);
+    hipLaunchKernelGGL(k_scan, dim3(L.scan_blocks), dim3(kScanThreads), 0, st, L.scan);
     te(1);
     tb(2);
-    float4* brec = vel ? G.brec : nullptr;
-    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, st, G.keys, n, G.shift, G.bstart, G.arrival, G.items,
-                       flags, pos, vel, stride, brec);
+    hipLaunchKernelGGL(k_scatter, dim3(L.scatter_blocks), dim3(256), 0, st, L.scatter);
     te(2);
-    const bool rank_map = grid_map_in_rank(G);
-    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr,
-              G.bm, rank_map ? G.gpos : nullptr, G.shift};
     tb(3);
-    hipLaunchKernelGGL(k_rank, dim3(blocks), dim3(256), 0, st, G.items, n, G.shift, G.bstart, G.biglist, G.ctrl,
-                       G.big_cap, o);
+    hipLaunchKernelGGL(k_rank, dim3(L.rank_blocks), dim3(256), 0, st, L.rank);
     te(3);
     tb(4);
-    if (G.gpos && !rank_map)
-        hipLaunchKernelGGL(k_occ, dim3((std::max(G.cap, 1) + 255) / 256), dim3(256), 0, st, G.keys_sorted, n, G.cap,
-                           G.table, G.bm, G.gpos);
+    if (L.occ_blocks) hipLaunchKernelGGL(k_occ, dim3(L.occ_blocks), dim3(256), 0, st, L.occ);
     te(4);
     GRID_HIP(hipGetLastError());
     return BCS_OK;
+}
+
+void grid_build_batch(const GridBatch& b, hipStream_t st)
+{
+    if (b.grid[0]) hipLaunchKernelGGL(k_key_batch, dim3(b.grid[0]), dim3(256), 0, st, b.key, b.start[0], b.nsim, b.run);
+    if (b.grid[1])
+        hipLaunchKernelGGL(k_scan_batch, dim3(b.grid[1]), dim3(kScanThreads), 0, st, b.scan, b.start[1], b.nsim, b.run);
+    if (b.grid[2])
+        hipLaunchKernelGGL(k_scatter_batch, dim3(b.grid[2]), dim3(256), 0, st, b.scatter, b.start[2], b.nsim, b.run);
+    if (b.grid[3]) hipLaunchKernelGGL(k_rank_batch, dim3(b.grid[3]), dim3(256), 0, st, b.rank, b.start[3], b.nsim, b.run);
+    if (b.grid[4]) hipLaunchKernelGGL(k_occ_batch, dim3(b.grid[4]), dim3(256), 0, st, b.occ, b.start[4], b.nsim, b.run);
+}
+
+void scan_batch(const ScanArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run, hipStream_t st)
+{
+    if (grid) hipLaunchKernelGGL(k_scan_batch, dim3(grid), dim3(kScanThreads), 0, st, a, start, nsim, run);
 }
 
 // Generic exclusive scan used by the vein-hit segments (same kernel).
@@ -835,9 +994,8 @@ int scan_exclusive(const uint32_t* count, uint32_t* out, int64_t m, uint64_t* st
                    uint32_t* flags, hipStream_t st)
 {
     // count is only read here (clear_count = 0)
-    hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, const_cast<uint32_t*>(count), out, m, status,
-                       ntiles, ctrl,
-                       (uint32_t*)nullptr, 0u, 0u, flags, 0);
+    const ScanArgs a{const_cast<uint32_t*>(count), out, m, status, ntiles, ctrl, nullptr, 0u, 0u, flags, 0};
+    hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(kScanThreads), 0, st, a);
     GRID_HIP(hipGetLastError());
     return BCS_OK;
 }

@@ -474,6 +474,50 @@ extern "C" int bcs_scene_default_params(int32_t preset, bcs_scene_params* out)
     return BCS_OK;
 }
 
+// Vein `index` of an ensemble of n_veins straight veins: radius uniform in
+// [20, 80], length 10 radius, cell count log-uniform in [500, 50000] (70%
+// 42-particle discs, 30% 12-particle icosahedra).  Radii and counts are
+// drawn independently and paired by rank (the k-th smallest count goes to
+// the k-th smallest vein), so that every vein holds its cells at the 35%
+// lattice packing; the pairs are then dealt to the veins in a shuffled order.
+extern "C" int bcs_scene_ensemble_params(int32_t n_veins, int32_t index, uint64_t seed, double cell_fraction,
+                                         bcs_scene_params* out)
+{
+    if (!out || n_veins < 1 || index < 0 || index >= n_veins || !(cell_fraction > 0.0))
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_scene_ensemble_params: bad arguments");
+    Rng rng(seed ^ 0xE5E5E5E5ull);
+    std::vector<double> r(n_veins), n(n_veins);
+    for (int k = 0; k < n_veins; ++k) {
+        r[k] = 20.0 + 60.0 * rng.uniform();
+        n[k] = std::exp(std::log(500.0) + rng.uniform() * (std::log(50000.0) - std::log(500.0)));
+    }
+    std::sort(r.begin(), r.end());
+    std::sort(n.begin(), n.end());
+    std::vector<int> deal(n_veins);
+    for (int k = 0; k < n_veins; ++k) deal[k] = k;
+    for (int k = n_veins - 1; k > 0; --k) std::swap(deal[k], deal[(int)(rng.uniform() * (k + 1)) % (k + 1)]);
+    const int q = deal[index];
+    const double R = r[q];
+    const int cells = std::max(2, (int)std::lround(n[q] * cell_fraction));
+    const int rc = bcs_scene_default_params(0, out);
+    if (rc != BCS_OK) return rc;
+    out->vein_kind = BCS_VEIN_CYLINDER;
+    out->radius = (float)R;
+    out->length = (float)(10.0 * R);
+    out->segments = std::max(16, (int)std::lround(2.0 * M_PI * R / 4.0));
+    out->rings = std::max(10, (int)std::lround(10.0 * R / 4.0));
+    out->n_cells = 2;
+    const int discs = (int)std::lround(0.7 * cells);
+    out->cells[0] = {BCS_CELL_DISC42, discs, 2.0f, 0.35f, 0.5f};
+    out->cells[1] = {BCS_CELL_ICOSAHEDRON, cells - discs, 2.0f, 1.0f, 0.5f};
+    out->placement = BCS_PLACE_LATTICE;
+    out->packing_fraction = 0.35f;
+    out->physics.dt = 0.005f;
+    out->seed = seed * 1000003ull + (uint64_t)index;
+    out->physics.seed = out->seed;
+    return BCS_OK;
+}
+
 extern "C" int bcs_scene_build(const bcs_scene_params* p, bcs_scene** out)
 {
     if (!p || !out) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_scene_build: null argument");

@@ -8,6 +8,7 @@
 // src/main.cu:130-133.  All device memory is owned by the bcs_sim object and
 // all work is enqueued on its single stream (default priority).
 #include "step.hpp"
+#include "surrogate.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -43,9 +44,34 @@ void launch_export(const float* soa, int64_t n, int64_t first, int64_t count, in
 void launch_refresh_copies(const float* pos, const float* vel, int32_t n, const int32_t* perm, float4* pvs,
                            hipStream_t st);
 void launch_chunk_map(const StepArgs& A, int2* out, hipStream_t st);
+void launch_surr_features(const float* pos, const float* vel, int64_t np, int32_t pstart, int32_t ncells,
+                          const uint8_t* touched, float pos_scale, float vel_scale, uint16_t* X, float* f32,
+                          hipStream_t st);
+void launch_cells_surr(const StepArgs& A, int nchunks, hipStream_t st);
 int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
                  uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
                  uint32_t* flags, hipStream_t st);
+// batched launches (ensembles)
+VeinPrepArgs vein_prep_args(const VeinArgs& V, int32_t ntri, const uint4* trun, const uint4* trec, float4* tdata,
+                            float4* aabb, uint32_t* blocks);
+uint32_t ray_split(int nchunks);
+uint32_t cell_blocks(int nchunks);
+uint32_t ray_blocks(int nchunks);
+uint32_t vein_blocks(int32_t nv);
+uint32_t order_key_blocks(const TypeTable& tt, int32_t otiles);
+uint32_t order_scatter_blocks(const TypeTable& tt);
+void batch_raycand(bool fused, const StepArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run,
+                   hipStream_t st);
+void batch_ray(const StepArgs* a, const uint32_t* split, const uint32_t* start, int32_t nsim, uint32_t grid,
+               uint32_t run, hipStream_t st);
+void batch_cells(const StepArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run, hipStream_t st);
+void batch_vein(const VeinArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run, hipStream_t st);
+void batch_vein_prep(const VeinPrepArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run,
+                     hipStream_t st);
+void batch_order(const OrderKeyArgs* ka, const uint32_t* kstart, uint32_t kgrid, const ScanArgs* sa,
+                 const uint32_t* sstart, uint32_t sgrid, const OrderScatterArgs* oa, const uint32_t* ostart,
+                 uint32_t ogrid, const StepArgs* a, int2* const* out, const uint32_t* cstart, uint32_t cgrid,
+                 int32_t nsim, uint32_t run, hipStream_t st);
 }  // namespace bcs
 
 using namespace bcs;
@@ -118,6 +144,19 @@ struct bcs_sim {
     // dense gridCellStarts / gridCellEnds views (bcs_sim_grid_view), per grid
     int32_t* dense_start[2] = {nullptr, nullptr};
     int32_t* dense_end[2] = {nullptr, nullptr};
+    // learned spring surrogate (bcs_sim_attach_surrogate, bcs_sim_surrogate_sample)
+    bcs_surrogate* surr = nullptr;
+    SurrNet surr_net{};
+    float surr_tol = 0.0f;
+    int32_t surr_type = -1;    // the first type of 42 particles per cell
+    int32_t surr_cells = 0;
+    bool sampling = false;     // this frame writes the plain spring displacements (sf_S)
+    bool recording = false;    // this frame keeps the contact flags (touched)
+    uint16_t *sf_X = nullptr, *sf_a0 = nullptr, *sf_a1 = nullptr;
+    float *sf_out = nullptr, *sf_S = nullptr, *sf_f32 = nullptr;
+    uint8_t* touched = nullptr;
+    uint32_t* surr_stats = nullptr;
+    bool surr_active() const { return surr_type >= 0 && (surr || sampling || recording); }
 
     ~bcs_sim()
     {
@@ -162,6 +201,18 @@ struct bcs_sim {
         A.t_aabb = t_aabb; A.t_nmask = t_nmask;
         A.best = best; A.work = work; A.wcount = wcount; A.region_cap = region_cap;
         A.flags = flags; A.step = step;
+        A.surr = SurrArgs{};
+        A.surr.type = -1;
+        if (surr_active()) {
+            A.surr.D = (surr && !sampling) ? sf_out : nullptr;
+            A.surr.S_out = sampling ? sf_S : nullptr;
+            A.surr.touched = touched;
+            A.surr.stats = surr_stats;
+            A.surr.type = surr_type;
+            A.surr.scale = (pc.dt * pc.dt) * 0.25f;
+            A.surr.out_scale = surr_net.out_scale;
+            A.surr.tol = surr_tol;
+        }
         return A;
     }
     VeinArgs vein_args() const
@@ -230,6 +281,15 @@ int enqueue_particle_grid(bcs_sim* s, const KernelTimer* timer = nullptr)
 }
 
 
+// Network input of the served cells from the frame's starting state, then
+// the network itself (the spring displacements k_cells checks and uses).
+void enqueue_surrogate(bcs_sim* s, hipStream_t st)
+{
+    launch_surr_features(s->pos, s->vel, s->np, s->tt.pstart[s->surr_type], s->surr_cells, s->touched,
+                         s->surr_net.pos_scale, s->surr_net.vel_scale, s->sf_X, nullptr, st);
+    surrogate_infer(s->surr_net, s->sf_X, s->surr_cells, s->sf_a0, s->sf_a1, s->sf_out, st);
+}
+
 // One fused frame.  Main branch: particle grid, then k_cells (centres,
 // springs, contacts, vein hit response, integrate, vein end).  Vein branch,
 // beside the grid build: triangle data and boxes, vein ray candidates, ray
@@ -261,6 +321,9 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
             launch_raycand(A, s->nchunks, s->side);
             launch_ray(A, s->side);
         }
+        // the network's springs need only the frame's starting x, v and the
+        // last frame's contact flags: they ride on the same branch
+        if (s->surr && !s->sampling) enqueue_surrogate(s, s->side);
         HIPCHK(hipEventRecord(s->ev_join, s->side));
         RC(enqueue_particle_grid(s));
         HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
@@ -279,9 +342,11 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         tb(kGridKernels + 2);
         if (!s->ray_fused()) launch_ray(A, st);
         te(kGridKernels + 2);
+        if (s->surr && !s->sampling) enqueue_surrogate(s, st);
     }
     tb(kGridKernels + 3);
-    launch_cells(A, s->nchunks, true, st);
+    if (s->surr_active()) launch_cells_surr(A, s->nchunks, st);
+    else launch_cells(A, s->nchunks, true, st);
     te(kGridKernels + 3);
     tb(kGridKernels + 4);
     if (tail_vein) launch_vein(Va, 7, st);
@@ -1233,5 +1298,592 @@ extern "C" int bcs_calibration_copy(int64_t n_bytes, int32_t repeats, float* ms)
     (void)hipStreamDestroy(st);
     (void)hipFree(a);
     (void)hipFree(b);
+    return BCS_OK;
+}
+
+// ============================================================ ensembles ==
+// Members keep their own buffers; the ensemble holds each member's kernel
+// arguments in device arrays and runs every frame kernel once for all members
+// through the batched kernels (a persistent grid over the members' blocks).
+// A member's blocks see the same block indices, block counts and arguments
+// as in its own launches, so its results are those of bcs_sim_step.
+
+namespace {
+
+enum EnsLaunch {
+    EL_KEY, EL_SCAN, EL_SCATTER, EL_RANK, EL_OCC,  // grid build (GridBatch order)
+    EL_PREP, EL_RAYCAND, EL_RAY, EL_CELLS, EL_VEIN,
+    EL_OKEY, EL_OSCAN, EL_OSCAT, EL_CMAP,
+    EL_COUNT
+};
+
+}  // namespace
+
+struct bcs_ensemble {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<bcs_sim*> members;
+    std::vector<int32_t> order;           // member of each argument slot
+    bcs_ensemble_tuning tuning{0, 1, BCS_RAY_AUTO, 1};
+    bool fused = true;
+    // device argument arrays (one entry per slot)
+    StepArgs* d_step = nullptr;
+    VeinArgs* d_vein = nullptr;
+    VeinPrepArgs* d_prep = nullptr;
+    KeyArgs* d_key = nullptr;
+    ScanArgs* d_scan = nullptr;
+    ScatterArgs* d_scatter = nullptr;
+    RankArgs* d_rank = nullptr;
+    OccArgs* d_occ = nullptr;
+    OrderKeyArgs* d_okey = nullptr;
+    ScanArgs* d_oscan = nullptr;
+    OrderScatterArgs* d_oscat = nullptr;
+    int2** d_cmap = nullptr;
+    uint32_t* d_split = nullptr;
+    uint32_t* d_start = nullptr;  // EL_COUNT arrays of n + 1 virtual-block offsets
+    uint32_t total[EL_COUNT]{};
+    std::vector<uint32_t*> shapes;  // per member: its particle-grid bucket arrays when the arguments were taken
+    int32_t frames_since_order = 0;
+    hipGraph_t graph[3]{};
+    hipGraphExec_t exec[3]{};  // first frame, a frame after another, kGraphFrames frames after the order refresh
+    std::vector<void*> allocs;
+
+    ~bcs_ensemble()
+    {
+        drop_graphs();
+        for (void* p : allocs) (void)hipFree(p);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void drop_graphs()
+    {
+        for (int k = 0; k < 3; ++k) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            exec[k] = nullptr;
+            graph[k] = nullptr;
+        }
+    }
+    template <class T>
+    int dalloc(T** p, size_t count)
+    {
+        hipError_t e = hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(BCS_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back((void*)*p);
+        return BCS_OK;
+    }
+    const uint32_t* start(int l) const { return d_start + (size_t)l * (members.size() + 1); }
+    uint32_t grid(int l) const
+    {
+        const uint32_t t = total[l];
+        if (tuning.blocks_per_cu <= 0) return (t + (uint32_t)std::max(tuning.run, 1) - 1) / (uint32_t)std::max(tuning.run, 1);
+        return std::min<uint32_t>(t, (uint32_t)tuning.blocks_per_cu * (uint32_t)cus);
+    }
+};
+
+namespace {
+
+int ens_dev(bcs_ensemble* e)
+{
+    if (!e) return fail(BCS_ERR_INVALID_ARGUMENT, "null ensemble");
+    HIPCHK(hipSetDevice(e->device));
+    return BCS_OK;
+}
+
+// Arguments and virtual-block offsets of every member, in slot order.
+int ens_build_args(bcs_ensemble* e)
+{
+    const size_t n = e->members.size();
+    size_t np_total = 0;
+    for (bcs_sim* s : e->members) np_total += (size_t)s->nchunks;
+    e->fused = e->tuning.ray_mode == BCS_RAY_FUSED ||
+               (e->tuning.ray_mode == BCS_RAY_AUTO && cell_blocks((int)std::min<size_t>(np_total, INT32_MAX)) > 4096u);
+    std::vector<StepArgs> step(n);
+    std::vector<VeinArgs> vein(n);
+    std::vector<VeinPrepArgs> prep(n);
+    std::vector<KeyArgs> key(n);
+    std::vector<ScanArgs> scan(n), oscan(n);
+    std::vector<ScatterArgs> scatter(n);
+    std::vector<RankArgs> rank(n);
+    std::vector<OccArgs> occ(n);
+    std::vector<OrderKeyArgs> okey(n);
+    std::vector<OrderScatterArgs> oscat(n);
+    std::vector<int2*> cmap(n);
+    std::vector<uint32_t> split(n);
+    std::vector<uint32_t> blocks((size_t)EL_COUNT * n, 0u);
+    e->shapes.assign(e->members.size(), nullptr);
+    for (size_t k = 0; k < n; ++k) {
+        bcs_sim* s = e->members[e->order[k]];
+        e->shapes[e->order[k]] = s->pg.bcount;
+        GridLaunch L;
+        RC(grid_launch_args(s->pg, s->pos, s->np, s->np, s->vel, s->flags, s->chunk_pi, s->nchunks * kBlock, &L));
+        key[k] = L.key;
+        scan[k] = L.scan;
+        scatter[k] = L.scatter;
+        rank[k] = L.rank;
+        occ[k] = L.occ;
+        step[k] = s->step_args();
+        vein[k] = s->vein_args();
+        uint32_t pb = 0;
+        prep[k] = vein_prep_args(vein[k], s->ntri, s->trun, s->trec, s->tdata, s->t_aabb, &pb);
+        okey[k] = OrderKeyArgs{s->tt, s->pg.g, s->centers, s->okey, s->ocount, s->ostatus, s->otiles, s->octrl};
+        oscan[k] = ScanArgs{s->ocount, s->ostart, (int64_t)s->tt.nt << kOrderBits, s->ostatus, s->otiles, s->octrl,
+                            nullptr, 0u, 0u, s->flags, 0};
+        oscat[k] = OrderScatterArgs{s->tt.nc, s->okey, s->ocount, s->ostart, s->corder, s->flags};
+        cmap[k] = s->chunk_pi;
+        split[k] = ray_split(s->nchunks);
+        uint32_t* b = &blocks[0];
+        b[EL_KEY * n + k] = L.key_blocks;
+        b[EL_SCAN * n + k] = L.scan_blocks;
+        b[EL_SCATTER * n + k] = L.scatter_blocks;
+        b[EL_RANK * n + k] = L.rank_blocks;
+        b[EL_OCC * n + k] = L.occ_blocks;
+        b[EL_PREP * n + k] = pb;
+        b[EL_RAYCAND * n + k] = cell_blocks(s->nchunks);
+        b[EL_RAY * n + k] = e->fused ? 0u : ray_blocks(s->nchunks);
+        b[EL_CELLS * n + k] = cell_blocks(s->nchunks);
+        b[EL_VEIN * n + k] = vein_blocks(s->nv);
+        b[EL_OKEY * n + k] = order_key_blocks(s->tt, s->otiles);
+        b[EL_OSCAN * n + k] = s->tt.nc > 0 ? (uint32_t)s->otiles : 0u;
+        b[EL_OSCAT * n + k] = order_scatter_blocks(s->tt);
+        b[EL_CMAP * n + k] = (uint32_t)std::max(s->nchunks, 0);
+    }
+    std::vector<uint32_t> start((size_t)EL_COUNT * (n + 1));
+    for (int l = 0; l < EL_COUNT; ++l) {
+        uint64_t acc = 0;
+        for (size_t k = 0; k < n; ++k) {
+            start[(size_t)l * (n + 1) + k] = (uint32_t)acc;
+            acc += blocks[(size_t)l * n + k];
+        }
+        if (acc > 0xFFFFFFFFull) return fail(BCS_ERR_INVALID_ARGUMENT, "ensemble: more than 2^32 blocks in a launch");
+        start[(size_t)l * (n + 1) + n] = (uint32_t)acc;
+        e->total[l] = (uint32_t)acc;
+    }
+    hipStream_t st = e->stream;
+    HIPCHK(hipMemcpyAsync(e->d_step, step.data(), n * sizeof(StepArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_vein, vein.data(), n * sizeof(VeinArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_prep, prep.data(), n * sizeof(VeinPrepArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_key, key.data(), n * sizeof(KeyArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_scan, scan.data(), n * sizeof(ScanArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_scatter, scatter.data(), n * sizeof(ScatterArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_rank, rank.data(), n * sizeof(RankArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_occ, occ.data(), n * sizeof(OccArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_okey, okey.data(), n * sizeof(OrderKeyArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_oscan, oscan.data(), n * sizeof(ScanArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_oscat, oscat.data(), n * sizeof(OrderScatterArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_cmap, cmap.data(), n * sizeof(int2*), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_split, split.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->d_start, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    e->drop_graphs();
+    return BCS_OK;
+}
+
+// The members' bucket arrays move when their bucket size changes: take the
+// arguments again.
+int ens_check_shapes(bcs_ensemble* e)
+{
+    for (size_t m = 0; m < e->members.size(); ++m)
+        if (e->members[m]->pg.bcount != e->shapes[m]) return ens_build_args(e);
+    return BCS_OK;
+}
+
+int ens_enqueue_order(bcs_ensemble* e)
+{
+    const int32_t n = (int32_t)e->members.size();
+    const uint32_t run = (uint32_t)std::max(e->tuning.run, 1);
+    batch_order(e->d_okey, e->start(EL_OKEY), e->grid(EL_OKEY), e->d_oscan, e->start(EL_OSCAN), e->grid(EL_OSCAN),
+                e->d_oscat, e->start(EL_OSCAT), e->grid(EL_OSCAT), e->d_step, e->d_cmap, e->start(EL_CMAP),
+                e->grid(EL_CMAP), n, run, e->stream);
+    HIPCHK(hipGetLastError());
+    return BCS_OK;
+}
+
+// One frame of every member (enqueue_fused_frame, batched).
+int ens_enqueue_frame(bcs_ensemble* e, bool lead_vein)
+{
+    const int32_t n = (int32_t)e->members.size();
+    const uint32_t run = (uint32_t)std::max(e->tuning.run, 1);
+    hipStream_t st = e->stream;
+    HIPCHK(hipEventRecord(e->ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(e->side, e->ev_fork, 0));
+    if (lead_vein) batch_vein(e->d_vein, e->start(EL_VEIN), n, e->grid(EL_VEIN), run, e->side);
+    batch_vein_prep(e->d_prep, e->start(EL_PREP), n, e->grid(EL_PREP), run, e->side);
+    batch_raycand(e->fused, e->d_step, e->start(EL_RAYCAND), n, e->grid(EL_RAYCAND), run, e->side);
+    if (!e->fused) batch_ray(e->d_step, e->d_split, e->start(EL_RAY), n, e->grid(EL_RAY), run, e->side);
+    HIPCHK(hipEventRecord(e->ev_join, e->side));
+    GridBatch gb{e->d_key, e->d_scan, e->d_scatter, e->d_rank, e->d_occ,
+                 {e->start(EL_KEY), e->start(EL_SCAN), e->start(EL_SCATTER), e->start(EL_RANK), e->start(EL_OCC)},
+                 {e->grid(EL_KEY), e->grid(EL_SCAN), e->grid(EL_SCATTER), e->grid(EL_RANK), e->grid(EL_OCC)},
+                 n, run};
+    grid_build_batch(gb, st);
+    HIPCHK(hipStreamWaitEvent(st, e->ev_join, 0));
+    batch_cells(e->d_step, e->start(EL_CELLS), n, e->grid(EL_CELLS), run, st);
+    HIPCHK(hipGetLastError());
+    return BCS_OK;
+}
+
+int ens_capture(bcs_ensemble* e, int which)
+{
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = BCS_OK;
+    if (which == 2) {
+        rc = ens_enqueue_order(e);
+        for (int f = 0; f < kGraphFrames && !rc; ++f) rc = ens_enqueue_frame(e, true);
+    } else {
+        rc = ens_enqueue_frame(e, which == 1);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t err = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (err != hipSuccess) return fail(BCS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(err));
+    e->graph[which] = g;
+    HIPCHK(hipGraphInstantiate(&e->exec[which], g, nullptr, nullptr, 0));
+    return BCS_OK;
+}
+
+int ens_step(bcs_ensemble* e, int32_t n_steps)
+{
+    RC(ens_check_shapes(e));
+    for (int k = 0; k < 3; ++k)
+        if (!e->exec[k]) RC(ens_capture(e, k));
+    for (int32_t k = 0; k < n_steps;) {
+        const bool lead = k > 0;
+        int32_t nf = 1;
+        if (lead && n_steps - k >= kGraphFrames) {
+            HIPCHK(hipGraphLaunch(e->exec[2], e->stream));
+            nf = kGraphFrames;
+            e->frames_since_order = 0;
+        } else {
+            if (e->frames_since_order >= kReorderPeriod) {
+                RC(ens_enqueue_order(e));
+                e->frames_since_order = 0;
+            }
+            HIPCHK(hipGraphLaunch(e->exec[lead ? 1 : 0], e->stream));
+        }
+        k += nf;
+        e->frames_since_order += nf;
+    }
+    // the last frame's vein update
+    batch_vein(e->d_vein, e->start(EL_VEIN), (int32_t)e->members.size(), e->grid(EL_VEIN),
+               (uint32_t)std::max(e->tuning.run, 1), e->stream);
+    HIPCHK(hipGetLastError());
+    for (bcs_sim* s : e->members) {
+        s->copies_current = false;
+        s->frames_since_order = e->frames_since_order;
+    }
+    return BCS_OK;
+}
+
+int ens_set_tuning(bcs_ensemble* e, const bcs_ensemble_tuning& t)
+{
+    if (t.blocks_per_cu < 0 || t.blocks_per_cu > 64 || t.run < 1 || t.run > 64)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "ensemble tuning: blocks_per_cu in 0..64, run in 1..64");
+    if (t.ray_mode != BCS_RAY_AUTO && t.ray_mode != BCS_RAY_SPLIT && t.ray_mode != BCS_RAY_FUSED)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "ensemble tuning: ray_mode is BCS_RAY_AUTO, _SPLIT or _FUSED");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->tuning = t;
+    e->order.resize(e->members.size());
+    for (size_t k = 0; k < e->order.size(); ++k) e->order[k] = (int32_t)k;
+    if (t.largest_first)
+        std::stable_sort(e->order.begin(), e->order.end(),
+                         [&](int32_t a, int32_t b) { return e->members[a]->np > e->members[b]->np; });
+    return ens_build_args(e);
+}
+
+}  // namespace
+
+extern "C" int bcs_ensemble_create(bcs_sim* const* members, int32_t n_members, bcs_ensemble** out)
+{
+    if (!members || !out || n_members < 1) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_ensemble_create: no members");
+    *out = nullptr;
+    const int device = members[0] ? members[0]->device : 0;
+    for (int32_t k = 0; k < n_members; ++k) {
+        bcs_sim* s = members[k];
+        if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_ensemble_create: null member");
+        if (s->device != device) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_ensemble_create: members on two devices");
+        if (!defer_vein(s))
+            return fail(BCS_ERR_UNSUPPORTED, "bcs_ensemble_create: members need a static triangle grid");
+        for (int32_t j = 0; j < k; ++j)
+            if (members[j] == s) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_ensemble_create: a member twice");
+    }
+    std::unique_ptr<bcs_ensemble> e(new bcs_ensemble());
+    e->device = device;
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop{};
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    e->cus = std::max(prop.multiProcessorCount, 1);
+    for (int32_t k = 0; k < n_members; ++k) {
+        HIPCHK(hipStreamSynchronize(members[k]->stream));
+        e->members.push_back(members[k]);
+    }
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+    const size_t n = (size_t)n_members;
+    RC(e->dalloc(&e->d_step, n));
+    RC(e->dalloc(&e->d_vein, n));
+    RC(e->dalloc(&e->d_prep, n));
+    RC(e->dalloc(&e->d_key, n));
+    RC(e->dalloc(&e->d_scan, n));
+    RC(e->dalloc(&e->d_scatter, n));
+    RC(e->dalloc(&e->d_rank, n));
+    RC(e->dalloc(&e->d_occ, n));
+    RC(e->dalloc(&e->d_okey, n));
+    RC(e->dalloc(&e->d_oscan, n));
+    RC(e->dalloc(&e->d_oscat, n));
+    RC(e->dalloc(&e->d_cmap, n));
+    RC(e->dalloc(&e->d_split, n));
+    RC(e->dalloc(&e->d_start, (size_t)EL_COUNT * (n + 1)));
+    e->frames_since_order = 0;
+    for (bcs_sim* s : e->members) e->frames_since_order = std::max(e->frames_since_order, s->frames_since_order);
+    RC(ens_set_tuning(e.get(), e->tuning));
+    *out = e.release();
+    return BCS_OK;
+}
+
+extern "C" int bcs_ensemble_destroy(bcs_ensemble* e)
+{
+    if (!e) return BCS_OK;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    delete e;
+    return BCS_OK;
+}
+
+extern "C" int bcs_ensemble_set_tuning(bcs_ensemble* e, const bcs_ensemble_tuning* t)
+{
+    RC(ens_dev(e));
+    if (!t) return fail(BCS_ERR_INVALID_ARGUMENT, "null tuning");
+    return ens_set_tuning(e, *t);
+}
+
+extern "C" int bcs_ensemble_get_tuning(bcs_ensemble* e, bcs_ensemble_tuning* t)
+{
+    if (!e || !t) return fail(BCS_ERR_INVALID_ARGUMENT, "null argument");
+    *t = e->tuning;
+    return BCS_OK;
+}
+
+extern "C" int bcs_ensemble_step(bcs_ensemble* e, int32_t n_steps)
+{
+    RC(ens_dev(e));
+    if (n_steps < 0) return fail(BCS_ERR_INVALID_ARGUMENT, "negative step count");
+    if (n_steps == 0) return BCS_OK;
+    return ens_step(e, n_steps);
+}
+
+extern "C" int bcs_ensemble_synchronize(bcs_ensemble* e)
+{
+    RC(ens_dev(e));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_ensemble_time_steps(bcs_ensemble* e, int32_t n_steps, float* total_ms)
+{
+    RC(ens_dev(e));
+    if (!total_ms || n_steps < 1) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_ensemble_time_steps: bad arguments");
+    RC(ens_check_shapes(e));
+    for (int k = 0; k < 3; ++k)
+        if (!e->exec[k]) RC(ens_capture(e, k));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, e->stream));
+    int rc = ens_step(e, n_steps);
+    HIPCHK(hipEventRecord(e1, e->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *total_ms = ms;
+    return rc;
+}
+
+extern "C" int bcs_ensemble_autotune(bcs_ensemble* e, int32_t frames_per_trial, bcs_ensemble_tuning* best,
+                                     char* report, int32_t report_capacity)
+{
+    RC(ens_dev(e));
+    if (frames_per_trial < 1) return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_ensemble_autotune: frames_per_trial >= 1");
+    static const int32_t kBpc[] = {0, 2, 4, 8, 16};
+    static const int32_t kRun[] = {1, 4};
+    static const int32_t kRay[] = {BCS_RAY_SPLIT, BCS_RAY_FUSED};
+    static const int32_t kLargest[] = {0, 1};
+    std::string js = "{\"space\": {\"blocks_per_cu\": [0, 2, 4, 8, 16], \"run\": [1, 4], \"ray_mode\": [\"split\", "
+                     "\"fused\"], \"largest_first\": [0, 1]}, \"frames_per_trial\": " +
+                     std::to_string(frames_per_trial) + ", \"trials\": [";
+    bcs_ensemble_tuning chosen = e->tuning;
+    float chosen_ms = 3.4e38f;
+    bool firstj = true;
+    for (int32_t bpc : kBpc)
+        for (int32_t run : kRun)
+            for (int32_t ray : kRay)
+                for (int32_t lf : kLargest) {
+                    if (bpc == 0 && run != 1) continue;  // without persistence a run only merges blocks
+                    const bcs_ensemble_tuning t{bpc, run, ray, lf};
+                    RC(ens_set_tuning(e, t));
+                    RC(ens_step(e, 1));  // graphs captured, caches warm
+                    float ms = 0.f;
+                    RC(bcs_ensemble_time_steps(e, frames_per_trial, &ms));
+                    const float per = ms / (float)frames_per_trial;
+                    char buf[160];
+                    std::snprintf(buf, sizeof(buf),
+                                  "%s{\"blocks_per_cu\": %d, \"run\": %d, \"ray_mode\": \"%s\", \"largest_first\": %d, "
+                                  "\"ms_per_frame\": %.5f}",
+                                  firstj ? "" : ", ", bpc, run, ray == BCS_RAY_SPLIT ? "split" : "fused", lf, per);
+                    js += buf;
+                    firstj = false;
+                    if (per < chosen_ms) {
+                        chosen_ms = per;
+                        chosen = t;
+                    }
+                }
+    RC(ens_set_tuning(e, chosen));
+    char buf[200];
+    std::snprintf(buf, sizeof(buf),
+                  "], \"chosen\": {\"blocks_per_cu\": %d, \"run\": %d, \"ray_mode\": \"%s\", \"largest_first\": %d, "
+                  "\"ms_per_frame\": %.5f}}",
+                  chosen.blocks_per_cu, chosen.run, chosen.ray_mode == BCS_RAY_SPLIT ? "split" : "fused",
+                  chosen.largest_first, chosen_ms);
+    js += buf;
+    if (best) *best = chosen;
+    if (report && report_capacity > 0) {
+        const size_t len = std::min<size_t>(js.size(), (size_t)report_capacity - 1);
+        std::memcpy(report, js.data(), len);
+        report[len] = '\0';
+    }
+    return BCS_OK;
+}
+
+// ============================================================ surrogate ==
+
+namespace {
+
+// Buffers of the served type (the first with 42 particles per cell).
+int surr_alloc(bcs_sim* s)
+{
+    if (s->surr_type < 0) {
+        for (int t = 0; t < s->nt; ++t)
+            if (s->tt.ppc[t] == kSurrPpc && s->tt.cnt[t] > 0) {
+                s->surr_type = t;
+                break;
+            }
+        if (s->surr_type < 0) return fail(BCS_ERR_UNSUPPORTED, "surrogate: no cell type of 42 particles");
+    }
+    s->surr_cells = s->tt.cnt[s->surr_type];
+    const size_t nc = (size_t)s->surr_cells;
+    if (!s->sf_X) {
+        RC(s->dalloc(&s->sf_X, nc * kSurrInPad));
+        RC(s->dalloc(&s->sf_a0, nc * kSurrHidden));
+        RC(s->dalloc(&s->sf_a1, nc * kSurrHidden));
+        RC(s->dalloc(&s->sf_out, nc * kSurrOutPad));
+        RC(s->dalloc(&s->sf_S, nc * kSurrOutPad));
+        RC(s->dalloc(&s->sf_f32, nc * kSurrIn));
+        RC(s->dalloc(&s->touched, (size_t)std::max(s->np, 1)));
+        RC(s->dalloc(&s->surr_stats, 4));
+        HIPCHK(hipMemsetAsync(s->touched, 0, (size_t)std::max(s->np, 1), s->stream));
+        HIPCHK(hipMemsetAsync(s->surr_stats, 0, 16, s->stream));
+    }
+    return BCS_OK;
+}
+
+// One frame enqueued directly (what one frame of bcs_sim_step does).
+int enqueue_frame_direct(bcs_sim* s)
+{
+    if (s->frames_since_order >= kReorderPeriod) {
+        RC(enqueue_order(s));
+        s->frames_since_order = 0;
+    }
+    RC(enqueue_fused_frame(s, nullptr, false, true));
+    s->frames_since_order += 1;
+    s->copies_current = false;
+    return BCS_OK;
+}
+
+}  // namespace
+
+extern "C" int bcs_sim_attach_surrogate(bcs_sim* s, bcs_surrogate* net, float tolerance)
+{
+    RC(set_dev(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    drop_graph(s);
+    if (!net) {
+        s->surr = nullptr;
+        return BCS_OK;
+    }
+    SurrNet n{};
+    int dev = -1;
+    RC(bcs_surrogate_net(net, &n, &dev));
+    if (dev != s->device) return fail(BCS_ERR_INVALID_ARGUMENT, "surrogate: network on another device");
+    RC(surr_alloc(s));
+    s->surr = net;
+    s->surr_net = n;
+    s->surr_tol = tolerance;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_surrogate_sample(bcs_sim* s, float* features, float* targets, int32_t capacity, int32_t* n_out)
+{
+    RC(set_dev(s));
+    if (!n_out || capacity < 0 || (capacity > 0 && (!features || !targets)))
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_surrogate_sample: bad arguments");
+    RC(surr_alloc(s));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // frame 1 keeps the contact flags; frame 2 samples the network input of
+    // its starting state and the plain spring displacements it computes
+    s->recording = true;
+    int rc = enqueue_frame_direct(s);
+    if (!rc) {
+        launch_surr_features(s->pos, s->vel, s->np, s->tt.pstart[s->surr_type], s->surr_cells, s->touched, 1.0f, 1.0f,
+                             s->sf_X, s->sf_f32, s->stream);
+        s->sampling = true;
+        rc = enqueue_frame_direct(s);
+    }
+    s->sampling = false;
+    s->recording = false;
+    RC(rc);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int32_t n = std::min(capacity, s->surr_cells);
+    if (n > 0) {
+        HIPCHK(hipMemcpy(features, s->sf_f32, (size_t)n * kSurrIn * 4, hipMemcpyDeviceToHost));
+        std::vector<float> S((size_t)n * kSurrOutPad);
+        HIPCHK(hipMemcpy(S.data(), s->sf_S, S.size() * 4, hipMemcpyDeviceToHost));
+        for (int32_t m = 0; m < n; ++m)
+            std::memcpy(targets + (size_t)m * kSurrOut, S.data() + (size_t)m * kSurrOutPad, kSurrOut * 4);
+    }
+    *n_out = n;
+    return BCS_OK;
+}
+
+extern "C" int bcs_sim_surrogate_stats(bcs_sim* s, int32_t reset, uint64_t* accepted, uint64_t* fallback,
+                                       float* max_error_accepted, float* max_error_all)
+{
+    RC(set_dev(s));
+    if (!s->surr_stats) return fail(BCS_ERR_STATE, "surrogate: none attached");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    uint32_t h[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpy(h, s->surr_stats, 16, hipMemcpyDeviceToHost));
+    if (accepted) *accepted = h[0];
+    if (fallback) *fallback = h[1];
+    float f2, f3;
+    std::memcpy(&f2, &h[2], 4);
+    std::memcpy(&f3, &h[3], 4);
+    if (max_error_accepted) *max_error_accepted = f2;
+    if (max_error_all) *max_error_all = f3;
+    if (reset) {
+        HIPCHK(hipMemsetAsync(s->surr_stats, 0, 16, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
     return BCS_OK;
 }

@@ -25,6 +25,7 @@
 // in particle-index order by k_vein (deterministic; the reference adds them
 // without atomics).
 #include "step.hpp"
+#include "surrogate.hpp"
 
 namespace bcs {
 
@@ -85,7 +86,6 @@ __device__ __forceinline__ int chunk_of_block(int32_t nchunks, uint32_t block)
     const uint32_t c = sg * S + x * run + q;
     return c < (uint32_t)nchunks ? (int)c : -1;
 }
-__device__ __forceinline__ int chunk_of_block(int32_t nchunks) { return chunk_of_block(nchunks, blockIdx.x); }
 
 // Particle of vertex a of the cl-th cell of chunk (T, first).
 __device__ __forceinline__ int32_t chunk_particle(const StepArgs& A, int T, int first, int cl, int a, int p,
@@ -436,15 +436,15 @@ __device__ __forceinline__ bool item_first_lane(bool hit, uint32_t g)
 // the frame's starting x, v and the triangle data, so it runs beside the
 // particle grid build.
 template <bool FUSED>
-__device__ __forceinline__ void raycand_block(const StepArgs& A)
+__device__ __forceinline__ void raycand_block(const StepArgs& A, VBlock vb)
 {
     __shared__ uint32_t s_wsum[kBlock / 64];
     __shared__ uint32_t s_end;
     const TypeTable& tt = A.tt;
     const int l = threadIdx.x;
-    const int chunk = chunk_of_block(A.nchunks);
+    const int chunk = chunk_of_block(A.nchunks, vb.id);
     if (chunk < 0) {
-        if (l == 0) A.wcount[blockIdx.x] = 0u;
+        if (l == 0) A.wcount[vb.id] = 0u;
         return;
     }
     // the lane's particle (-1: no slot), independent of the chunk's tables
@@ -497,7 +497,7 @@ __device__ __forceinline__ void raycand_block(const StepArgs& A)
             }
         }
     }
-    // ---- work-list space: every block owns region blockIdx.x of region_cap
+    // ---- work-list space: every block owns region vb.id of region_cap
     // items; offsets from a block-wide exclusive scan (no global atomics)
     const int lane = l & 63, wv = l >> 6;
     uint32_t incl = nitems;
@@ -521,12 +521,12 @@ __device__ __forceinline__ void raycand_block(const StepArgs& A)
     if (act && nitems > 0 && fits) atomicMax(&s_end, off + nitems);
     if (!FUSED) {
         __syncthreads();
-        if (l == 0) A.wcount[blockIdx.x] = s_end;
+        if (l == 0) A.wcount[vb.id] = s_end;
         if (act) {
             A.best[i] = (nitems > 0 && !fits) ? kScanSerial : kNoHit;
             if (nitems > 0 && !fits) fl |= BCS_FLAG_RAY_SERIAL;
             if (nitems > 0 && fits) {
-                uint4* out = A.work + (size_t)blockIdx.x * A.region_cap;
+                uint4* out = A.work + (size_t)vb.id * A.region_cap;
                 uint32_t w = off;
                 uint32_t m = pmask;
                 while (m) {
@@ -566,7 +566,7 @@ __device__ __forceinline__ void raycand_block(const StepArgs& A)
         }
         __syncthreads();
         const uint32_t total = s_end;
-        if (l == 0) A.wcount[blockIdx.x] = total;
+        if (l == 0) A.wcount[vb.id] = total;
         constexpr uint32_t G = kRayLanes, K = kItemTris / kRayLanes;
         const uint32_t g = l % G;
         // base is block-uniform, so every lane reaches the ballot
@@ -589,11 +589,11 @@ __device__ __forceinline__ void raycand_block(const StepArgs& A)
     if (fl) atomicOr(A.flags, fl);
 }
 
-__global__ void __launch_bounds__(kBlock) k_raycand(StepArgs A) { raycand_block<false>(A); }
+__global__ void __launch_bounds__(kBlock) k_raycand(StepArgs A) { raycand_block<false>(A, hw_block()); }
 
 // k_raycand and k_ray in one launch: each block keeps its work items in LDS
 // (at most kRayLdsItems; rays that do not fit are scanned serially by k_cells).
-__global__ void __launch_bounds__(kBlock) k_raycand_ray(StepArgs A) { raycand_block<true>(A); }
+__global__ void __launch_bounds__(kBlock) k_raycand_ray(StepArgs A) { raycand_block<true>(A, hw_block()); }
 
 template <bool FUSED>
 __device__ __forceinline__ void respond_integrate(const StepArgs& A, bool act, int32_t i, int T, int a, int cl,
@@ -601,9 +601,11 @@ __device__ __forceinline__ void respond_integrate(const StepArgs& A, bool act, i
 
 static_assert(sizeof(StepArgs) <= 4096, "kernel arguments exceed 4 KiB");
 
-template <bool FUSED>
-__global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
+template <bool FUSED, bool SURR = false>
+__device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
 {
+    __shared__ uint32_t s_err[SURR ? kBlock : 1];  // per cell of the chunk: the surrogate's largest error
+    __shared__ uint32_t s_stat[4];                 // the chunk's accepted / fallback cells, largest errors
     __shared__ float sx[kBlock], sy[kBlock], sz[kBlock];
     __shared__ float svx[kBlock], svy[kBlock], svz[kBlock];
     __shared__ float sfx[kBlock], sfy[kBlock], sfz[kBlock];
@@ -616,7 +618,7 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
     const int l = threadIdx.x;
-    const int chunk = chunk_of_block(A.nchunks);
+    const int chunk = chunk_of_block(A.nchunks, vb.id);
     if (chunk < 0) return;
     // contact radii per model vertex (cellModelsBoundingSpheres), from LDS
     // when the table fits
@@ -691,17 +693,19 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
     }
     const float r0a = act ? A.r0[m0 + a] : 1.0f;
     if (l < ncell) s_tele[l] = 0;
+    if (SURR && l < ncell) s_err[l] = 0u;
+    if (SURR && l < 4) s_stat[l] = 0u;
     __syncthreads();
 
     uint32_t fl = 0;
-    f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
+    f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1, F0 = x1, cen = x1, newF = x1;
+    bool more = false;
+    const int base = cl * p;
     if (act) {
         x1 = mk(sx[l], sy[l], sz[l]);
         v1 = mk(svx[l], svy[l], svz[l]);
-        const f3 F0 = mk(sfx[l], sfy[l], sfz[l]);
+        F0 = mk(sfx[l], sfy[l], sfz[l]);
         // ---- cell centre (sequential sum in vertex order)
-        const int base = cl * p;
-        f3 cen = mk(0.f, 0.f, 0.f);
         for (int b = 0; b < p; ++b) cen = add(cen, mk(sx[base + b], sy[base + b], sz[base + b]));
         cen = sdiv(cen, (float)p);
         if (a == 0) {
@@ -709,10 +713,9 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
             A.centers[tt.nc + cell] = cen.y;
             A.centers[2 * (int64_t)tt.nc + cell] = cen.z;
         }
-        bool more = mask != 0u;
+        more = mask != 0u;
         // ---- springs + environment (F of neighbours from the LDS snapshot)
         const int ma = m0 + a;
-        f3 newF = mk(0.f, 0.f, 0.f);
         auto spring_sum = [&](auto spb, auto spL, int e0, int e1) {
         for (int e = e0; e < e1; ++e) {
             const int nb = base + spb(e);
@@ -743,6 +746,47 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
         else
             spring_sum([&](int e) { return A.sp_b[e]; }, [&](int e) { return A.sp_L[e]; }, A.sp_off[ma],
                        A.sp_off[ma + 1]);
+    }
+    if constexpr (SURR) {
+        // ---- learned springs: the served type's cells take the network's
+        // spring displacement when the whole cell is within tolerance of the
+        // plain solve (T is one per block, so the barrier is uniform)
+        const SurrArgs& sa = A.surr;
+        if (T == sa.type) {
+            const int64_t row = (int64_t)(cell - tt.cstart[T]) * kSurrOutPad + 3 * a;
+            const f3 d = smul(sa.scale, newF);  // the plain spring displacement (dt^2/4) S
+            if (act && sa.S_out) {
+                sa.S_out[row] = d.x;
+                sa.S_out[row + 1] = d.y;
+                sa.S_out[row + 2] = d.z;
+            }
+            if (sa.D) {
+                f3 dh = mk(0.f, 0.f, 0.f);
+                if (act) {
+                    dh = smul(sa.out_scale, mk(sa.D[row], sa.D[row + 1], sa.D[row + 2]));
+                    atomicMax(&s_err[cl], __float_as_uint(len(sub(dh, d))));
+                }
+                __syncthreads();
+                if (act) {
+                    const float ce = __uint_as_float(s_err[cl]);
+                    const bool ok = sa.tol > 0.0f && ce <= sa.tol;
+                    if (ok) newF = sdiv(dh, sa.scale);
+                    if (a == 0) {  // the block's tallies (errors are >= 0: float order = bit order)
+                        atomicAdd(&s_stat[ok ? 0 : 1], 1u);
+                        atomicMax(&s_stat[3], __float_as_uint(ce));
+                        if (ok) atomicMax(&s_stat[2], __float_as_uint(ce));
+                    }
+                }
+                __syncthreads();
+                if (l < 4 && s_stat[l] != 0u) {  // one global update per tally and block
+                    if (l < 2) atomicAdd(&sa.stats[l], s_stat[l]);
+                    else atomicMax(&sa.stats[l], s_stat[l]);
+                }
+            }
+        }
+    }
+    uint32_t ncontact = 0;
+    if (act) {
         const float ratio = len(sub(x1, cen)) / r0a;
         f3 env;
         const f3 G = mk(pc.G[0], pc.G[1], pc.G[2]);
@@ -808,15 +852,21 @@ __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
                         if (d2 <= minD * minD && d2 >= 0.0001f) {
                             const f3 rv = sub(v1, mk(w[t].x, w[t].y, w[t].z));
                             F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
+                            ++ncontact;
                         }
                     }
                 }
                 more = mask != 0u;
             }
         }
-
+        if (SURR && A.surr.touched) A.surr.touched[i] = ncontact > 0u ? 1 : 0;
     }
     respond_integrate<FUSED>(A, act, i, T, a, cl, cell, x1, v1, F, fl, s_tele);  // raises fl
+}
+template <bool FUSED, bool SURR = false>
+__global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
+{
+    cells_body<FUSED, SURR>(A, hw_block());
 }
 
 // kRayLanes lanes per work item; the item's first hit is folded into the
@@ -850,13 +900,14 @@ __device__ __forceinline__ void ray_region(const StepArgs& A, uint32_t r, uint32
     if (fl) atomicOr(A.flags, fl);
 }
 
-__global__ void __launch_bounds__(256) k_ray(StepArgs A, uint32_t split)
+__device__ __forceinline__ void ray_body(const StepArgs& A, uint32_t split, VBlock vb)
 {
     // `split` blocks share the work region of k_raycand block r; block
     // b = s * nregions + r runs on the XCD (b % 8 == r % 8) whose L2 holds it
-    const uint32_t nreg = gridDim.x / split;
-    ray_region(A, blockIdx.x % nreg, blockIdx.x / nreg, split);
+    const uint32_t nreg = vb.count / split;
+    ray_region(A, vb.id % nreg, vb.id / nreg, split);
 }
+__global__ void __launch_bounds__(256) k_ray(StepArgs A, uint32_t split) { ray_body(A, split, hw_block()); }
 
 // Vein hit response of particle i (detectVeinCollisions response,
 // src/simulation/vein_collisions.cu:234-275), then for FUSED frames the Heun
@@ -968,7 +1019,7 @@ __global__ void __launch_bounds__(kBlock) k_vein_end(StepArgs A)
     __shared__ int s_tele[kBlock];
     const TypeTable& tt = A.tt;
     const int l = threadIdx.x;
-    const int chunk = chunk_of_block(A.nchunks);
+    const int chunk = chunk_of_block(A.nchunks, blockIdx.x);
     if (chunk < 0) return;
     const int T = A.chunk_type[chunk], first = A.chunk_first[chunk], ncell = A.chunk_n[chunk];
     const int p = tt.ppc[T];
@@ -1044,9 +1095,9 @@ __global__ void __launch_bounds__(256) k_vein_springs(VeinArgs A)
 // (mode & 2) integrate: V += dt*Fv; X += dt*V; Fv = 0
 // (propagateForcesIntoPositionsKernel, src/objects/vein_triangles.cu:88-117).
 template <int MODE>
-__global__ void __launch_bounds__(256) k_vein(VeinArgs A)
+__device__ __forceinline__ void vein_body(const VeinArgs& A, VBlock vb)
 {
-    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t id = vb.id * blockDim.x + threadIdx.x;
     if (MODE & 1) {
         if (id == 0) {  // hits consumed (the hit buffer is only read through the lists)
             A.hctrl[4] = A.hctrl[0];
@@ -1096,6 +1147,11 @@ __global__ void __launch_bounds__(256) k_vein(VeinArgs A)
         store3(A.Fv, nv, id, F);
     }
 }
+template <int MODE>
+__global__ void __launch_bounds__(256) k_vein(VeinArgs A)
+{
+    vein_body<MODE>(A, hw_block());
+}
 
 // Triangle centres, calculateCentersKernel, src/objects/vein_triangles.cu:14-27.
 __global__ void __launch_bounds__(256) k_tri_centers(const float* __restrict__ X, int32_t nv,
@@ -1115,16 +1171,6 @@ __global__ void __launch_bounds__(256) k_tri_centers(const float* __restrict__ X
 // the current vein positions, stored at its sorted slot -- and the wave
 // reduces the cell's box (exact vertex bounds), stored under the cell key.
 // Non-finite coordinates widen the box to everything.
-struct TriBoxArgs {
-    const uint4* trun;  // per occupied cell r: {run start, cell key, count, 1}; w = 0 ends the list
-    const uint4* trec;  // per sorted slot u: {i0, i1, i2, triangle}
-    int32_t ntri;
-    const float* X;
-    int32_t nv;
-    float4* tdata;
-    float4* aabb;
-};
-
 // Blocks [0, nblocks) of 256 threads share the occupied triangle cells, one
 // wave per cell at a time: run record, then the sorted triangles' records,
 // then their vertices (three dependent loads).
@@ -1211,11 +1257,12 @@ __global__ void __launch_bounds__(256) k_tri_records(const uint32_t* __restrict_
 
 // Vein springs and triangle data in one launch (both need only the updated
 // vein): the first nsb blocks take the vertices, the rest the triangle cells.
-__global__ void __launch_bounds__(256) k_vein_prep(VeinArgs A, TriBoxArgs B, uint32_t nsb)
+__device__ __forceinline__ void vein_prep_body(const VeinPrepArgs& P, VBlock vb)
 {
-    if (blockIdx.x < nsb) vein_springs_at(A, blockIdx.x * 256u + threadIdx.x);
-    else tri_box_blocks(B, blockIdx.x - nsb, gridDim.x - nsb);
+    if (vb.id < P.nsb) vein_springs_at(P.V, vb.id * 256u + threadIdx.x);
+    else tri_box_blocks(P.B, vb.id - P.nsb, vb.count - P.nsb);
 }
+__global__ void __launch_bounds__(256) k_vein_prep(VeinPrepArgs P) { vein_prep_body(P, hw_block()); }
 
 // ---- processing order of the cells: by type, then by the Morton code of the
 // cell centre (previous frame) on a 1024^3 lattice over the particle-grid box,
@@ -1239,14 +1286,14 @@ __device__ __forceinline__ uint32_t lattice10(float c, float lo, float ext)
     return (uint32_t)t;
 }
 
-__global__ void __launch_bounds__(256) k_order_key(TypeTable tt, GridGeom g, const float* __restrict__ centers,
-                                                   uint32_t* __restrict__ okey, uint32_t* __restrict__ ocount,
-                                                   uint64_t* __restrict__ ostatus, int32_t otiles,
-                                                   uint32_t* __restrict__ octrl)
+__device__ __forceinline__ void order_key_body(const OrderKeyArgs& O, VBlock vb)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int32_t t = c; t < otiles; t += gridDim.x * blockDim.x) ostatus[t] = 0ull;
-    if (c == 0) octrl[0] = 0u;
+    const TypeTable& tt = O.tt;
+    const GridGeom& g = O.g;
+    const float* __restrict__ centers = O.centers;
+    const int32_t c = vb.id * blockDim.x + threadIdx.x;
+    for (int32_t t = c; t < O.otiles; t += (int32_t)(vb.count * blockDim.x)) O.ostatus[t] = 0ull;
+    if (c == 0) O.octrl[0] = 0u;
     if (c >= tt.nc) return;
     int T = 0;
     while (T + 1 < tt.nt && c >= tt.cstart[T + 1]) ++T;
@@ -1256,26 +1303,25 @@ __global__ void __launch_bounds__(256) k_order_key(TypeTable tt, GridGeom g, con
     const uint32_t qz = lattice10(centers[2 * nc + c], g.minv[2], g.extent[2]);
     const uint32_t m = (spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2)) >> (30 - kOrderBits);
     const uint32_t key = ((uint32_t)T << kOrderBits) | m;
-    okey[c] = key;
-    atomicAdd(&ocount[key], 1u);
+    O.okey[c] = key;
+    atomicAdd(&O.ocount[key], 1u);
 }
+__global__ void __launch_bounds__(256) k_order_key(OrderKeyArgs O) { order_key_body(O, hw_block()); }
 
-__global__ void __launch_bounds__(256) k_order_scatter(int32_t nc, const uint32_t* __restrict__ okey,
-                                                       uint32_t* __restrict__ ocount,
-                                                       const uint32_t* __restrict__ ostart, int32_t* __restrict__ corder,
-                                                       uint32_t* __restrict__ flags)
+__device__ __forceinline__ void order_scatter_body(const OrderScatterArgs& O, VBlock vb)
 {
-    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const uint32_t key = okey[c];
-    const uint32_t old = atomicSub(&ocount[key], 1u);  // counts return to zero
-    const uint32_t pos = ostart[key] + old - 1u;
-    if (old == 0u || pos >= (uint32_t)nc) {
-        atomicOr(flags, BCS_FLAG_SCAN_TIMEOUT);
+    const int32_t c = vb.id * blockDim.x + threadIdx.x;
+    if (c >= O.nc) return;
+    const uint32_t key = O.okey[c];
+    const uint32_t old = atomicSub(&O.ocount[key], 1u);  // counts return to zero
+    const uint32_t pos = O.ostart[key] + old - 1u;
+    if (old == 0u || pos >= (uint32_t)O.nc) {
+        atomicOr(O.flags, BCS_FLAG_SCAN_TIMEOUT);
         return;
     }
-    corder[pos] = c;
+    O.corder[pos] = c;
 }
+__global__ void __launch_bounds__(256) k_order_scatter(OrderScatterArgs O) { order_scatter_body(O, hw_block()); }
 
 // Per triangle-grid cell: mask of the occupied cells of its neighbourhood
 // (ranges of axis_range_u on the cell's own coordinates), over the visit rank.
@@ -1423,11 +1469,12 @@ int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, u
 {
     if (tt.nc <= 0) return BCS_OK;
     const int nb = (tt.nc + 255) / 256;
-    hipLaunchKernelGGL(k_order_key, dim3(std::max(nb, (otiles + 255) / 256)), dim3(256), 0, st, tt, g, centers,
-                       okey, ocount, ostatus, otiles, octrl);
+    const OrderKeyArgs ka{tt, g, centers, okey, ocount, ostatus, otiles, octrl};
+    hipLaunchKernelGGL(k_order_key, dim3(std::max(nb, (otiles + 255) / 256)), dim3(256), 0, st, ka);
     const int rc = scan_exclusive(ocount, ostart, (int64_t)tt.nt << kOrderBits, ostatus, otiles, octrl, flags, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, st, tt.nc, okey, ocount, ostart, corder, flags);
+    const OrderScatterArgs sa{tt.nc, okey, ocount, ostart, corder, flags};
+    hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, st, sa);
     return BCS_OK;
 }
 
@@ -1453,15 +1500,19 @@ void launch_tri_nmask(const DeviceGrid& tg, uint32_t* nmask, hipStream_t st)
 
 // {particle, blood cell} of every lane of every chunk, from the processing
 // order (run whenever corder changes).
-__global__ void __launch_bounds__(kBlock) k_chunk_map(StepArgs A, int2* __restrict__ out)
+__device__ __forceinline__ void chunk_map_body(const StepArgs& A, int2* __restrict__ out, VBlock vb)
 {
-    const int chunk = blockIdx.x, l = threadIdx.x;
+    const int chunk = (int)vb.id, l = threadIdx.x;
     const int T = A.chunk_type[chunk];
     const int p = A.tt.ppc[T];
     const bool act = l < A.chunk_n[chunk] * p;
     int32_t cell = 0;
     const int32_t i = act ? chunk_particle(A, T, A.chunk_first[chunk], l / p, l % p, p, cell) : -1;
     out[(size_t)chunk * kBlock + l] = make_int2(i, act ? cell : 0);
+}
+__global__ void __launch_bounds__(kBlock) k_chunk_map(StepArgs A, int2* __restrict__ out)
+{
+    chunk_map_body(A, out, hw_block());
 }
 
 static dim3 chunk_grid(int nchunks) { return dim3(8 * ((nchunks + 7) / 8)); }
@@ -1544,14 +1595,20 @@ void launch_tri_records(const DeviceGrid& tg, const uint32_t* idx, uint4* trun, 
         hipLaunchKernelGGL(k_tri_records, dim3((tg.n + 255) / 256), dim3(256), 0, st, tg.runs, tg.ctrl, tg.table,
                            tg.keys_sorted, tg.perm, tg.n, idx, trun, trec);
 }
-void launch_vein_prep(const VeinArgs& V, int32_t ntri, const uint4* trun, const uint4* trec, float4* tdata,
-                      float4* aabb, hipStream_t st)
+VeinPrepArgs vein_prep_args(const VeinArgs& V, int32_t ntri, const uint4* trun, const uint4* trec, float4* tdata,
+                            float4* aabb, uint32_t* blocks)
 {
     const uint32_t nsb = (uint32_t)((std::max(V.nv, 0) + 255) / 256);
     const uint32_t ntb = ntri > 0 ? kTriBoxBlocks : 0u;
-    if (nsb + ntb > 0)
-        hipLaunchKernelGGL(k_vein_prep, dim3(nsb + ntb), dim3(256), 0, st, V,
-                           tri_box_args(ntri, trun, trec, V.X, V.nv, tdata, aabb), nsb);
+    *blocks = nsb + ntb;
+    return VeinPrepArgs{V, tri_box_args(ntri, trun, trec, V.X, V.nv, tdata, aabb), nsb};
+}
+void launch_vein_prep(const VeinArgs& V, int32_t ntri, const uint4* trun, const uint4* trec, float4* tdata,
+                      float4* aabb, hipStream_t st)
+{
+    uint32_t blocks = 0;
+    const VeinPrepArgs P = vein_prep_args(V, ntri, trun, trec, tdata, aabb, &blocks);
+    if (blocks > 0) hipLaunchKernelGGL(k_vein_prep, dim3(blocks), dim3(256), 0, st, P);
 }
 void launch_gather_start(const uint2* table, const uint32_t* idx, int32_t n, uint32_t* out, hipStream_t st)
 {
@@ -1569,6 +1626,172 @@ void launch_dense_ranges(const uint32_t* keys_sorted, int32_t n, uint64_t cell_c
 void launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st)
 {
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, (const float4*)src, (float4*)dst, n16);
+}
+
+// Network input of every cell of the surrogate's type, one wave per cell
+// (lane a = the cell's particle a): 126 centroid-relative positions /
+// pos_scale, 126 velocities relative to the cell's mean velocity / vel_scale,
+// 42 contact flags of the last frame, zeros to 320 columns (bf16 X); f32
+// (optional) receives the first 294 columns unscaled.
+__global__ void __launch_bounds__(256) k_surr_features(const float* __restrict__ pos, const float* __restrict__ vel,
+                                                       int64_t np, int32_t pstart, int32_t ncells,
+                                                       const uint8_t* __restrict__ touched, float pos_scale,
+                                                       float vel_scale, uint16_t* __restrict__ X,
+                                                       float* __restrict__ f32)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= ncells) return;  // whole waves: no barrier below
+    const bool act = lane < kSurrPpc;
+    const int64_t i = pstart + c * kSurrPpc + lane;
+    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float flag = 0.f;
+    if (act) {
+        v[0] = pos[i];
+        v[1] = pos[np + i];
+        v[2] = pos[2 * np + i];
+        v[3] = vel[i];
+        v[4] = vel[np + i];
+        v[5] = vel[2 * np + i];
+        flag = touched && touched[i] ? 1.0f : 0.0f;
+    }
+    float mean[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        float t = v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        mean[k] = t / (float)kSurrPpc;
+    }
+    uint16_t* row = X + c * kSurrInPad;
+    if (act) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float rp = v[d] - mean[d], rv = v[3 + d] - mean[3 + d];
+            row[3 * lane + d] = bf16_bits(rp / pos_scale);
+            row[126 + 3 * lane + d] = bf16_bits(rv / vel_scale);
+            if (f32) {
+                f32[c * kSurrIn + 3 * lane + d] = rp;
+                f32[c * kSurrIn + 126 + 3 * lane + d] = rv;
+            }
+        }
+        row[252 + lane] = bf16_bits(flag);
+        if (f32) f32[c * kSurrIn + 252 + lane] = flag;
+    }
+    if (lane < kSurrInPad - kSurrIn) row[kSurrIn + lane] = 0;
+}
+
+void launch_surr_features(const float* pos, const float* vel, int64_t np, int32_t pstart, int32_t ncells,
+                          const uint8_t* touched, float pos_scale, float vel_scale, uint16_t* X, float* f32,
+                          hipStream_t st)
+{
+    if (ncells > 0)
+        hipLaunchKernelGGL(k_surr_features, dim3((unsigned)((ncells + 3) / 4)), dim3(256), 0, st, pos, vel, np, pstart,
+                           ncells, touched, pos_scale, vel_scale, X, f32);
+}
+void launch_cells_surr(const StepArgs& A, int nchunks, hipStream_t st)
+{
+    if (nchunks > 0) hipLaunchKernelGGL((k_cells<true, true>), chunk_grid(nchunks), dim3(kBlock), 0, st, A);
+}
+
+// ---- batched launches: one launch per kernel for many simulations ----------
+// Each kernel below runs the body of its single-simulation form on the
+// virtual blocks of every simulation k (arguments a[k]); see for_virtual_blocks.
+
+__global__ void __launch_bounds__(kBlock) k_raycand_batch(const StepArgs* __restrict__ a,
+                                                          const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { raycand_block<false>(a[k], vb); });
+}
+__global__ void __launch_bounds__(kBlock) k_raycand_ray_batch(const StepArgs* __restrict__ a,
+                                                              const uint32_t* __restrict__ start, int32_t nsim,
+                                                              uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { raycand_block<true>(a[k], vb); });
+}
+__global__ void __launch_bounds__(256) k_ray_batch(const StepArgs* __restrict__ a, const uint32_t* __restrict__ split,
+                                                   const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { ray_body(a[k], split[k], vb); });
+}
+__global__ void __launch_bounds__(kBlock) k_cells_batch(const StepArgs* __restrict__ a,
+                                                        const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { cells_body<true>(a[k], vb); });
+}
+__global__ void __launch_bounds__(256) k_vein_batch(const VeinArgs* __restrict__ a, const uint32_t* __restrict__ start,
+                                                    int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { vein_body<7>(a[k], vb); });
+}
+__global__ void __launch_bounds__(256) k_vein_prep_batch(const VeinPrepArgs* __restrict__ a,
+                                                         const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { vein_prep_body(a[k], vb); });
+}
+__global__ void __launch_bounds__(256) k_order_key_batch(const OrderKeyArgs* __restrict__ a,
+                                                         const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { order_key_body(a[k], vb); });
+}
+__global__ void __launch_bounds__(256) k_order_scatter_batch(const OrderScatterArgs* __restrict__ a,
+                                                             const uint32_t* __restrict__ start, int32_t nsim,
+                                                             uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { order_scatter_body(a[k], vb); });
+}
+__global__ void __launch_bounds__(kBlock) k_chunk_map_batch(const StepArgs* __restrict__ a, int2* const* __restrict__ out,
+                                                            const uint32_t* __restrict__ start, int32_t nsim,
+                                                            uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { chunk_map_body(a[k], out[k], vb); });
+}
+
+// Blocks of each single-simulation launch, as the batched launches split
+// them (the same counts the launchers above use).
+uint32_t cell_blocks(int nchunks) { return nchunks > 0 ? chunk_grid(nchunks).x : 0u; }
+uint32_t ray_blocks(int nchunks) { return nchunks > 0 ? chunk_grid(nchunks).x * ray_split(nchunks) : 0u; }
+uint32_t vein_blocks(int32_t nv) { return (uint32_t)((std::max(nv, 1) + 255) / 256); }
+uint32_t order_key_blocks(const TypeTable& tt, int32_t otiles)
+{
+    return tt.nc > 0 ? (uint32_t)std::max((tt.nc + 255) / 256, (otiles + 255) / 256) : 0u;
+}
+uint32_t order_scatter_blocks(const TypeTable& tt) { return tt.nc > 0 ? (uint32_t)((tt.nc + 255) / 256) : 0u; }
+
+void batch_raycand(bool fused, const StepArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run,
+                   hipStream_t st)
+{
+    if (!grid) return;
+    if (fused) hipLaunchKernelGGL(k_raycand_ray_batch, dim3(grid), dim3(kBlock), 0, st, a, start, nsim, run);
+    else hipLaunchKernelGGL(k_raycand_batch, dim3(grid), dim3(kBlock), 0, st, a, start, nsim, run);
+}
+void batch_ray(const StepArgs* a, const uint32_t* split, const uint32_t* start, int32_t nsim, uint32_t grid,
+               uint32_t run, hipStream_t st)
+{
+    if (grid) hipLaunchKernelGGL(k_ray_batch, dim3(grid), dim3(256), 0, st, a, split, start, nsim, run);
+}
+void batch_cells(const StepArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run, hipStream_t st)
+{
+    if (grid) hipLaunchKernelGGL(k_cells_batch, dim3(grid), dim3(kBlock), 0, st, a, start, nsim, run);
+}
+void batch_vein(const VeinArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run, hipStream_t st)
+{
+    if (grid) hipLaunchKernelGGL(k_vein_batch, dim3(grid), dim3(256), 0, st, a, start, nsim, run);
+}
+void batch_vein_prep(const VeinPrepArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run,
+                     hipStream_t st)
+{
+    if (grid) hipLaunchKernelGGL(k_vein_prep_batch, dim3(grid), dim3(256), 0, st, a, start, nsim, run);
+}
+void batch_order(const OrderKeyArgs* ka, const uint32_t* kstart, uint32_t kgrid, const ScanArgs* sa,
+                 const uint32_t* sstart, uint32_t sgrid, const OrderScatterArgs* oa, const uint32_t* ostart,
+                 uint32_t ogrid, const StepArgs* a, int2* const* out, const uint32_t* cstart, uint32_t cgrid,
+                 int32_t nsim, uint32_t run, hipStream_t st)
+{
+    if (kgrid) hipLaunchKernelGGL(k_order_key_batch, dim3(kgrid), dim3(256), 0, st, ka, kstart, nsim, run);
+    scan_batch(sa, sstart, nsim, sgrid, run, st);
+    if (ogrid) hipLaunchKernelGGL(k_order_scatter_batch, dim3(ogrid), dim3(256), 0, st, oa, ostart, nsim, run);
+    if (cgrid) hipLaunchKernelGGL(k_chunk_map_batch, dim3(cgrid), dim3(kBlock), 0, st, a, out, cstart, nsim, run);
 }
 
 }  // namespace bcs

@@ -40,9 +40,27 @@ struct PhysConst {
     float endr[kMaxEndings];
 };
 
+// The learned spring surrogate inside the cell kernel (k_cells<.., true>):
+// for the cells of type `type` the spring force S of each particle is
+// replaced by the network's prediction when the cell's largest error
+// |d_hat - (dt^2/4) S| is at most tol (else the plain S stays: fallback).
+struct SurrArgs {
+    const float* D;        // network output per cell of `type`: [cell - cstart][3a + axis] (ld 128),
+                           // normalised units (times out_scale: spring displacement); null: no prediction
+    float* S_out;          // sampling: (dt^2/4) S per particle of `type`, same layout; null: none
+    uint8_t* touched;      // per particle: 1 if it had a contact in this frame's search (null: not kept)
+    uint32_t* stats;       // [0] accepted cells, [1] fallback cells, [2] max error of accepted cells (float
+                           // bits), [3] max error of all predicted cells (float bits)
+    int32_t type;          // the served type (ppc 42); -1: none
+    float scale;           // dt^2 / 4
+    float out_scale;       // network units -> displacement
+    float tol;             // accepted when the cell's error <= tol (tol <= 0: every cell falls back)
+};
+
 struct StepArgs {
     TypeTable tt;
     PhysConst pc;
+    SurrArgs surr;
     // particles (SoA, stride np)
     float* pos;
     float* vel;
@@ -112,6 +130,42 @@ struct VeinArgs {
     const uint32_t* tidx;
     uint32_t* flags;
     uint64_t* step;
+};
+
+// Triangle data and cell boxes of one frame (the triangle role of k_vein_prep).
+struct TriBoxArgs {
+    const uint4* trun;  // per occupied cell r: {run start, cell key, count, 1}; w = 0 ends the list
+    const uint4* trec;  // per sorted slot u: {i0, i1, i2, triangle}
+    int32_t ntri;
+    const float* X;
+    int32_t nv;
+    float4* tdata;
+    float4* aabb;
+};
+// k_vein_prep: the first nsb blocks compute vein springs, the rest the triangle data.
+struct VeinPrepArgs {
+    VeinArgs V;
+    TriBoxArgs B;
+    uint32_t nsb;
+};
+// Processing order of the cells (k_order_key, scan, k_order_scatter).
+struct OrderKeyArgs {
+    TypeTable tt;
+    GridGeom g;
+    const float* centers;
+    uint32_t* okey;
+    uint32_t* ocount;
+    uint64_t* ostatus;
+    int32_t otiles;
+    uint32_t* octrl;
+};
+struct OrderScatterArgs {
+    int32_t nc;
+    const uint32_t* okey;
+    uint32_t* ocount;
+    const uint32_t* ostart;
+    int32_t* corder;
+    uint32_t* flags;
 };
 
 }  // namespace bcs

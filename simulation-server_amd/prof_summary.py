@@ -28,7 +28,14 @@ sys.path.insert(0, str(ROOT))
 
 
 def _short(name: str) -> str:
-    """Kernel name without its argument list (templates kept)."""
+    """Kernel name without return type, namespace and argument list
+    (template arguments kept): 'void bcs::k_cells<true>(StepArgs)' ->
+    'k_cells<true>'."""
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    if name.startswith("bcs::"):
+        name = name[5:]
     depth = 0
     for i, ch in enumerate(name):
         if ch == "<":
@@ -37,7 +44,7 @@ def _short(name: str) -> str:
             depth -= 1
         elif ch == "(" and depth == 0:
             return name[:i].strip()
-    return name.strip()
+    return name
 
 
 def _rows(directory: Path, suffix: str):
