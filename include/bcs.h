@@ -282,6 +282,19 @@ int bcs_sim_use_graph(bcs_sim* sim, int32_t enabled);
 #define BCS_RAY_FUSED 2
 int bcs_sim_set_ray_mode(bcs_sim* sim, int32_t mode);
 
+/* Where a frame runs the particle-particle contact search
+ * (calculateParticleCollisions, src/simulation/particle_collisions.cuh:104-269):
+ * BCS_CONTACT_CELLS = inside the cell kernel, whose lanes follow the particles
+ * of whole blood cells; BCS_CONTACT_SLOTS = in a kernel of its own whose lanes
+ * follow the sorted slots of the particle grid (neighbouring lanes share their
+ * neighbour cells), between a springs kernel and a response kernel;
+ * BCS_CONTACT_AUTO (the default) = SLOTS.  Results are identical in every
+ * mode.  No counterpart in the reference. */
+#define BCS_CONTACT_AUTO 0
+#define BCS_CONTACT_CELLS 1
+#define BCS_CONTACT_SLOTS 2
+int bcs_sim_set_contact_mode(bcs_sim* sim, int32_t mode);
+
 /* Bucket size of the particle sort: 2^shift grid cells per bucket (0..24).
  * bcs_sim_set_particles chooses it from the particles' positions; this call
  * overrides that choice until the next bcs_sim_set_particles.  Buckets of 2^8

@@ -35,6 +35,7 @@ FLAG_RADIUS_CLAMPED = 0x8
 FLAG_RAY_SERIAL = 0x10
 
 RAY_AUTO, RAY_SPLIT, RAY_FUSED = 0, 1, 2
+CONTACT_AUTO, CONTACT_CELLS, CONTACT_SLOTS = 0, 1, 2
 
 VEIN_CYLINDER, VEIN_Y = 0, 1
 CELL_ICOSAHEDRON, CELL_DISC42 = 0, 1
@@ -245,6 +246,7 @@ def _sig(lib):
     lib.bcs_sim_step.argtypes = [vp, C.c_int32]
     lib.bcs_sim_use_graph.argtypes = [vp, C.c_int32]
     lib.bcs_sim_set_ray_mode.argtypes = [vp, C.c_int32]
+    lib.bcs_sim_set_contact_mode.argtypes = [vp, C.c_int32]
     lib.bcs_sim_set_bucket_shift.argtypes = [vp, C.c_int32]
     lib.bcs_sim_synchronize.argtypes = [vp]
     lib.bcs_sim_time_steps.argtypes = [vp, C.c_int32, f32p]
@@ -535,6 +537,11 @@ class Simulation:
         """RAY_AUTO, RAY_SPLIT or RAY_FUSED: how frames run the vein ray test
         (identical results)."""
         check(self.lib.bcs_sim_set_ray_mode(self.h, int(mode)), "set_ray_mode")
+
+    def set_contact_mode(self, mode: int):
+        """CONTACT_AUTO, CONTACT_CELLS or CONTACT_SLOTS: where frames run the
+        particle contact search (identical results)."""
+        check(self.lib.bcs_sim_set_contact_mode(self.h, int(mode)), "set_contact_mode")
 
     def set_bucket_shift(self, shift: int):
         """2^shift grid cells per bucket of the particle sort (identical

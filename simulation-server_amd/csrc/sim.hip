@@ -48,6 +48,7 @@ void launch_surr_features(const float* pos, const float* vel, int64_t np, int32_
                           const uint8_t* touched, float pos_scale, float vel_scale, uint16_t* X, float* f32,
                           hipStream_t st);
 void launch_cells_surr(const StepArgs& A, int nchunks, hipStream_t st);
+void launch_split_part(const StepArgs& A, int part, bool surr, hipStream_t st);
 int launch_order(const TypeTable& tt, const GridGeom& g, const float* centers, uint32_t* okey, uint32_t* ocount,
                  uint32_t* ostart, uint64_t* ostatus, int32_t otiles, uint32_t* octrl, int32_t* corder,
                  uint32_t* flags, hipStream_t st);
@@ -138,6 +139,9 @@ struct bcs_sim {
     {
         return ray_mode == BCS_RAY_FUSED || (ray_mode == BCS_RAY_AUTO && ray_fuse_auto(nchunks));
     }
+    int32_t contact_mode = BCS_CONTACT_AUTO;
+    float4* fslot = nullptr;  // split frames: F per sorted slot
+    bool contact_split() const { return contact_mode != BCS_CONTACT_CELLS; }
     hipGraph_t graph = nullptr, graph_b = nullptr, graph_m = nullptr;
     std::vector<float> host_bs;
     std::vector<float> host_smallest;
@@ -195,6 +199,8 @@ struct bcs_sim {
         A.corder = corder; A.chunk_pi = chunk_pi; A.nchunks = nchunks;
         A.pg = pg.g; A.p_keys = pg.keys_sorted; A.p_perm = pg.perm; A.p_slot = pg.slot_of; A.p_table = pg.table; A.p_map = pg.bm; A.p_key_of = pg.keys;
         A.pvs = pg.pvs;
+        A.fslot = fslot;
+        A.ntiles = (int32_t)((np + kBlock - 1) / kBlock);
         A.tg = tg.g; A.t_table = tg.table; A.tdata = tdata; A.ntri = ntri;
         A.nv = nv; A.X = X; A.tidx = tidx;
         A.hits = hits; A.vhead = vhead; A.vnext = vnext; A.hctrl = hctrl;
@@ -344,10 +350,20 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
         te(kGridKernels + 2);
         if (s->surr && !s->sampling) enqueue_surrogate(s, st);
     }
-    tb(kGridKernels + 3);
-    if (s->surr_active()) launch_cells_surr(A, s->nchunks, st);
-    else launch_cells(A, s->nchunks, true, st);
-    te(kGridKernels + 3);
+    if (s->contact_split()) {
+        // springs per blood-cell chunk, contacts per tile of sorted slots,
+        // response and integration per chunk again
+        for (int part = 0; part < 3; ++part) {
+            tb(kGridKernels + 5 + part);
+            launch_split_part(A, part, s->surr_active(), st);
+            te(kGridKernels + 5 + part);
+        }
+    } else {
+        tb(kGridKernels + 3);
+        if (s->surr_active()) launch_cells_surr(A, s->nchunks, st);
+        else launch_cells(A, s->nchunks, true, st);
+        te(kGridKernels + 3);
+    }
     tb(kGridKernels + 4);
     if (tail_vein) launch_vein(Va, 7, st);
     te(kGridKernels + 4);
@@ -356,8 +372,8 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
 }
 
 const char* kKernelNames =
-    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_prep;k_raycand;k_ray;k_cells;k_vein";
-constexpr int kFusedKernels = kGridKernels + 5;
+    "k_key;k_scan;k_scatter;k_rank;k_occ;k_vein_prep;k_raycand;k_ray;k_cells;k_vein;k_springs;k_contacts;k_respond";
+constexpr int kFusedKernels = kGridKernels + 8;
 constexpr int32_t kGraphFrames = 16;  // frames per multi-frame graph launch
 
 }  // namespace
@@ -599,6 +615,7 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     RC(s->dalloc(&s->t_aabb, 2 * (size_t)tgeom.cell_count));
     RC(s->dalloc(&s->trun, (size_t)std::max(s->ntri, 1)));
     RC(s->dalloc(&s->trec, (size_t)std::max(s->ntri, 1)));
+    RC(s->dalloc(&s->fslot, (size_t)s->np));
     RC(build_triangle_grid(s.get()));
     HIPCHK(hipStreamSynchronize(s->stream));
     *out = s.release();
@@ -946,6 +963,19 @@ extern "C" int bcs_sim_set_ray_mode(bcs_sim* s, int32_t mode)
     return BCS_OK;
 }
 
+extern "C" int bcs_sim_set_contact_mode(bcs_sim* s, int32_t mode)
+{
+    if (!s) return fail(BCS_ERR_INVALID_ARGUMENT, "null simulation");
+    if (mode != BCS_CONTACT_AUTO && mode != BCS_CONTACT_CELLS && mode != BCS_CONTACT_SLOTS)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_set_contact_mode: mode is BCS_CONTACT_AUTO, _CELLS or _SLOTS");
+    if (mode == s->contact_mode) return BCS_OK;
+    RC(set_dev(s));
+    HIPCHK(hipStreamSynchronize(s->stream));  // the graphs hold the old launches
+    drop_graph(s);
+    s->contact_mode = mode;
+    return BCS_OK;
+}
+
 extern "C" int bcs_sim_synchronize(bcs_sim* s)
 {
     RC(set_dev(s));
@@ -986,6 +1016,11 @@ extern "C" int bcs_sim_profile_kernels(bcs_sim* s, int32_t n_steps, float* ms_ou
     }
     KernelTimer timer{&eb, &ee, 0};
     std::vector<double> acc(kFusedKernels, 0.0);
+    // the slots this frame records: k_cells or the three split kernels (an
+    // event that was never recorded must not be queried)
+    std::vector<bool> used(kFusedKernels, true);
+    if (s->contact_split()) used[kGridKernels + 3] = false;
+    else used[kGridKernels + 5] = used[kGridKernels + 6] = used[kGridKernels + 7] = false;
     int rc = BCS_OK;
     for (int32_t it = 0; it < n_steps && !rc; ++it) {
         rc = enqueue_fused_frame(s, &timer);
@@ -993,7 +1028,10 @@ extern "C" int bcs_sim_profile_kernels(bcs_sim* s, int32_t n_steps, float* ms_ou
         HIPCHK(hipStreamSynchronize(s->stream));
         for (int k = 0; k < kFusedKernels; ++k) {
             float ms = 0;
-            if (hipEventElapsedTime(&ms, eb[k], ee[k]) == hipSuccess) acc[k] += ms;
+            if (!used[k]) continue;
+            const hipError_t e = hipEventElapsedTime(&ms, eb[k], ee[k]);
+            if (e != hipSuccess && !rc) rc = fail(BCS_ERR_HIP, std::string("hipEventElapsedTime: ") + hipGetErrorString(e));
+            acc[k] += ms;
         }
     }
     if (s->ray_fused()) acc[kGridKernels + 2] = 0.0;  // no k_ray launch

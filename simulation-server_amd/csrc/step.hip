@@ -601,7 +601,104 @@ __device__ __forceinline__ void respond_integrate(const StepArgs& A, bool act, i
 
 static_assert(sizeof(StepArgs) <= 4096, "kernel arguments exceed 4 KiB");
 
-template <bool FUSED, bool SURR = false>
+// Contact forces on particle i from the occupied neighbour cells in `mask`
+// (rank r = (dx+1)*9 + (dy+1)*3 + (dz+1), from the occupancy map), visited in
+// the reference order (x, then y, then z; calculateParticleCollisions,
+// src/simulation/particle_collisions.cuh:104-269): each contribution is added
+// to F in visit order, slot by slot.  Tl is the launch type of i: the radius
+// of every partner is looked up through Tl's model.  Returns the number of
+// contacts.
+template <class Radius>
+__device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, int Tl, uint32_t key_of, uint32_t mask,
+                                                f3 x1, f3 v1, f3& F, uint32_t& fl, Radius radius)
+{
+    const TypeTable& tt = A.tt;
+    const PhysConst& pc = A.pc;
+    const int64_t np = tt.np;
+    const GridGeom& g = A.pg;
+    const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
+    // table entries {first slot, end slot} of the next kCellBatch occupied
+    // cells in visit order (empty: {0, 0}), taken off mask
+    auto load_cells = [&](uint2 (&ent)[kCellBatch]) {
+#pragma unroll
+        for (int k = 0; k < kCellBatch; ++k) {
+            ent[k] = make_uint2(0u, 0u);
+            if (mask) {
+                const int r = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                const int dx = r / 9 - 1, c = r % 9;
+                const uint2 e = A.p_table[(int64_t)key_of + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
+                ent[k] = make_uint2(e.x, min(e.x + e.y, (uint32_t)np));
+            }
+        }
+    };
+    const ModelIx mi = model_ix(tt, Tl);
+    const float r1 = radius(model_index(mi, i, fl));
+    uint32_t ncontact = 0;
+    bool more = mask != 0u;
+    // The occupied cells in visit order, kCellBatch at a time: their
+    // table entries are loaded together, then each cell's run of
+    // sorted-slot records is read 4 slots per load batch.
+    // Contributions are added in visit order, slot by slot.
+    while (more) {
+        uint2 ent[kCellBatch];
+        load_cells(ent);
+        // One stream of candidate slots over these cells in visit order
+        // (stream position p lies in cell k when cum[k] <= p < cum[k+1]);
+        // each load batch takes the next kContactBatch of them, across
+        // cell boundaries (a cell of one or two objects costs no round
+        // trip of its own).
+        uint32_t cum[kCellBatch + 1];
+        cum[0] = 0u;
+#pragma unroll
+        for (int k = 0; k < kCellBatch; ++k) cum[k + 1] = cum[k] + (ent[k].y > ent[k].x ? ent[k].y - ent[k].x : 0u);
+        const uint32_t total = cum[kCellBatch];
+        for (uint32_t p0 = 0; p0 < total; p0 += kContactBatch) {
+            uint32_t slot[kContactBatch];
+            const int nval = (int)min((uint32_t)kContactBatch, total - p0);
+#pragma unroll
+            for (int t = 0; t < kContactBatch; ++t) {
+                const uint32_t pp = p0 + (uint32_t)t;
+                uint32_t sl = 0u;
+#pragma unroll
+                for (int k = 0; k < kCellBatch; ++k)
+                    if (pp >= cum[k] && pp < cum[k + 1]) sl = ent[k].x + (pp - cum[k]);
+                slot[t] = sl;
+            }
+            float4 q[kContactBatch], w[kContactBatch];
+#pragma unroll
+            for (int t = 0; t < kContactBatch; ++t)
+                if (t < nval) {
+                    q[t] = A.pvs[2 * (size_t)slot[t]];
+                    w[t] = A.pvs[2 * (size_t)slot[t] + 1];
+                }
+#pragma unroll
+            for (int t = 0; t < kContactBatch; ++t) {
+                if (t >= nval) break;
+                const int32_t j = __float_as_int(q[t].w);
+                if (j == i) continue;
+                const float r2 = radius(model_index(mi, j, fl));
+                const f3 rel = sub(x1, mk(q[t].x, q[t].y, q[t].z));
+                const float d2 = dot(rel, rel);
+                const float minD = r1 + r2;
+                if (d2 <= minD * minD && d2 >= 0.0001f) {
+                    const f3 rv = sub(v1, mk(w[t].x, w[t].y, w[t].z));
+                    F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
+                    ++ncontact;
+                }
+            }
+        }
+        more = mask != 0u;
+    }
+    return ncontact;
+}
+
+// What one launch of the cell kernel does: the whole particle part of a
+// frame, or (split frames) only the centres, springs and environment, the
+// contact search and the response then running as k_contacts and k_respond.
+constexpr int kCellsWhole = 0, kCellsSprings = 1;
+
+template <bool FUSED, bool SURR = false, int MODE = kCellsWhole>
 __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
 {
     __shared__ uint32_t s_err[SURR ? kBlock : 1];  // per cell of the chunk: the surrogate's largest error
@@ -650,34 +747,23 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
         sx[l] = px; sy[l] = py; sz[l] = pz;
         svx[l] = A.vel[i]; svy[l] = A.vel[np + i]; svz[l] = A.vel[2 * np + i];
         sfx[l] = A.frc[i]; sfy[l] = A.frc[np + i]; sfz[l] = A.frc[2 * np + i];
-        key_of = A.p_key_of[i];
-        if (tt.nt > 1) slot_i = A.p_slot[i];
-        // the cell the grid build gave i (gridCellIds of its slot); the face
-        // clipping uses the current position, as the reference does
-        const int32_t xId = sat_i32((px - g.minv[0]) / g.cell[0]);
-        const int32_t yId = sat_i32((py - g.minv[1]) / g.cell[1]);
-        const int32_t zId = sat_i32((pz - g.minv[2]) / g.cell[2]);
-        NbrRange nr;
-        axis_range_i(xId, g.cnt[0], nr.xl, nr.xh);
-        axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
-        axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
-        mask = neighbour_mask(A.p_map, g, key_of, nr, cX, cXY, true);
-    }
-    // table entries {first slot, end slot} of the next kCellBatch occupied
-    // cells in visit order (empty: {0, 0}), taken off mask
-    auto load_cells = [&](uint2 (&ent)[kCellBatch]) {
-#pragma unroll
-        for (int k = 0; k < kCellBatch; ++k) {
-            ent[k] = make_uint2(0u, 0u);
-            if (mask) {
-                const int r = __builtin_ctz(mask);
-                mask &= mask - 1u;
-                const int dx = r / 9 - 1, c = r % 9;
-                const uint2 e = A.p_table[(int64_t)key_of + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
-                ent[k] = make_uint2(e.x, min(e.x + e.y, (uint32_t)np));
-            }
+        if (MODE == kCellsWhole) {
+            key_of = A.p_key_of[i];
+            if (tt.nt > 1) slot_i = A.p_slot[i];
+            // the cell the grid build gave i (gridCellIds of its slot); the
+            // face clipping uses the current position, as the reference does
+            const int32_t xId = sat_i32((px - g.minv[0]) / g.cell[0]);
+            const int32_t yId = sat_i32((py - g.minv[1]) / g.cell[1]);
+            const int32_t zId = sat_i32((pz - g.minv[2]) / g.cell[2]);
+            NbrRange nr;
+            axis_range_i(xId, g.cnt[0], nr.xl, nr.xh);
+            axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
+            axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
+            mask = neighbour_mask(A.p_map, g, key_of, nr, cX, cXY, true);
+        } else {
+            slot_i = A.p_slot[i];  // where F after the springs goes
         }
-    };
+    }
     // spring lists of this type's model vertices (one type per block), from
     // LDS when they fit
     const int m0 = tt.mstart[T];
@@ -699,7 +785,6 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
 
     uint32_t fl = 0;
     f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1, F0 = x1, cen = x1, newF = x1;
-    bool more = false;
     const int base = cl * p;
     if (act) {
         x1 = mk(sx[l], sy[l], sz[l]);
@@ -713,7 +798,6 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
             A.centers[tt.nc + cell] = cen.y;
             A.centers[2 * (int64_t)tt.nc + cell] = cen.z;
         }
-        more = mask != 0u;
         // ---- springs + environment (F of neighbours from the LDS snapshot)
         const int ma = m0 + a;
         auto spring_sum = [&](auto spb, auto spL, int e0, int e1) {
@@ -795,79 +879,117 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
         newF = add(newF, env);
         F = sdiv(add(F0, newF), 2.0f);
 
-        // ---- particle-particle contacts over the 27 neighbour cells, visited
-        // in the reference order (x, then y, then z); only occupied cells
-        // (from the occupancy map) are looked up in the cell table
-        // (calculateParticleCollisions, src/simulation/particle_collisions.cuh:104-269).
-        {
+        if constexpr (MODE == kCellsWhole) {
+            // ---- particle-particle contacts over the 27 neighbour cells
             int Tl = T;
             if (tt.nt > 1) Tl = type_of_index(tt, slot_i);
-            const ModelIx mi = model_ix(tt, Tl);
-            const float r1 = radius(model_index(mi, i, fl));
-            // The occupied cells in visit order, kCellBatch at a time: their
-            // table entries are loaded together, then each cell's run of
-            // sorted-slot records is read 4 slots per load batch.
-            // Contributions are added in visit order, slot by slot.
-            while (more) {
-                uint2 ent[kCellBatch];
-                load_cells(ent);
-                // One stream of candidate slots over these cells in visit order
-                // (stream position p lies in cell k when cum[k] <= p < cum[k+1]);
-                // each load batch takes the next kContactBatch of them, across
-                // cell boundaries (a cell of one or two objects costs no round
-                // trip of its own).
-                uint32_t cum[kCellBatch + 1];
-                cum[0] = 0u;
-#pragma unroll
-                for (int k = 0; k < kCellBatch; ++k) cum[k + 1] = cum[k] + (ent[k].y > ent[k].x ? ent[k].y - ent[k].x : 0u);
-                const uint32_t total = cum[kCellBatch];
-                for (uint32_t p0 = 0; p0 < total; p0 += kContactBatch) {
-                    uint32_t slot[kContactBatch];
-                    const int nval = (int)min((uint32_t)kContactBatch, total - p0);
-#pragma unroll
-                    for (int t = 0; t < kContactBatch; ++t) {
-                        const uint32_t pp = p0 + (uint32_t)t;
-                        uint32_t sl = 0u;
-#pragma unroll
-                        for (int k = 0; k < kCellBatch; ++k)
-                            if (pp >= cum[k] && pp < cum[k + 1]) sl = ent[k].x + (pp - cum[k]);
-                        slot[t] = sl;
-                    }
-                    float4 q[kContactBatch], w[kContactBatch];
-#pragma unroll
-                    for (int t = 0; t < kContactBatch; ++t)
-                        if (t < nval) {
-                            q[t] = A.pvs[2 * (size_t)slot[t]];
-                            w[t] = A.pvs[2 * (size_t)slot[t] + 1];
-                        }
-#pragma unroll
-                    for (int t = 0; t < kContactBatch; ++t) {
-                        if (t >= nval) break;
-                        const int32_t j = __float_as_int(q[t].w);
-                        if (j == i) continue;
-                        const float r2 = radius(model_index(mi, j, fl));
-                        const f3 rel = sub(x1, mk(q[t].x, q[t].y, q[t].z));
-                        const float d2 = dot(rel, rel);
-                        const float minD = r1 + r2;
-                        if (d2 <= minD * minD && d2 >= 0.0001f) {
-                            const f3 rv = sub(v1, mk(w[t].x, w[t].y, w[t].z));
-                            F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
-                            ++ncontact;
-                        }
-                    }
-                }
-                more = mask != 0u;
-            }
+            ncontact = contact_sum(A, i, Tl, key_of, mask, x1, v1, F, fl, radius);
+        } else {
+            A.fslot[slot_i] = make_float4(F.x, F.y, F.z, 0.0f);
         }
-        if (SURR && A.surr.touched) A.surr.touched[i] = ncontact > 0u ? 1 : 0;
+        if (MODE == kCellsWhole && SURR && A.surr.touched) A.surr.touched[i] = ncontact > 0u ? 1 : 0;
     }
-    respond_integrate<FUSED>(A, act, i, T, a, cl, cell, x1, v1, F, fl, s_tele);  // raises fl
+    if constexpr (MODE == kCellsWhole) respond_integrate<FUSED>(A, act, i, T, a, cl, cell, x1, v1, F, fl, s_tele);
+    else if (fl) atomicOr(A.flags, fl);
 }
 template <bool FUSED, bool SURR = false>
 __global__ void __launch_bounds__(kBlock) k_cells(StepArgs A)
 {
     cells_body<FUSED, SURR>(A, hw_block());
 }
+template <bool SURR>
+__global__ void __launch_bounds__(kBlock) k_springs(StepArgs A)
+{
+    cells_body<true, SURR, kCellsSprings>(A, hw_block());
+}
+
+// Contact search of a split frame, one lane per sorted slot s of the particle
+// grid.  The lanes of a wave hold particles of the same or adjacent grid
+// cells, so they walk nearly the same neighbour cells and read the same
+// records at the same time (in the cell kernel, whose lanes follow the
+// particles of whole blood cells, every lane reads records of its own).  F
+// of the particle after springs and environment comes from fslot[s], and F
+// after the contacts goes back there.
+template <bool SURR>
+__device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
+{
+    __shared__ float s_bs[kBsLds];
+    const TypeTable& tt = A.tt;
+    const int l = threadIdx.x;
+    const int tile = chunk_of_block(A.ntiles, vb.id);
+    if (tile < 0) return;
+    const bool bs_lds = tt.nm <= kBsLds;
+    if (bs_lds)
+        for (int t = l; t < tt.nm; t += kBlock) s_bs[t] = A.bs[t];
+    auto radius = [&](int32_t m) -> float { return bs_lds ? s_bs[m] : A.bs[m]; };
+    const int64_t np = tt.np;
+    const int64_t u = (int64_t)tile * kBlock + l;
+    const bool act = u < np;
+    const GridGeom& g = A.pg;
+    const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
+    uint32_t fl = 0, mask = 0u, key_of = 0u;
+    int32_t i = 0;
+    f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
+    if (act) {
+        const float4 q = A.pvs[2 * u], w = A.pvs[2 * u + 1], f = A.fslot[u];
+        i = __float_as_int(q.w);
+        x1 = mk(q.x, q.y, q.z);
+        v1 = mk(w.x, w.y, w.z);
+        F = mk(f.x, f.y, f.z);
+        key_of = A.p_keys[u];
+        const int32_t xId = sat_i32((x1.x - g.minv[0]) / g.cell[0]);
+        const int32_t yId = sat_i32((x1.y - g.minv[1]) / g.cell[1]);
+        const int32_t zId = sat_i32((x1.z - g.minv[2]) / g.cell[2]);
+        NbrRange nr;
+        axis_range_i(xId, g.cnt[0], nr.xl, nr.xh);
+        axis_range_i(yId, g.cnt[1], nr.yl, nr.yh);
+        axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
+        mask = neighbour_mask(A.p_map, g, key_of, nr, cX, cXY, true);
+    }
+    __syncthreads();  // s_bs
+    if (act) {
+        const int Tl = tt.nt > 1 ? type_of_index(tt, (int32_t)u) : 0;
+        const uint32_t nc = contact_sum(A, i, Tl, key_of, mask, x1, v1, F, fl, radius);
+        A.fslot[u] = make_float4(F.x, F.y, F.z, 0.0f);
+        if (SURR && A.surr.touched) A.surr.touched[i] = nc > 0u ? 1 : 0;
+    }
+    if (fl) atomicOr(A.flags, fl);
+}
+template <bool SURR>
+__global__ void __launch_bounds__(kBlock) k_contacts(StepArgs A)
+{
+    contacts_body<SURR>(A, hw_block());
+}
+
+// Last part of a split frame, per chunk of whole blood cells: the vein hit
+// response, integration and vein end, with F after the contacts from fslot.
+__device__ __forceinline__ void respond_body(const StepArgs& A, VBlock vb)
+{
+    __shared__ int s_tele[kBlock];
+    const int l = threadIdx.x;
+    const int chunk = chunk_of_block(A.nchunks, vb.id);
+    if (chunk < 0) return;
+    const TypeTable& tt = A.tt;
+    const int T = A.chunk_type[chunk];
+    const int ncell = A.chunk_n[chunk];
+    const int p = tt.ppc[T];
+    const int2 pcl = A.chunk_pi[(size_t)chunk * kBlock + l];
+    const bool act = pcl.x >= 0;
+    const int a = act ? l % p : 0;
+    const int cl = act ? l / p : 0;
+    const int32_t i = act ? pcl.x : 0, cell = act ? pcl.y : 0;
+    const int64_t np = tt.np;
+    f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
+    if (act) {
+        x1 = load3(A.pos, np, i);
+        v1 = load3(A.vel, np, i);
+        const float4 f = A.fslot[A.p_slot[i]];
+        F = mk(f.x, f.y, f.z);
+    }
+    if (l < ncell) s_tele[l] = 0;
+    respond_integrate<true>(A, act, i, T, a, cl, cell, x1, v1, F, 0u, s_tele);
+}
+__global__ void __launch_bounds__(kBlock) k_respond(StepArgs A) { respond_body(A, hw_block()); }
 
 // kRayLanes lanes per work item; the item's first hit is folded into the
 // particle's minimum (cell rank, sorted slot).  The hit test alone decides;
@@ -1692,6 +1814,22 @@ void launch_surr_features(const float* pos, const float* vel, int64_t np, int32_
 void launch_cells_surr(const StepArgs& A, int nchunks, hipStream_t st)
 {
     if (nchunks > 0) hipLaunchKernelGGL((k_cells<true, true>), chunk_grid(nchunks), dim3(kBlock), 0, st, A);
+}
+// The particle part of a split frame: k_springs (per chunk), k_contacts (per
+// tile of sorted slots), k_respond (per chunk).  part: 0, 1 or 2.
+void launch_split_part(const StepArgs& A, int part, bool surr, hipStream_t st)
+{
+    if (A.nchunks <= 0) return;
+    if (part == 0) {
+        if (surr) hipLaunchKernelGGL(k_springs<true>, chunk_grid(A.nchunks), dim3(kBlock), 0, st, A);
+        else hipLaunchKernelGGL(k_springs<false>, chunk_grid(A.nchunks), dim3(kBlock), 0, st, A);
+    } else if (part == 1) {
+        if (A.ntiles <= 0) return;
+        if (surr) hipLaunchKernelGGL(k_contacts<true>, chunk_grid(A.ntiles), dim3(kBlock), 0, st, A);
+        else hipLaunchKernelGGL(k_contacts<false>, chunk_grid(A.ntiles), dim3(kBlock), 0, st, A);
+    } else {
+        hipLaunchKernelGGL(k_respond, chunk_grid(A.nchunks), dim3(kBlock), 0, st, A);
+    }
 }
 
 // ---- batched launches: one launch per kernel for many simulations ----------

@@ -90,6 +90,8 @@ struct StepArgs {
     const uint32_t* p_key_of;  // per particle: its cell key in the last build (gridCellIds by particle)
     const float4* pvs;      // per sorted slot u, as of the last grid build: pvs[2u] = (x, y, z,
                             // particle perm[u] as int bits), pvs[2u+1] = (vx, vy, vz, 0)
+    float4* fslot;          // split frames: per sorted slot, F after the springs, then after the contacts
+    int32_t ntiles;         // split frames: slot tiles of kBlock lanes (k_contacts blocks)
     // triangle grid + vein
     GridGeom tg;
     const uint2* t_table;

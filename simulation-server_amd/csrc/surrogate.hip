@@ -41,13 +41,23 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 __host__ __device__ __forceinline__ uint16_t to_bf16(float f) { return bf16_bits(f); }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// GELU(x) = x * Phi(x) with the normal CDF from erff; and its derivative.
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU in its tanh form (Hendrycks & Gimpel 2016): 0.5 x (1 + tanh(z / 2))
+// = x / (1 + exp(-z)), z = 2 sqrt(2/pi) (x + 0.044715 x^3); and its
+// derivative s + x s (1 - s) dz/dx with s = 1 / (1 + exp(-z)).  exp through
+// the hardware base-2 exponential and 1/(1 + e) through its reciprocal: a
+// few instructions per element, against a result rounded to bf16.
+constexpr float kGeluC = 1.5957691216057308f;  // 2 sqrt(2/pi)
+constexpr float kGeluA = 0.044715f;
+__device__ __forceinline__ float gelu_sig(float x)
+{
+    const float z = kGeluC * (x + kGeluA * x * x * x);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-    return cdf + x * pdf;
+    const float sg = gelu_sig(x);
+    return sg + x * sg * (1.0f - sg) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
 }
 
 constexpr int kBM = 128, kBN = 128, kBK = 32;
@@ -239,6 +249,128 @@ __global__ void __launch_bounds__(256) k_gather_rows(const uint16_t* __restrict_
     }
 }
 
+// ---- fused inference: all four layers in one kernel ------------------------
+// A block takes kFM rows (cells).  Their activations stay in LDS (bf16, row
+// pitch kHPitch) from the input to the last hidden layer; each weight matrix
+// streams through LDS 32 columns at a time, double-buffered, fetched into
+// registers one chunk ahead.  The 4 waves split the layer output 2 x 2:
+// 64 rows x 128 columns per wave for the hidden layers (4 x 8 MFMA tiles),
+// 64 x 64 for the 128-column output layer.  Only the input rows and the
+// output rows touch HBM.
+constexpr int kFM = 128;
+constexpr int kHPitch = kSurrInPad + 8;  // 328 bf16: rows 16 B apart modulo the LDS banks
+constexpr int kWPitch = kBK + 8;         // 40 bf16
+constexpr int kWBuf = kSurrHidden * kWPitch;
+
+// Columns [k0, k0 + 32) of the first `rows` rows of W (ld K): 4 16-byte pieces
+// per row, piece c = tid + 256 q of the block.
+__device__ __forceinline__ void w_chunk_load(const uint16_t* __restrict__ W, int K, int rows, int k0, int tid,
+                                             uint4 (&r)[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = tid + 256 * q, row = c >> 2, part = c & 3;
+        r[q] = row < rows ? *reinterpret_cast<const uint4*>(W + (size_t)row * K + k0 + 8 * part)
+                          : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+__device__ __forceinline__ void w_chunk_store(uint16_t* buf, int rows, int tid, const uint4 (&r)[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = tid + 256 * q, row = c >> 2, part = c & 3;
+        if (row < rows) *reinterpret_cast<uint4*>(buf + row * kWPitch + 8 * part) = r[q];
+    }
+}
+
+// acc = H[wm .. wm + 64) x W^T[.., wn .. wn + 16 NT) over K columns; W has
+// 32 NT rows (256 or 128).  Ends with a barrier: every wave is done with H.
+template <int NT>
+__device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, int K, const uint16_t* H, uint16_t* wbuf,
+                                          f32x4 (&acc)[4][NT], int wm, int wn, int lane, int tid)
+{
+    constexpr int rows = 32 * NT;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    uint4 r[4];
+    w_chunk_load(W, K, rows, 0, tid, r);
+    w_chunk_store(wbuf, rows, tid, r);
+    __syncthreads();
+    const int nks = K / kBK;
+    for (int ks = 0; ks < nks; ++ks) {
+        const uint16_t* cur = wbuf + (ks & 1) * kWBuf;
+        if (ks + 1 < nks) w_chunk_load(W, K, rows, (ks + 1) * kBK, tid, r);
+        bf16x8 a[4], b[NT];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            a[i] = *reinterpret_cast<const bf16x8*>(H + (wm + 16 * i + (lane & 15)) * kHPitch + ks * kBK +
+                                                    8 * (lane >> 4));
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+            b[j] = *reinterpret_cast<const bf16x8*>(cur + (wn + 16 * j + (lane & 15)) * kWPitch + 8 * (lane >> 4));
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (ks + 1 < nks) w_chunk_store(wbuf + ((ks + 1) & 1) * kWBuf, rows, tid, r);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mlp_infer(SurrNet net, const uint16_t* __restrict__ X, int32_t n,
+                                                   float* __restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t H[kFM * kHPitch];
+    __shared__ __attribute__((aligned(16))) uint16_t wbuf[2 * kWBuf];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t m0 = (int64_t)blockIdx.x * kFM;
+    // input rows (zeros past the last row)
+    for (int c = tid; c < kFM * (kSurrInPad / 8); c += 256) {
+        const int row = c / (kSurrInPad / 8), part = c % (kSurrInPad / 8);
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (m0 + row < n) v = *reinterpret_cast<const uint4*>(X + (m0 + row) * kSurrInPad + 8 * part);
+        *reinterpret_cast<uint4*>(H + row * kHPitch + 8 * part) = v;
+    }
+    __syncthreads();
+    const int wm = (wave >> 1) * 64;
+    {
+        f32x4 acc[4][8];
+        const int wn = (wave & 1) * 128;
+        for (int l = 0; l < 3; ++l) {
+            mlp_layer<8>(net.W[l], l == 0 ? kSurrInPad : kSurrHidden, H, wbuf, acc, wm, wn, lane, tid);
+            // bias, GELU, bf16: the next layer's input, in place
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int nn = wn + 16 * j + (lane & 15);
+                const float bn = net.b[l][nn];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        H[(wm + 16 * i + 4 * (lane >> 4) + r) * kHPitch + nn] = to_bf16(gelu(acc[i][j][r] + bn));
+            }
+            __syncthreads();
+        }
+    }
+    f32x4 acc[4][4];
+    const int wn = (wave & 1) * 64;
+    mlp_layer<4>(net.W[3], kSurrHidden, H, wbuf, acc, wm, wn, lane, tid);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int nn = wn + 16 * j + (lane & 15);
+        const float bn = net.b[3][nn];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                if (m < n) out[m * kSurrOutPad + nn] = acc[i][j][r] + bn;
+            }
+    }
+}
+
 void gemm(const uint16_t* A, int64_t lda, const uint16_t* B, int64_t ldb, int32_t M, int32_t N, int32_t K,
           const float* bias, float* Cf, int64_t ldcf, uint16_t* Cb, int64_t ldcb, int act, hipStream_t st)
 {
@@ -275,14 +407,7 @@ void surrogate_infer(const SurrNet& net, const uint16_t* X, int32_t n, uint16_t*
                      hipStream_t st)
 {
     if (n <= 0) return;
-    gemm(X, kSurrInPad, net.W[0], kSurrInPad, n, kSurrHidden, kSurrInPad, net.b[0], nullptr, 0, act0, kSurrHidden, 1,
-         st);
-    gemm(act0, kSurrHidden, net.W[1], kSurrHidden, n, kSurrHidden, kSurrHidden, net.b[1], nullptr, 0, act1,
-         kSurrHidden, 1, st);
-    gemm(act1, kSurrHidden, net.W[2], kSurrHidden, n, kSurrHidden, kSurrHidden, net.b[2], nullptr, 0, act0,
-         kSurrHidden, 1, st);
-    gemm(act0, kSurrHidden, net.W[3], kSurrHidden, n, kSurrOutPad, kSurrHidden, net.b[3], out, kSurrOutPad, nullptr, 0,
-         0, st);
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + kFM - 1) / kFM)), dim3(256), 0, st, net, X, n, out);
 }
 
 }  // namespace bcs

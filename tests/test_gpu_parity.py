@@ -283,6 +283,31 @@ def test_ray_mode_switch_mid_run(bcs, orc):
     assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], "ray mode switch vein")
 
 
+@pytest.mark.parametrize("mode", ["cells", "slots"])
+def test_contact_modes_match_oracle(bcs, orc, mode):
+    """The contact search inside the cell kernel (lanes by blood cell) or in a
+    kernel of its own (lanes by sorted grid slot, between a springs kernel and
+    a response kernel): 40 graph frames of the mixed scene against the oracle,
+    then a switch to the other mode for 8 more."""
+    sc = _scene(bcs, "mixed")
+    sim, o = _pair(bcs, orc, sc)
+    first, second = (bcs.CONTACT_CELLS, bcs.CONTACT_SLOTS) if mode == "cells" else (bcs.CONTACT_SLOTS,
+                                                                                       bcs.CONTACT_CELLS)
+    sim.set_contact_mode(first)
+    sim.step(40)
+    o.step(40)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, f"contacts by {mode}, 40 frames: {n}")
+    sim.set_contact_mode(second)
+    sim.step(8)
+    o.step(8)
+    for x, y, n in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, f"contacts by {mode}, then switched: {n}")
+    assert_bits_equal(sim.get_vein()[0], o.get_vein()[0], f"contacts by {mode}: vein")
+    with pytest.raises(bcs.BcsError):
+        sim.set_contact_mode(5)
+
+
 @pytest.mark.parametrize("shift", [4, 8, 11])
 def test_bucket_sizes_keep_parity(bcs, orc, shift):
     """40 graph frames with buckets of 2^4 cells (occupancy map from its own

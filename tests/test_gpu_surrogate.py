@@ -5,7 +5,6 @@ from the plain method against the oracle, the bit-exact fallback, and the
 per-cell error bound of surrogate frames."""
 import numpy as np
 import pytest
-from scipy.special import erf
 
 from conftest import assert_bits_equal
 
@@ -21,12 +20,29 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32)
 
 
+def gelu_tanh(z):
+    """GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))."""
+    return 0.5 * z * (1.0 + np.tanh(np.sqrt(2.0 / np.pi) * (z + 0.044715 * z ** 3)))
+
+
+def test_gelu_tanh_reference_values():
+    """The reference activation at known points: GELU(0) = 0, the odd part
+    GELU(x) - GELU(-x) = x, the tanh-form values at 1 and -1 (0.841192,
+    -0.158808), and within 2e-4 of the erf form x Phi(x) there (0.841345,
+    -0.158655)."""
+    assert gelu_tanh(0.0) == 0.0
+    x = np.linspace(-6, 6, 101)
+    np.testing.assert_allclose(gelu_tanh(x) - gelu_tanh(-x), x, atol=1e-12)
+    np.testing.assert_allclose(gelu_tanh(np.array([1.0, -1.0])), [0.8411920, -0.1588080], atol=1e-6)
+    np.testing.assert_allclose(gelu_tanh(np.array([1.0, -1.0])), [0.8413447, -0.1586553], atol=2e-4)
+
+
 def reference_forward(ws, bs, x):
     """x: rows already divided by the feature scales."""
     h = bf16(x).astype(np.float64)
     for l in range(3):
         z = h @ bf16(ws[l]).astype(np.float64).T + bs[l]
-        h = bf16((0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))).astype(np.float32)).astype(np.float64)
+        h = bf16(gelu_tanh(z).astype(np.float32)).astype(np.float64)
     return h @ bf16(ws[3]).astype(np.float64).T + bs[3]
 
 
