@@ -94,6 +94,9 @@ typedef struct bcs_physics {
                                           least 65,536 in all and 1,024 per cell block); n > 0 = n in
                                           all.  Rays that do not fit are scanned serially (same
                                           result, BCS_FLAG_RAY_SERIAL). */
+    int32_t particle_rk4;              /* 1: the reference's USE_RUNGE_KUTTA_FOR_PARTICLE build
+                                          (src/simulation/physics.cuh:10,56-73,
+                                          src/objects/blood_cells.cu:114-119,166-171); 0: Heun (its default) */
 } bcs_physics;
 
 /* ---- blood-cell type ---------------------------------------------------

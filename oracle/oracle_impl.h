@@ -548,6 +548,13 @@ static void ORC(centers)(ORC(sim)* s)
  * accumulateEnvironmentForcesForParticles, physics.cuh:102-120.  F of
  * neighbours is read from the pre-phase snapshot (the reference reads it while
  * neighbours overwrite it). */
+/* springMassForceWithDampingForParticle, src/simulation/physics.cuh:24-27. */
+static inline R ORC(spring_particle)(const ORC(sim)* s, V3 dP, V3 dv, R L, uint32_t* fl)
+{
+    return (ORC(len)(dP) - L) * (R)s->ph.particle_k_sniff +
+           ORC(dot)(ORC(normalize)(dP, fl), dv) * (R)s->ph.particle_d_fact;
+}
+
 static void ORC(cell_springs)(ORC(sim)* s)
 {
     const R k = (R)s->ph.particle_k_sniff, d = (R)s->ph.particle_d_fact, dt = (R)s->ph.dt;
@@ -580,9 +587,27 @@ static void ORC(cell_springs)(ORC(sim)* s)
                     V3 dv = ORC(sub)(velocity, ORC(mk)(s->vx[nb], s->vy[nb], s->vz[nb]));
                     V3 f2 = ORC(mk)(sfx[nb], sfy[nb], sfz[nb]);
                     V3 shift = ORC(normalize)(ORC(smul)((R)-1, dP), &fl);
-                    V3 dvv = ORC(add)(dv, ORC(smul)(dt, ORC(sub)(initialForce, f2)));
-                    R mag = (ORC(len)(dP) - L) * k + ORC(dot)(ORC(normalize)(dP, &fl), dvv) * d;
-                    newForce = ORC(add)(newForce, ORC(smul)(mag, shift));
+                    if (s->ph.particle_rk4) {
+                        /* USE_RUNGE_KUTTA_FOR_PARTICLE branch of
+                         * calculateParticlesSpringForceComponent (physics.cuh:57-71):
+                         * returns 2 k2_v + 2 k3_v + k4_v (k1_v is not part of it) */
+                        V3 k1x = ORC(smul)(dt, dv);
+                        V3 k1v = ORC(smul)(dt, initialForce);
+                        V3 k2x = ORC(smul)(dt, ORC(add)(dv, ORC(sdiv)(k1x, (R)2)));
+                        V3 k2v = ORC(smul)(dt * ORC(spring_particle)(s, ORC(add)(dP, ORC(sdiv)(k1x, (R)2)),
+                                                                     ORC(add)(dv, ORC(sdiv)(k1v, (R)2)), L, &fl), shift);
+                        V3 k3x = ORC(smul)(dt, ORC(add)(dv, ORC(sdiv)(k2x, (R)2)));
+                        V3 k3v = ORC(smul)(dt * ORC(spring_particle)(s, ORC(add)(dP, ORC(sdiv)(k2x, (R)2)),
+                                                                     ORC(add)(dv, ORC(sdiv)(k2v, (R)2)), L, &fl), shift);
+                        V3 k4v = ORC(smul)(dt * ORC(spring_particle)(s, ORC(add)(dP, k3x), ORC(add)(dv, k3v), L, &fl),
+                                           shift);
+                        newForce = ORC(add)(newForce, ORC(add)(ORC(add)(ORC(smul)((R)2, k2v), ORC(smul)((R)2, k3v)), k4v));
+                        (void)f2;
+                    } else {
+                        V3 dvv = ORC(add)(dv, ORC(smul)(dt, ORC(sub)(initialForce, f2)));
+                        R mag = (ORC(len)(dP) - L) * k + ORC(dot)(ORC(normalize)(dP, &fl), dvv) * d;
+                        newForce = ORC(add)(newForce, ORC(smul)(mag, shift));
+                    }
                 }
             }
             R ratio = ORC(len)(radius) / initialRadius;
@@ -592,7 +617,8 @@ static void ORC(cell_springs)(ORC(sim)* s)
             else
                 env = ORC(sub)(G, ORC(smul)(vdamp, velocity));
             newForce = ORC(add)(newForce, env);
-            V3 out = ORC(sdiv)(ORC(add)(initialForce, newForce), (R)2);
+            /* (F + S + E) / 6 in the Runge-Kutta build (blood_cells.cu:114-119) */
+            V3 out = ORC(sdiv)(ORC(add)(initialForce, newForce), s->ph.particle_rk4 ? (R)6 : (R)2);
             s->fx[i] = out.x; s->fy[i] = out.y; s->fz[i] = out.z;
         }
     }
@@ -828,7 +854,17 @@ static void ORC(integrate_particles)(ORC(sim)* s)
         V3 v0 = ORC(sdiv)(ORC(mk)(s->vx[i], s->vy[i], s->vz[i]), (R)1);
         V3 v = ORC(add)(v0, ORC(smul)(dt, F));
         s->vx[i] = v.x; s->vy[i] = v.y; s->vz[i] = v.z;
-        V3 dp = ORC(smul)((R)0.5f * dt, ORC(add)(v, v0));
+        V3 dp;
+        if (s->ph.particle_rk4) {
+            /* USE_RUNGE_KUTTA_FOR_PARTICLE branch, blood_cells.cu:166-171 */
+            V3 k1 = ORC(smul)(dt, v);
+            V3 k2 = ORC(smul)(dt, ORC(add)(v, ORC(sdiv)(k1, (R)2)));
+            V3 k3 = ORC(smul)(dt, ORC(add)(v, ORC(sdiv)(k2, (R)2)));
+            V3 k4 = ORC(smul)(dt, ORC(add)(v, k3));
+            dp = ORC(sdiv)(ORC(add)(ORC(add)(ORC(add)(k1, ORC(smul)((R)2, k2)), ORC(smul)((R)2, k3)), k4), (R)6);
+        } else {
+            dp = ORC(smul)((R)0.5f * dt, ORC(add)(v, v0));
+        }
         s->px[i] = s->px[i] + dp.x; s->py[i] = s->py[i] + dp.y; s->pz[i] = s->pz[i] + dp.z;
     }
 }

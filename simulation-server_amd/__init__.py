@@ -78,6 +78,7 @@ class Physics(C.Structure):
         ("seed", C.c_uint64),
         ("dynamic_triangle_grid", C.c_int32),
         ("ray_work_items", C.c_int32),
+        ("particle_rk4", C.c_int32),
     ]
 
 

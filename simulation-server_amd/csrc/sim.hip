@@ -497,12 +497,19 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     pc.use_blood_flow = phys->use_blood_flow;
     pc.reaction = phys->enable_reaction_force;
     pc.brake_on = phys->enable_big_cells_brake;
+    pc.rk4 = phys->particle_rk4 ? 1 : 0;
     pc.seed = phys->seed;
     pc.nend = mesh->ending_count;
     {
         float mabs = 0.f;
         for (int d = 0; d < 3; ++d) mabs = std::max(mabs, std::max(std::fabs(bmin[d]), std::fabs(bmax[d])));
         pc.aabb_margin = 0.05f + 1e-5f * mabs;
+        // contact-search bound: +inf unless every radius is finite and >= 0
+        float rmax = 0.0f;
+        for (float r : s->host_bs)
+            if (!(r >= 0.0f && r <= 3.0e38f)) rmax = INFINITY;
+            else rmax = std::max(rmax, r);
+        pc.rmax = rmax;
     }
     for (int e = 0; e < mesh->ending_count; ++e) {
         for (int d = 0; d < 3; ++d) pc.endc[3 * e + d] = mesh->ending_centers[3 * e + d];
@@ -1863,6 +1870,7 @@ extern "C" int bcs_sim_attach_surrogate(bcs_sim* s, bcs_surrogate* net, float to
     int dev = -1;
     RC(bcs_surrogate_net(net, &n, &dev));
     if (dev != s->device) return fail(BCS_ERR_INVALID_ARGUMENT, "surrogate: network on another device");
+    if (s->pc.rk4) return fail(BCS_ERR_UNSUPPORTED, "surrogate: it learns the Heun springs (particle_rk4 is set)");
     RC(surr_alloc(s));
     s->surr = net;
     s->surr_net = n;
@@ -1876,6 +1884,7 @@ extern "C" int bcs_sim_surrogate_sample(bcs_sim* s, float* features, float* targ
     RC(set_dev(s));
     if (!n_out || capacity < 0 || (capacity > 0 && (!features || !targets)))
         return fail(BCS_ERR_INVALID_ARGUMENT, "bcs_sim_surrogate_sample: bad arguments");
+    if (s->pc.rk4) return fail(BCS_ERR_UNSUPPORTED, "surrogate: it learns the Heun springs (particle_rk4 is set)");
     RC(surr_alloc(s));
     HIPCHK(hipStreamSynchronize(s->stream));
     // frame 1 keeps the contact flags; frame 2 samples the network input of

@@ -95,6 +95,29 @@ __device__ __forceinline__ int32_t chunk_particle(const StepArgs& A, int T, int 
     return A.tt.pstart[T] + (cell - A.tt.cstart[T]) * p + a;
 }
 
+// springMassForceWithDampingForParticle, src/simulation/physics.cuh:24-27 (the
+// Runge-Kutta build evaluates it at three perturbed states per spring).
+__device__ __forceinline__ float spring_rk(const PhysConst& pc, f3 dP, f3 dv, float L, uint32_t& fl)
+{
+    return (len(dP) - L) * pc.k + dot(normalize(dP, fl), dv) * pc.d;
+}
+
+// Position increment of a particle, v0 -> v (propagateParticleForcesKernel,
+// src/objects/blood_cells.cu:155-179): Heun 0.5 dt (v + v0), or in the
+// Runge-Kutta build (k1 + 2 k2 + 2 k3 + k4) / 6 with k1 = dt v,
+// k2 = dt (v + k1 / 2), k3 = dt (v + k2 / 2), k4 = dt (v + k3).
+__device__ __forceinline__ f3 position_step(const PhysConst& pc, f3 v0, f3 v)
+{
+    if (pc.rk4) {
+        const f3 k1 = smul(pc.dt, v);
+        const f3 k2 = smul(pc.dt, add(v, sdiv(k1, 2.0f)));
+        const f3 k3 = smul(pc.dt, add(v, sdiv(k2, 2.0f)));
+        const f3 k4 = smul(pc.dt, add(v, k3));
+        return sdiv(add(add(add(k1, smul(2.0f, k2)), smul(2.0f, k3)), k4), 6.0f);
+    }
+    return smul(0.5f * pc.dt, add(v, v0));
+}
+
 __device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVel, float d2, float radius, float coef,
                                         uint32_t& fl)
 {
@@ -320,13 +343,23 @@ __device__ __forceinline__ uint32_t neighbour_mask(const BrickMap& bm, const Gri
                         (int64_t)y + nr.yh < bm.cnt[1] && (int64_t)z + nr.zl >= 0 && (int64_t)z + nr.zh < bm.cnt[2];
     if (inside) {
         const uint32_t xa = x + nr.xl, xb = x + nr.xh;
+        // the words of the low and high x ends of the 9 (dy, dz) rows, all
+        // loaded before any is used (one round trip); a row clipped away
+        // loads the particle's own row instead and is skipped below
+        uint32_t wa_[9], wb_[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int dy = c / 3 - 1, dz = c % 3 - 1;
+            const bool row = dy >= nr.yl && dy <= nr.yh && dz >= nr.zl && dz <= nr.zh;
+            const uint32_t yy = row ? y + dy : y, zz = row ? z + dz : z;
+            wa_[c] = bm.words[brick_word(bm, xa, yy, zz)];
+            wb_[c] = bm.words[brick_word(bm, xb, yy, zz)];
+        }
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
             const int dy = c / 3 - 1, dz = c % 3 - 1;
             if (dy < nr.yl || dy > nr.yh || dz < nr.zl || dz > nr.zh) continue;
-            const uint32_t yy = y + dy, zz = z + dz;
-            const uint32_t wa = bm.words[brick_word(bm, xa, yy, zz)];
-            const uint32_t wb = ((xa ^ xb) >> 3) ? (uint32_t)bm.words[brick_word(bm, xb, yy, zz)] : wa;
+            const uint32_t wa = wa_[c], wb = wb_[c];  // the same word when xa and xb share a brick
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx) {
                 if (dx < nr.xl || dx > nr.xh) continue;
@@ -618,7 +651,8 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
     const GridGeom& g = A.pg;
     const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
     // table entries {first slot, end slot} of the next kCellBatch occupied
-    // cells in visit order (empty: {0, 0}), taken off mask
+    // cells in visit order (empty: {0, 0}), taken off mask: the raw
+    // {start, count} entries are all loaded first (one round trip)
     auto load_cells = [&](uint2 (&ent)[kCellBatch]) {
 #pragma unroll
         for (int k = 0; k < kCellBatch; ++k) {
@@ -627,13 +661,23 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                 const int r = __builtin_ctz(mask);
                 mask &= mask - 1u;
                 const int dx = r / 9 - 1, c = r % 9;
-                const uint2 e = A.p_table[(int64_t)key_of + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
-                ent[k] = make_uint2(e.x, min(e.x + e.y, (uint32_t)np));
+                ent[k] = A.p_table[(int64_t)key_of + (c % 3 - 1) * cXY + (c / 3 - 1) * cX + dx];
             }
         }
+#pragma unroll
+        for (int k = 0; k < kCellBatch; ++k) ent[k].y = min(ent[k].x + ent[k].y, (uint32_t)np);
     };
     const ModelIx mi = model_ix(tt, Tl);
     const float r1 = radius(model_index(mi, i, fl));
+    // A partner farther than r1 + rmax (rmax: the largest radius of the
+    // table) cannot be in contact: d2 > rb^2 implies d2 > (r1 + r2)^2, every
+    // rounding being monotonic on these non-negative values, so its radius is
+    // not looked up.  Partners below the launch type's first particle can
+    // give a model index outside the table, which is flagged: theirs is
+    // always looked up when the type's model can be left that way.
+    const float rb = r1 + pc.rmax;
+    const float bound2 = rb * rb;
+    const int32_t look_below = mi.mstart < min(mi.ppc - 1, mi.pstart) ? mi.pstart : 0;
     uint32_t ncontact = 0;
     bool more = mask != 0u;
     // The occupied cells in visit order, kCellBatch at a time: their
@@ -653,39 +697,57 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
 #pragma unroll
         for (int k = 0; k < kCellBatch; ++k) cum[k + 1] = cum[k] + (ent[k].y > ent[k].x ? ent[k].y - ent[k].x : 0u);
         const uint32_t total = cum[kCellBatch];
-        for (uint32_t p0 = 0; p0 < total; p0 += kContactBatch) {
-            uint32_t slot[kContactBatch];
-            const int nval = (int)min((uint32_t)kContactBatch, total - p0);
+        // Windows of kContactWindow stream positions.  Phase 1 tests them,
+        // kContactBatch records at a time, and keeps one bit per partner in
+        // contact.  Phase 2 then takes each lane's contacts in stream order,
+        // adding them to F in visit order: the wave runs the force code once
+        // per contact of its busiest lane, instead of once per stream
+        // position at which any lane has a contact.
+        auto slot_at = [&](uint32_t pp) {
+            uint32_t sl = 0u;
 #pragma unroll
-            for (int t = 0; t < kContactBatch; ++t) {
-                const uint32_t pp = p0 + (uint32_t)t;
-                uint32_t sl = 0u;
+            for (int k = 0; k < kCellBatch; ++k)
+                if (pp >= cum[k] && pp < cum[k + 1]) sl = ent[k].x + (pp - cum[k]);
+            return sl;
+        };
+        for (uint32_t p0 = 0; p0 < total; p0 += kContactWindow) {
+            uint32_t hits = 0u;
 #pragma unroll
-                for (int k = 0; k < kCellBatch; ++k)
-                    if (pp >= cum[k] && pp < cum[k + 1]) sl = ent[k].x + (pp - cum[k]);
-                slot[t] = sl;
+            for (int g = 0; g < kContactWindow / kContactBatch; ++g) {
+                const uint32_t b0 = p0 + (uint32_t)(g * kContactBatch);
+                if (b0 >= total) break;
+                const int nval = (int)min((uint32_t)kContactBatch, total - b0);
+                uint32_t slot[kContactBatch];
+                float4 q[kContactBatch];
+#pragma unroll
+                for (int t = 0; t < kContactBatch; ++t) slot[t] = slot_at(b0 + (uint32_t)t);
+#pragma unroll
+                for (int t = 0; t < kContactBatch; ++t)
+                    if (t < nval) q[t] = A.pvs[2 * (size_t)slot[t]];
+#pragma unroll
+                for (int t = 0; t < kContactBatch; ++t) {
+                    if (t >= nval) continue;
+                    const int32_t j = __float_as_int(q[t].w);
+                    if (j == i) continue;
+                    const f3 rel = sub(x1, mk(q[t].x, q[t].y, q[t].z));
+                    const float d2 = dot(rel, rel);
+                    if (d2 <= bound2 || j < look_below) {
+                        const float r2 = radius(model_index(mi, j, fl));
+                        const float minD = r1 + r2;
+                        if (d2 <= minD * minD && d2 >= 0.0001f) hits |= 1u << (g * kContactBatch + t);
+                    }
+                }
             }
-            float4 q[kContactBatch], w[kContactBatch];
-#pragma unroll
-            for (int t = 0; t < kContactBatch; ++t)
-                if (t < nval) {
-                    q[t] = A.pvs[2 * (size_t)slot[t]];
-                    w[t] = A.pvs[2 * (size_t)slot[t] + 1];
-                }
-#pragma unroll
-            for (int t = 0; t < kContactBatch; ++t) {
-                if (t >= nval) break;
-                const int32_t j = __float_as_int(q[t].w);
-                if (j == i) continue;
-                const float r2 = radius(model_index(mi, j, fl));
-                const f3 rel = sub(x1, mk(q[t].x, q[t].y, q[t].z));
+            while (hits) {
+                const int t = __builtin_ctz(hits);
+                hits &= hits - 1u;
+                const uint32_t sl = slot_at(p0 + (uint32_t)t);
+                const float4 q = A.pvs[2 * (size_t)sl], w = A.pvs[2 * (size_t)sl + 1];
+                const f3 rel = sub(x1, mk(q.x, q.y, q.z));
                 const float d2 = dot(rel, rel);
-                const float minD = r1 + r2;
-                if (d2 <= minD * minD && d2 >= 0.0001f) {
-                    const f3 rv = sub(v1, mk(w[t].x, w[t].y, w[t].z));
-                    F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
-                    ++ncontact;
-                }
+                const f3 rv = sub(v1, mk(w.x, w.y, w.z));
+                F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
+                ++ncontact;
             }
         }
         more = mask != 0u;
@@ -820,9 +882,21 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
                 nrm = mk(0.f, 0.f, 0.f);
                 shift = nrm;
             }
-            f3 dvv = add(dv, smul(pc.dt, sub(F0, f2)));
-            float mag = (sl - L) * pc.k + dot(nrm, dvv) * pc.d;
-            newF = add(newF, smul(mag, shift));
+            if (pc.rk4) {
+                const f3 k1x = smul(pc.dt, dv), k1v = smul(pc.dt, F0);
+                const f3 k2x = smul(pc.dt, add(dv, sdiv(k1x, 2.0f)));
+                const f3 k2v =
+                    smul(pc.dt * spring_rk(pc, add(dP, sdiv(k1x, 2.0f)), add(dv, sdiv(k1v, 2.0f)), L, fl), shift);
+                const f3 k3x = smul(pc.dt, add(dv, sdiv(k2x, 2.0f)));
+                const f3 k3v =
+                    smul(pc.dt * spring_rk(pc, add(dP, sdiv(k2x, 2.0f)), add(dv, sdiv(k2v, 2.0f)), L, fl), shift);
+                const f3 k4v = smul(pc.dt * spring_rk(pc, add(dP, k3x), add(dv, k3v), L, fl), shift);
+                newF = add(newF, add(add(smul(2.0f, k2v), smul(2.0f, k3v)), k4v));
+            } else {
+                f3 dvv = add(dv, smul(pc.dt, sub(F0, f2)));
+                float mag = (sl - L) * pc.k + dot(nrm, dvv) * pc.d;
+                newF = add(newF, smul(mag, shift));
+            }
         }
         };
         if (sp_lds)
@@ -877,7 +951,7 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
         if (pc.brake_on && ratio > pc.brake) env = sub(G, smul(pc.vdamp * ratio * pc.bint, v1));
         else env = sub(G, smul(pc.vdamp, v1));
         newF = add(newF, env);
-        F = sdiv(add(F0, newF), 2.0f);
+        F = sdiv(add(F0, newF), pc.rk4 ? 6.0f : 2.0f);
 
         if constexpr (MODE == kCellsWhole) {
             // ---- particle-particle contacts over the 27 neighbour cells
@@ -1098,7 +1172,7 @@ __device__ __forceinline__ void respond_integrate(const StepArgs& A, bool act, i
         if (act) {
             const f3 v0 = sdiv(v1, 1.0f);
             v1 = add(v0, smul(pc.dt, F));
-            x1 = add(x1, smul(0.5f * pc.dt, add(v1, v0)));
+            x1 = add(x1, position_step(pc, v0, v1));
         }
         __syncthreads();  // s_tele initialised
         if (act && pc.use_blood_flow && vein_end_flag(pc, tt.end_spheres[T] != 0, x1)) s_tele[cl] = 1;
@@ -1131,7 +1205,7 @@ __global__ void __launch_bounds__(256) k_integrate(StepArgs A)
     const f3 v0 = sdiv(load3(A.vel, np, i), 1.0f);
     const f3 v = add(v0, smul(A.pc.dt, F));
     store3(A.vel, np, i, v);
-    store3(A.pos, np, i, add(load3(A.pos, np, i), smul(0.5f * A.pc.dt, add(v, v0))));
+    store3(A.pos, np, i, add(load3(A.pos, np, i), position_step(A.pc, v0, v)));
 }
 
 // HandleVeinEnd, src/simulation/vein_end.cu:141-173 (API path; one block per
@@ -1474,7 +1548,7 @@ __global__ void __launch_bounds__(256) k_tri_nmask(GridGeom g, const uint2* __re
 // Contact listing in sorted-slot order (calculateParticleCollisions visit
 // order; the force is not applied).  pass 0 counts, pass 1 writes pairs at
 // offs[i].
-__global__ void __launch_bounds__(256) k_contacts(StepArgs A, int pass, int32_t* counts, const int64_t* offs,
+__global__ void __launch_bounds__(256) k_contact_pairs(StepArgs A, int pass, int32_t* counts, const int64_t* offs,
                                                   int32_t* pairs)
 {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1703,7 +1777,7 @@ void launch_tri_centers(const float* X, int32_t nv, const uint32_t* idx, int32_t
 void launch_contacts(const StepArgs& A, int pass, int32_t* counts, const int64_t* offs, int32_t* pairs, hipStream_t st)
 {
     if (A.tt.np > 0)
-        hipLaunchKernelGGL(k_contacts, dim3((A.tt.np + 255) / 256), dim3(256), 0, st, A, pass, counts, offs, pairs);
+        hipLaunchKernelGGL(k_contact_pairs, dim3((A.tt.np + 255) / 256), dim3(256), 0, st, A, pass, counts, offs, pairs);
 }
 static TriBoxArgs tri_box_args(int32_t ntri, const uint4* trun, const uint4* trec, const float* X, int32_t nv,
                                float4* tdata, float4* aabb)

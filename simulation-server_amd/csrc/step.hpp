@@ -9,6 +9,7 @@ constexpr int kMaxEndings = 16;
 constexpr int kBlock = 256;  // threads per cell-chunk block
 constexpr int kBsLds = 1024;       // model-vertex radii held in LDS by the contact search (more: read globally)
 constexpr int kContactBatch = 4;   // sorted-slot records loaded together by the contact search
+constexpr int kContactWindow = 16; // stream positions tested before their contacts are applied (<= 32)
 constexpr int kCellBatch = 6;      // neighbour-cell table entries loaded together by the contact search
 constexpr uint32_t kXcdRun = 16;   // consecutive chunks per XCD run in the block -> chunk mapping
 constexpr int kSpringLds = 1024;   // springs of one cell type held in LDS by k_cells (more: read globally)
@@ -34,7 +35,9 @@ struct PhysConst {
     float thr_upper, thr_lower, thr_right, thr_left, thr_front, thr_back;
     int32_t use_blood_flow, reaction, brake_on, nend;
     float aabb_margin;      // absolute inflation of the triangle-cell boxes (rounding slack)
-    int32_t pad0;
+    float rmax;             // largest contact radius of the model table (+inf: unusable as a bound)
+    int32_t rk4;            // particle springs and integration of the reference's Runge-Kutta build
+    int32_t pad1;
     uint64_t seed;
     float endc[3 * kMaxEndings];
     float endr[kMaxEndings];

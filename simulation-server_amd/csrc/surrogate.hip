@@ -252,15 +252,21 @@ __global__ void __launch_bounds__(256) k_gather_rows(const uint16_t* __restrict_
 // ---- fused inference: all four layers in one kernel ------------------------
 // A block takes kFM rows (cells).  Their activations stay in LDS (bf16, row
 // pitch kHPitch) from the input to the last hidden layer; each weight matrix
-// streams through LDS 32 columns at a time, double-buffered, fetched into
-// registers one chunk ahead.  The 4 waves split the layer output 2 x 2:
-// 64 rows x 128 columns per wave for the hidden layers (4 x 8 MFMA tiles),
-// 64 x 64 for the 128-column output layer.  Only the input rows and the
-// output rows touch HBM.
+// streams through LDS 32 columns at a time (double-buffered), fetched into
+// registers two chunks ahead.  The products are formed transposed, W x H^T,
+// so that each lane ends with 4 consecutive output features of one cell: the
+// bias, GELU and bf16 rounding of a hidden layer go back to LDS as one 8-byte
+// store per 4 values.  The 4 waves split a layer's output 2 x 2: 64 cells x
+// 128 features per wave for the hidden layers (4 x 8 MFMA tiles), 64 x 64
+// for the 128-feature output layer.  Only the input rows and the output
+// rows touch HBM.
 constexpr int kFM = 128;
 constexpr int kHPitch = kSurrInPad + 8;  // 328 bf16: rows 16 B apart modulo the LDS banks
 constexpr int kWPitch = kBK + 8;         // 40 bf16
 constexpr int kWBuf = kSurrHidden * kWPitch;
+constexpr int kAhead = 2;                // weight chunks in flight in registers
+
+__device__ __forceinline__ uint16_t bf16_rn(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 // Columns [k0, k0 + 32) of the first `rows` rows of W (ld K): 4 16-byte pieces
 // per row, piece c = tid + 256 q of the block.
@@ -283,25 +289,30 @@ __device__ __forceinline__ void w_chunk_store(uint16_t* buf, int rows, int tid, 
     }
 }
 
-// acc = H[wm .. wm + 64) x W^T[.., wn .. wn + 16 NT) over K columns; W has
+// acc[i][j] = W[wn + 16 j ..][0 .. K) x H[wm + 16 i ..][0 .. K)^T: lane l holds
+// features wn + 16 j + 4 (l / 16) + r of cell wm + 16 i + l % 16.  W has
 // 32 NT rows (256 or 128).  Ends with a barrier: every wave is done with H.
-template <int NT>
-__device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, int K, const uint16_t* H, uint16_t* wbuf,
+template <int NT, int K>
+__device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, const uint16_t* H, uint16_t* wbuf,
                                           f32x4 (&acc)[4][NT], int wm, int wn, int lane, int tid)
 {
-    constexpr int rows = 32 * NT;
+    constexpr int rows = 32 * NT, nks = K / kBK;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    uint4 r[4];
-    w_chunk_load(W, K, rows, 0, tid, r);
-    w_chunk_store(wbuf, rows, tid, r);
+    uint4 r[kAhead][4];
+#pragma unroll
+    for (int a = 0; a < kAhead; ++a)
+        if (a < nks) w_chunk_load(W, K, rows, a * kBK, tid, r[a]);
+    w_chunk_store(wbuf, rows, tid, r[0]);
     __syncthreads();
-    const int nks = K / kBK;
+#pragma unroll
     for (int ks = 0; ks < nks; ++ks) {
+        // chunk ks + kAhead into the registers of chunk ks (in LDS since the
+        // last step); it is stored kAhead - 1 steps from now
+        if (ks + kAhead < nks) w_chunk_load(W, K, rows, (ks + kAhead) * kBK, tid, r[ks % kAhead]);
         const uint16_t* cur = wbuf + (ks & 1) * kWBuf;
-        if (ks + 1 < nks) w_chunk_load(W, K, rows, (ks + 1) * kBK, tid, r);
         bf16x8 a[4], b[NT];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -313,8 +324,8 @@ __device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, int K,
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        if (ks + 1 < nks) w_chunk_store(wbuf + ((ks + 1) & 1) * kWBuf, rows, tid, r);
+            for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+        if (ks + 1 < nks) w_chunk_store(wbuf + ((ks + 1) & 1) * kWBuf, rows, tid, r[(ks + 1) % kAhead]);
         __syncthreads();
     }
 }
@@ -335,39 +346,45 @@ __global__ void __launch_bounds__(256) k_mlp_infer(SurrNet net, const uint16_t* 
     }
     __syncthreads();
     const int wm = (wave >> 1) * 64;
+    const int cm = lane & 15, fq = 4 * (lane >> 4);  // the lane's cell within a tile, first feature of its 4
     {
         f32x4 acc[4][8];
         const int wn = (wave & 1) * 128;
+#pragma unroll 1
         for (int l = 0; l < 3; ++l) {
-            mlp_layer<8>(net.W[l], l == 0 ? kSurrInPad : kSurrHidden, H, wbuf, acc, wm, wn, lane, tid);
+            if (l == 0) mlp_layer<8, kSurrInPad>(net.W[0], H, wbuf, acc, wm, wn, lane, tid);
+            else mlp_layer<8, kSurrHidden>(net.W[l], H, wbuf, acc, wm, wn, lane, tid);
             // bias, GELU, bf16: the next layer's input, in place
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const int nn = wn + 16 * j + (lane & 15);
-                const float bn = net.b[l][nn];
+                const int nn = wn + 16 * j + fq;
+                const float4 bn = *reinterpret_cast<const float4*>(net.b[l] + nn);
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        H[(wm + 16 * i + 4 * (lane >> 4) + r) * kHPitch + nn] = to_bf16(gelu(acc[i][j][r] + bn));
+                for (int i = 0; i < 4; ++i) {
+                    const f32x4 v = acc[i][j];
+                    const uint32_t lo = (uint32_t)bf16_rn(gelu(v[0] + bn.x)) | ((uint32_t)bf16_rn(gelu(v[1] + bn.y)) << 16);
+                    const uint32_t hi = (uint32_t)bf16_rn(gelu(v[2] + bn.z)) | ((uint32_t)bf16_rn(gelu(v[3] + bn.w)) << 16);
+                    *reinterpret_cast<uint2*>(H + (wm + 16 * i + cm) * kHPitch + nn) = make_uint2(lo, hi);
+                }
             }
             __syncthreads();
         }
     }
     f32x4 acc[4][4];
     const int wn = (wave & 1) * 64;
-    mlp_layer<4>(net.W[3], kSurrHidden, H, wbuf, acc, wm, wn, lane, tid);
+    mlp_layer<4, kSurrHidden>(net.W[3], H, wbuf, acc, wm, wn, lane, tid);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int nn = wn + 16 * j + (lane & 15);
-        const float bn = net.b[3][nn];
+        const int nn = wn + 16 * j + fq;
+        const float4 bn = *reinterpret_cast<const float4*>(net.b[3] + nn);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
-                if (m < n) out[m * kSurrOutPad + nn] = acc[i][j][r] + bn;
-            }
+        for (int i = 0; i < 4; ++i) {
+            const int64_t m = m0 + wm + 16 * i + cm;
+            const f32x4 v = acc[i][j];
+            if (m < n)
+                *reinterpret_cast<float4*>(out + m * kSurrOutPad + nn) =
+                    make_float4(v[0] + bn.x, v[1] + bn.y, v[2] + bn.z, v[3] + bn.w);
+        }
     }
 }
 

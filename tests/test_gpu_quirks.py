@@ -161,6 +161,26 @@ def test_switch_off_40_frames(bcs, orc, switch):
     _compare(sim, o, f"{switch} = 0, 40 frames")
 
 
+@pytest.mark.parametrize("mode", ["slots", "cells"])
+def test_rk4_build_40_frames(bcs, orc, mode):
+    """The reference's USE_RUNGE_KUTTA_FOR_PARTICLE build (particle_rk4 = 1):
+    springs from three perturbed states and the Runge-Kutta position update,
+    40 frames of the mixed scene against the oracle, in both contact modes.
+    The surrogate refuses this build."""
+    sc = mixed_derived_scene(bcs, seed=3)
+    ph = sc.physics
+    ph.particle_rk4 = 1
+    sim = bcs.Simulation.from_scene(sc, physics=ph)
+    sim.set_contact_mode(bcs.CONTACT_SLOTS if mode == "slots" else bcs.CONTACT_CELLS)
+    o = orc.Oracle(ph, sc.types, sc.mesh)
+    o.set_particles(sc.positions, sc.velocities, sc.forces)
+    sim.step(40)
+    o.step(40)
+    _compare(sim, o, f"Runge-Kutta build ({mode}), 40 frames")
+    with pytest.raises(bcs.BcsError):
+        sim.attach_surrogate(bcs.Surrogate(seed=0), 1e-3)
+
+
 # ------------------------------------------- the reference's default types --
 
 def reference_default_scene(bcs, seed=0):

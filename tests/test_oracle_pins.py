@@ -266,6 +266,61 @@ def test_heun_integrator_constant_force(bcs, orc):
     np.testing.assert_array_equal(f, F)  # forces persist (not reset)
 
 
+def test_rk4_spring_closed_form(bcs, orc):
+    """The Runge-Kutta build's spring (physics.cuh:57-71) on the pair of
+    test_spring_and_environment_closed_form, worked along x by hand.
+    Particle 0: dP = -3, dv = -1, F0 = 0, shift = +x; k1x = dt dv = -0.01,
+    k1v = dt F0 = 0; spring(dP', dv') = (|dP'| - 2) 50 + sign(dP') dv' 0.5:
+    k2v = dt spring(-3.005, -1) = 0.5075; k2x = dt (dv + k1x / 2) = -0.01005;
+    k3v = dt spring(-3.005025, -1 + k2v / 2) = 0.50624375;
+    k3x = dt (dv + k2x / 2) = -0.01005025;
+    k4v = dt spring(-3 + k3x, -1 + k3v) = 0.50749390625;
+    S = 2 k2v + 2 k3v + k4v (k1v left out), F = (F0 + S + E) / 6.
+    Particle 1 mirrors it, with E = G - 0.39 v."""
+    mesh = _tri_mesh(bcs, [[500, 500, 500], [510, 500, 500], [500, 510, 510]])
+    ph = _phys(bcs, dt=0.01, particle_k_sniff=50.0, particle_d_fact=0.5, particle_cell=(5.0, 5.0, 5.0),
+               triangle_cell=(25.0, 25.0, 25.0), grid_xz_margin=600.0, grid_y_margin=600.0)
+    ph.particle_rk4 = 1
+    t = _two_particle_spring(bcs, 3.0, 1.0)
+    o = orc.Oracle(ph, [t], mesh)
+    o.set_particles(np.array([0, 3, 0, 0, 0, 0], F32), np.array([0, 1, 0, 0, 0, 0], F32), np.zeros(6, F32))
+    o.calculate_grid(0)
+    o.calculate_grid(1)
+    o.calculate_next_frame()
+    F = soa(o.get_particles()[2], 2)
+    S = 2 * 0.5075 + 2 * 0.50624375 + 0.50749390625
+    np.testing.assert_allclose(F[:, 0], [S / 6, -25.0 / 6, 0.0], rtol=2e-6)
+    np.testing.assert_allclose(F[:, 1], [(-S - 0.39) / 6, -25.0 / 6, 0.0], rtol=2e-6)
+    # the Heun build on the same pair divides by 2 and uses dv + dt (F0 - F0') instead
+    ph.particle_rk4 = 0
+    o2 = orc.Oracle(ph, [t], mesh)
+    o2.set_particles(np.array([0, 3, 0, 0, 0, 0], F32), np.array([0, 1, 0, 0, 0, 0], F32), np.zeros(6, F32))
+    o2.calculate_grid(0)
+    o2.calculate_grid(1)
+    o2.calculate_next_frame()
+    np.testing.assert_allclose(soa(o2.get_particles()[2], 2)[:, 0], [50.5 / 2, -12.5, 0.0], rtol=1e-6)
+
+
+def test_rk4_integrator_closed_form(bcs, orc):
+    """The Runge-Kutta build's position update (blood_cells.cu:166-171) with
+    the updated velocity v = v0 + dt F: k1 = dt v, k2 = dt (v + k1/2),
+    k3 = dt (v + k2/2), k4 = dt (v + k3), dx = (k1 + 2k2 + 2k3 + k4)/6
+    = dt v (6 + 3 dt + dt^2 + dt^3/4) / 6."""
+    mesh = _tri_mesh(bcs, [[500, 500, 500], [510, 500, 500], [500, 510, 510]])
+    ph = _phys(bcs, dt=0.01, grid_xz_margin=600.0, grid_y_margin=600.0)
+    ph.particle_rk4 = 1
+    o = orc.Oracle(ph, [_point_type(bcs, 1)], mesh)
+    x0, v0, F = np.array([1, 2, 3], F32), np.array([4, -5, 6], F32), np.array([10, 20, -30], F32)
+    o.set_particles(x0, v0, F)
+    o.propagate_all()
+    x, v, f = o.get_particles()
+    dt = 0.01
+    vn = v0.astype(np.float64) + dt * F
+    np.testing.assert_allclose(v, vn, rtol=1e-6)
+    np.testing.assert_allclose(x, x0 + dt * vn * (6 + 3 * dt + dt ** 2 + dt ** 3 / 4) / 6, rtol=1e-6)
+    np.testing.assert_array_equal(f, F)
+
+
 def test_vein_springs_and_euler(bcs, orc):
     # triangle (0,0,0), (4,0,0), (0,3,0); vertex 1 moved to (5,0,0)
     verts = np.array([[0, 0, 0], [4, 0, 0], [0, 3, 0]], F32)
