@@ -643,7 +643,7 @@ static_assert(sizeof(StepArgs) <= 4096, "kernel arguments exceed 4 KiB");
 // contacts.
 template <class Radius>
 __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, int Tl, uint32_t key_of, uint32_t mask,
-                                                f3 x1, f3 v1, f3& F, uint32_t& fl, Radius radius)
+                                                f3 x1, f3 v1, f3& F, uint32_t& fl, Radius radius, uint32_t* s_hit)
 {
     const TypeTable& tt = A.tt;
     const PhysConst& pc = A.pc;
@@ -698,11 +698,12 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
         for (int k = 0; k < kCellBatch; ++k) cum[k + 1] = cum[k] + (ent[k].y > ent[k].x ? ent[k].y - ent[k].x : 0u);
         const uint32_t total = cum[kCellBatch];
         // Windows of kContactWindow stream positions.  Phase 1 tests them,
-        // kContactBatch records at a time, and keeps one bit per partner in
-        // contact.  Phase 2 then takes each lane's contacts in stream order,
-        // adding them to F in visit order: the wave runs the force code once
-        // per contact of its busiest lane, instead of once per stream
-        // position at which any lane has a contact.
+        // kContactBatch records at a time, and lists the slots of the
+        // partners in contact (in LDS, s_hit[k * kBlock + lane]).  Phase 2
+        // then takes each lane's list in stream order, adding the contacts
+        // to F in visit order: the wave runs the force code once per contact
+        // of its busiest lane, instead of once per stream position at which
+        // any lane has a contact.
         auto slot_at = [&](uint32_t pp) {
             uint32_t sl = 0u;
 #pragma unroll
@@ -710,17 +711,35 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                 if (pp >= cum[k] && pp < cum[k + 1]) sl = ent[k].x + (pp - cum[k]);
             return sl;
         };
+        uint32_t* hit_list = s_hit + threadIdx.x;
         for (uint32_t p0 = 0; p0 < total; p0 += kContactWindow) {
-            uint32_t hits = 0u;
+            uint32_t nh = 0u;
 #pragma unroll
             for (int g = 0; g < kContactWindow / kContactBatch; ++g) {
                 const uint32_t b0 = p0 + (uint32_t)(g * kContactBatch);
                 if (b0 >= total) break;
                 const int nval = (int)min((uint32_t)kContactBatch, total - b0);
-                uint32_t slot[kContactBatch];
-                float4 q[kContactBatch];
+                // the run (cell) holding position b0: slot = position + delta
+                // up to its end; positions past it are mapped one by one
+                uint32_t delta = 0u, run_end = 0u;
 #pragma unroll
-                for (int t = 0; t < kContactBatch; ++t) slot[t] = slot_at(b0 + (uint32_t)t);
+                for (int k = 0; k < kCellBatch; ++k)
+                    if (b0 >= cum[k]) {
+                        delta = ent[k].x - cum[k];
+                        run_end = cum[k + 1];
+                    }
+                uint32_t slot[kContactBatch];
+                if (__any(b0 + (uint32_t)(nval - 1) >= run_end)) {
+#pragma unroll
+                    for (int t = 0; t < kContactBatch; ++t) {
+                        const uint32_t pp = b0 + (uint32_t)t;
+                        slot[t] = pp < run_end ? pp + delta : slot_at(pp);
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < kContactBatch; ++t) slot[t] = b0 + (uint32_t)t + delta;
+                }
+                float4 q[kContactBatch];
 #pragma unroll
                 for (int t = 0; t < kContactBatch; ++t)
                     if (t < nval) q[t] = A.pvs[2 * (size_t)slot[t]];
@@ -734,21 +753,22 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                     if (d2 <= bound2 || j < look_below) {
                         const float r2 = radius(model_index(mi, j, fl));
                         const float minD = r1 + r2;
-                        if (d2 <= minD * minD && d2 >= 0.0001f) hits |= 1u << (g * kContactBatch + t);
+                        if (d2 <= minD * minD && d2 >= 0.0001f) {
+                            hit_list[nh * kBlock] = slot[t];
+                            ++nh;
+                        }
                     }
                 }
             }
-            while (hits) {
-                const int t = __builtin_ctz(hits);
-                hits &= hits - 1u;
-                const uint32_t sl = slot_at(p0 + (uint32_t)t);
+            for (uint32_t h = 0; h < nh; ++h) {
+                const uint32_t sl = hit_list[h * kBlock];
                 const float4 q = A.pvs[2 * (size_t)sl], w = A.pvs[2 * (size_t)sl + 1];
                 const f3 rel = sub(x1, mk(q.x, q.y, q.z));
                 const float d2 = dot(rel, rel);
                 const f3 rv = sub(v1, mk(w.x, w.y, w.z));
                 F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
-                ++ncontact;
             }
+            ncontact += nh;
         }
         more = mask != 0u;
     }
@@ -770,6 +790,7 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
     __shared__ float sfx[kBlock], sfy[kBlock], sfz[kBlock];
     __shared__ int s_tele[kBlock];
     __shared__ float s_bs[kBsLds];
+    __shared__ uint32_t s_hit[MODE == kCellsWhole ? kContactWindow * kBlock : 1];  // contact lists
     __shared__ int32_t s_spoff[kBlock + 1];  // the block's cell type: spring list offsets per vertex,
     __shared__ int32_t s_spb[kSpringLds];    // partner vertex and rest length per spring
     __shared__ float s_spL[kSpringLds];
@@ -957,7 +978,7 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
             // ---- particle-particle contacts over the 27 neighbour cells
             int Tl = T;
             if (tt.nt > 1) Tl = type_of_index(tt, slot_i);
-            ncontact = contact_sum(A, i, Tl, key_of, mask, x1, v1, F, fl, radius);
+            ncontact = contact_sum(A, i, Tl, key_of, mask, x1, v1, F, fl, radius, s_hit);
         } else {
             A.fslot[slot_i] = make_float4(F.x, F.y, F.z, 0.0f);
         }
@@ -988,6 +1009,7 @@ template <bool SURR>
 __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
 {
     __shared__ float s_bs[kBsLds];
+    __shared__ uint32_t s_hit[kContactWindow * kBlock];  // per lane: the slots of its contacts in a window
     const TypeTable& tt = A.tt;
     const int l = threadIdx.x;
     const int tile = chunk_of_block(A.ntiles, vb.id);
@@ -1023,7 +1045,7 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
     __syncthreads();  // s_bs
     if (act) {
         const int Tl = tt.nt > 1 ? type_of_index(tt, (int32_t)u) : 0;
-        const uint32_t nc = contact_sum(A, i, Tl, key_of, mask, x1, v1, F, fl, radius);
+        const uint32_t nc = contact_sum(A, i, Tl, key_of, mask, x1, v1, F, fl, radius, s_hit);
         A.fslot[u] = make_float4(F.x, F.y, F.z, 0.0f);
         if (SURR && A.surr.touched) A.surr.touched[i] = nc > 0u ? 1 : 0;
     }
