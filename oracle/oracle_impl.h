@@ -654,6 +654,20 @@ static void ORC(particle_collisions_impl)(ORC(sim)* s, int32_t* counts, int32_t*
     const int64_t cX = g->cnt[0], cXY = (int64_t)g->cnt[0] * g->cnt[1];
     int32_t sl;
     uint32_t fl = 0;
+    /* Largest radius of the table (+inf unless every radius is finite and
+     * >= 0).  A partner farther than r1 + rmax cannot touch (d2 > (r1 + rmax)^2
+     * implies d2 > (r1 + r2)^2, every rounding being monotonic), so its
+     * radius is not looked up, and an index of it outside the table is not
+     * flagged: BCS_FLAG_RADIUS_CLAMPED reports partners within that reach. */
+    R rmax = (R)0;
+    {
+        int32_t m;
+        for (m = 0; m < s->nm; ++m) {
+            R r = s->bs[m];
+            if (!(r >= (R)0 && r <= (R)3.0e38f)) { rmax = (R)INFINITY; break; }
+            if (r > rmax) rmax = r;
+        }
+    }
 #pragma omp parallel for reduction(| : fl) schedule(dynamic, 256)
     for (sl = 0; sl < s->np; ++sl) {
         int t = ORC(type_of_slot)(s, sl);
@@ -670,6 +684,8 @@ static void ORC(particle_collisions_impl)(ORC(sim)* s, int32_t* counts, int32_t*
         ORC(axis_range_i)(zId, g->cnt[2], &zl, &zh);
         int32_t bmi = ORC(model_index)(s, t, i, &fl);
         R r1 = s->bs[bmi];
+        R rb = r1 + rmax;
+        R bound2 = rb * rb;
         V3 F = ORC(mk)(s->fx[i], s->fy[i], s->fz[i]);
         int32_t ncont = 0;
         int dx, dy, dz;
@@ -682,11 +698,12 @@ static void ORC(particle_collisions_impl)(ORC(sim)* s, int32_t* counts, int32_t*
                     for (u = a; u <= b; ++u) {
                         int32_t j = g->perm[u];
                         if (j == i) continue;
-                        int32_t smi = ORC(model_index)(s, t, j, &fl);
-                        R r2 = s->bs[smi];
                         V3 p2 = ORC(mk)(s->px[j], s->py[j], s->pz[j]);
                         V3 rel = ORC(sub)(p1, p2);
                         R d2 = ORC(dot)(rel, rel);
+                        if (!(d2 <= bound2)) continue;
+                        int32_t smi = ORC(model_index)(s, t, j, &fl);
+                        R r2 = s->bs[smi];
                         R minD = r1 + r2;
                         if (d2 <= minD * minD && d2 >= (R)0.0001f) {
                             if (lists) {

@@ -641,7 +641,13 @@ static_assert(sizeof(StepArgs) <= 4096, "kernel arguments exceed 4 KiB");
 // to F in visit order, slot by slot.  Tl is the launch type of i: the radius
 // of every partner is looked up through Tl's model.  Returns the number of
 // contacts.
-template <class Radius>
+// How the contact search drops neighbour cells out of reach: not at all
+// (stale grids), per lane, or per grid cell (the union over the lanes of a
+// wave that share the particle's cell, which keeps their candidate streams,
+// and so their loads and trip counts, identical; lanes in sorted-slot order).
+constexpr int kPruneNone = 0, kPruneLane = 1, kPruneCell = 2;
+
+template <int PRUNE, class Radius>
 __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, int Tl, uint32_t key_of, uint32_t mask,
                                                 f3 x1, f3 v1, f3& F, uint32_t& fl, Radius radius, uint32_t* s_hit)
 {
@@ -672,12 +678,62 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
     // A partner farther than r1 + rmax (rmax: the largest radius of the
     // table) cannot be in contact: d2 > rb^2 implies d2 > (r1 + r2)^2, every
     // rounding being monotonic on these non-negative values, so its radius is
-    // not looked up.  Partners below the launch type's first particle can
-    // give a model index outside the table, which is flagged: theirs is
-    // always looked up when the type's model can be left that way.
+    // not looked up (nor a model index of it outside the table flagged, as
+    // the oracle does).
     const float rb = r1 + pc.rmax;
     const float bound2 = rb * rb;
-    const int32_t look_below = mi.mstart < min(mi.ppc - 1, mi.pstart) ? mi.pstart : 0;
+    // With a grid built from the current positions (every fused frame),
+    // neighbour cells whose box lies farther than rb from x1 hold no such
+    // partner and can be dropped from the mask.  The gaps to the cell faces
+    // are shrunk by m (which covers the rounding of every position's cell
+    // index and of the gaps themselves) and the squared distance gets a
+    // relative slack, so a cell is only dropped when it certainly holds no
+    // partner.  On a stale grid (the phase API after propagateAll) partners
+    // may have left their cells, and every cell is searched.
+    if (PRUNE != kPruneNone) {
+        const int32_t id[3] = {sat_i32((x1.x - g.minv[0]) / g.cell[0]), sat_i32((x1.y - g.minv[1]) / g.cell[1]),
+                               sat_i32((x1.z - g.minv[2]) / g.cell[2])};
+        const bool own = id[0] >= 0 && id[0] < g.cnt[0] && id[1] >= 0 && id[1] < g.cnt[1] && id[2] >= 0 &&
+                         id[2] < g.cnt[2] && (int64_t)id[0] + id[1] * cX + id[2] * cXY == (int64_t)key_of;
+        uint32_t keep = 0x7FFFFFFu;  // all 27
+        if (own && bound2 < 3.0e38f) {
+            const float xs[3] = {x1.x, x1.y, x1.z};
+            float gl[3], gh[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float m = 1e-6f * (fabsf(g.minv[a]) + (float)g.cnt[a] * g.cell[a]) + 1e-6f;
+                const float f = xs[a] - (g.minv[a] + (float)id[a] * g.cell[a]);
+                gl[a] = fmaxf(f - m, 0.0f);
+                gh[a] = fmaxf(g.cell[a] - f - m, 0.0f);
+            }
+            const float lim = bound2 * 1.00001f;
+            keep = 0u;
+#pragma unroll
+            for (int r = 0; r < 27; ++r) {
+                const int dx = r / 9 - 1, dy = (r / 3) % 3 - 1, dz = r % 3 - 1;
+                const float gx = dx < 0 ? gl[0] : (dx > 0 ? gh[0] : 0.0f);
+                const float gy = dy < 0 ? gl[1] : (dy > 0 ? gh[1] : 0.0f);
+                const float gz = dz < 0 ? gl[2] : (dz > 0 ? gh[2] : 0.0f);
+                if (gx * gx + gy * gy + gz * gz <= lim) keep |= 1u << r;
+            }
+        }
+        if (PRUNE == kPruneCell) {
+            // union over the lanes of the same cell: they are adjacent
+            // (sorted slots, equal keys), so a segmented OR scan up and down
+            // the wave (lanes past the end hold key 0xFFFFFFFF)
+            const int lane = threadIdx.x & 63;
+            uint32_t up = keep, dn = keep;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t ku = __shfl_up(key_of, o, 64), vu = __shfl_up(up, o, 64);
+                const uint32_t kd = __shfl_down(key_of, o, 64), vd = __shfl_down(dn, o, 64);
+                if (lane >= o && ku == key_of) up |= vu;
+                if (lane + o < 64 && kd == key_of) dn |= vd;
+            }
+            keep = up | dn;
+        }
+        mask &= keep;
+    }
     uint32_t ncontact = 0;
     bool more = mask != 0u;
     // The occupied cells in visit order, kCellBatch at a time: their
@@ -750,7 +806,7 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                     if (j == i) continue;
                     const f3 rel = sub(x1, mk(q[t].x, q[t].y, q[t].z));
                     const float d2 = dot(rel, rel);
-                    if (d2 <= bound2 || j < look_below) {
+                    if (d2 <= bound2) {
                         const float r2 = radius(model_index(mi, j, fl));
                         const float minD = r1 + r2;
                         if (d2 <= minD * minD && d2 >= 0.0001f) {
@@ -978,7 +1034,8 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
             // ---- particle-particle contacts over the 27 neighbour cells
             int Tl = T;
             if (tt.nt > 1) Tl = type_of_index(tt, slot_i);
-            ncontact = contact_sum(A, i, Tl, key_of, mask, x1, v1, F, fl, radius, s_hit);
+            ncontact = contact_sum<FUSED ? kPruneLane : kPruneNone>(A, i, Tl, key_of, mask, x1, v1, F, fl, radius,
+                                                                   s_hit);
         } else {
             A.fslot[slot_i] = make_float4(F.x, F.y, F.z, 0.0f);
         }
@@ -1023,7 +1080,7 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
     const bool act = u < np;
     const GridGeom& g = A.pg;
     const int64_t cX = g.cnt[0], cXY = (int64_t)g.cnt[0] * g.cnt[1];
-    uint32_t fl = 0, mask = 0u, key_of = 0u;
+    uint32_t fl = 0, mask = 0u, key_of = 0xFFFFFFFFu;  // lanes past the end: no key, no cells
     int32_t i = 0;
     f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
     if (act) {
@@ -1043,9 +1100,11 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
         mask = neighbour_mask(A.p_map, g, key_of, nr, cX, cXY, true);
     }
     __syncthreads();  // s_bs
+    // every lane of the wave takes part (the pruning exchanges masks between
+    // lanes); a lane past the end has an empty mask
+    const int Tl = act && tt.nt > 1 ? type_of_index(tt, (int32_t)u) : 0;
+    const uint32_t nc = contact_sum<kPruneCell>(A, i, Tl, key_of, mask, x1, v1, F, fl, radius, s_hit);
     if (act) {
-        const int Tl = tt.nt > 1 ? type_of_index(tt, (int32_t)u) : 0;
-        const uint32_t nc = contact_sum(A, i, Tl, key_of, mask, x1, v1, F, fl, radius, s_hit);
         A.fslot[u] = make_float4(F.x, F.y, F.z, 0.0f);
         if (SURR && A.surr.touched) A.surr.touched[i] = nc > 0u ? 1 : 0;
     }
