@@ -86,6 +86,16 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0, map_in_rank=False):
                    + map code read 10 per occupied cell
                                                                    112 / particle + 12 / blood cell
                                                                    + 10 / cell
+      (the default frame runs k_cells as the three kernels below)
+      k_springs    x, v, F read 36, sorted slot read 4, F after the springs
+                   written to its slot 16; centre written 12 per blood cell
+                                                                   56 / particle + 12 / blood cell
+      k_contacts   own sorted-slot record read 32 (each record, as the
+                   candidate of others, counted once), sorted key 4, F read
+                   and written 16 + 16; table entry + map code read 10 per
+                   occupied cell                                   68 / particle + 10 / cell
+      k_respond    x, v read 24, sorted slot 4, F from its slot 16, first-hit
+                   key 4, x, v, F written 36                       84 / particle
       k_vein_prep  vein springs + triangle data: X, V read 24, 9 neighbour ids + lengths 72,
                    Fv read + written 24 per vertex; perm + indices 16, the
                    float4 triangle data written 48 and one box (32) per
@@ -101,6 +111,9 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0, map_in_rank=False):
         "k_ray": 40 * n_items,
         "k_raycand_ray": 28 * np_,
         "k_cells": 112 * np_ + 12 * nc + 10 * n_occ,
+        "k_springs": 56 * np_ + 12 * nc,
+        "k_contacts": 68 * np_ + 10 * n_occ,
+        "k_respond": 84 * np_,
         "k_vein_prep": 120 * nv + 76 * nt,
         "k_vein": 76 * nv,
     }
@@ -337,6 +350,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true",
                     help="enqueue the fused frame kernels directly instead of replaying the HIP graph")
+    ap.add_argument("--direct-timed", action="store_true",
+                    help="warm-up frames from the HIP graphs, timed frames launched directly (a counter "
+                         "profiler that does not follow graph replays then sees the timed frames only)")
     ap.add_argument("--surrogate", action="store_true",
                     help="config 4 with the learned spring surrogate (trained on config-2 rows first)")
     ap.add_argument("--surr-epochs", type=int, default=8)
@@ -370,6 +386,8 @@ def main():
 
     # warm-up (also builds and instantiates the frame graph)
     sim.step(max(args.warmup, 1))
+    if args.direct_timed:
+        sim.use_graph(False)
     # timed region: K graph-replayed frames between two synchronisations
     sim.synchronize()
     t0 = time.perf_counter()
@@ -433,7 +451,8 @@ def main():
             "particle_grid": list(info.particle_cells),
             "triangle_grid": list(info.triangle_cells),
             "dt": scene.physics.dt,
-            "frame_launch": "direct" if args.no_graph else "hip-graph",
+            "frame_launch": "direct" if args.no_graph else ("hip-graph warm-up, direct timed frames"
+                                                             if args.direct_timed else "hip-graph"),
         },
         "roofline": roof,
         "step_roofline": step_roof,

@@ -168,8 +168,12 @@ def pmc(bench_json: str, fetch_dir: str, write_dir: str, out_md: str, out_json: 
         f"dispatch for {kib:.0f} KiB moved each way, so FETCH_SIZE is scaled by {f_scale:.3f} and WRITE_SIZE by "
         f"{w_scale:.3f}.",
         "",
-        f"Per-dispatch means ({warm} warm-up frames left out). Algorithmic bytes per launch: bench.py "
-        f"kernel_bytes() at this workload's sizes.",
+        (f"Per-dispatch means over the directly launched frames {bench['config'].get('frames_timed', '')} "
+         f"and the {bench.get('profile_steps', 0)} event-timed ones (the {warm} warm-up frames ran from HIP graphs, "
+         f"which the counter runs do not see)"
+         if "direct timed" in bench["config"].get("frame_launch", "") else
+         f"Per-dispatch means ({warm} warm-up frames left out).") +
+        " Algorithmic bytes per launch: bench.py kernel_bytes() at this workload's sizes.",
         "",
         "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes / launch (scaled) | algorithmic bytes / launch | traffic / algorithmic |",
         "|---|---:|---:|---:|---:|---:|",
@@ -179,7 +183,7 @@ def pmc(bench_json: str, fetch_dir: str, write_dir: str, out_md: str, out_json: 
         ratio = f"{hbm / a:.2f}" if a else ""
         lines.append(f"| {k} | {f:.0f} | {w:.0f} | {hbm / 1e6:.2f} MB | {'' if not a else f'{a / 1e6:.2f} MB'} | "
                      f"{ratio} |")
-        js["kernels"][_base(k) if k.startswith(("k_cells", "k_vein<")) else k] = {
+        js["kernels"][_base(k)] = {
             "fetch_kib": f, "write_kib": w, "hbm_bytes_per_launch": hbm, "algorithmic_bytes_per_launch": a}
     Path(out_md).write_text("\n".join(lines) + "\n")
     Path(out_json).write_text(json.dumps(js, indent=1) + "\n")

@@ -279,3 +279,44 @@ def test_config2_crowded_regime(bcs, orc):
     sim.step(2)
     o.step(2)
     _compare(sim, o, "config 2 crowded regime (frames 301-302)")
+
+
+# ------------------------------------------------------------ FP64 drift --
+
+def test_fp64_drift_config2_subset_1000_frames(bcs, orc):
+    """Drift of the FP32 step against an FP64 rerun of the same step (the
+    oracle in double precision) over the first 1,000 frames of a named
+    config-2 subset: the config-2 vein and cell mix with 2% of its cells
+    (4,000 cells, 132,000 particles), so that the FP64 rerun fits a test on
+    the host cores.  Printed at frames 10, 50, 100, 250, 500 and 1,000, in
+    units of the cell radius r = 2: the median, 99th percentile and largest
+    particle distance |x32 - x64|, the distance of the two particle
+    centroids, and the relative difference of the mean speed.  The step is
+    chaotic (stiff springs, attractive contacts, teleports at the vein end):
+    single particles decorrelate within some 100 frames while the
+    aggregates stay close.  Bounds: at frame 10 every particle within
+    1e-3 r; at frame 1,000 everything finite, centroids within r and mean
+    speeds within 5%."""
+    sc = bcs.Scene(2, 0.02, seed=0)
+    sim = bcs.Simulation.from_scene(sc)
+    o = orc.Oracle(sc.physics, sc.types, sc.mesh, "f64")
+    o.set_particles(sc.positions, sc.velocities, sc.forces)
+    n = sim.np
+    done = 0
+    rows = {}
+    for mark in (10, 50, 100, 250, 500, 1000):
+        sim.step(mark - done)
+        o.step(mark - done)
+        done = mark
+        g, gv, _ = (a.reshape(3, n).astype(np.float64) for a in sim.get_particles())
+        d, dv, _ = (a.reshape(3, n) for a in o.get_particles())
+        assert np.isfinite(g).all() and np.isfinite(gv).all()
+        dist = np.sqrt(((g - d) ** 2).sum(axis=0)) / 2.0
+        cen = float(np.linalg.norm(g.mean(axis=1) - d.mean(axis=1)) / 2.0)
+        s32, s64 = np.sqrt((gv ** 2).sum(axis=0)).mean(), np.sqrt((dv ** 2).sum(axis=0)).mean()
+        rows[mark] = (float(np.median(dist)), float(np.percentile(dist, 99)), float(dist.max()), cen,
+                      float(abs(s32 - s64) / s64))
+        print(f"config-2 subset, frame {mark}: |x32 - x64| / r median {rows[mark][0]:.2e}, p99 {rows[mark][1]:.2e}, "
+              f"max {rows[mark][2]:.2e}; centroids {cen:.2e} r; mean speed {rows[mark][4]:.2e} rel", flush=True)
+    assert rows[10][2] < 1e-3, rows[10]
+    assert rows[1000][3] < 1.0 and rows[1000][4] < 0.05, rows[1000]
