@@ -291,8 +291,9 @@ int bcs_sim_set_ray_mode(bcs_sim* sim, int32_t mode);
  * of whole blood cells; BCS_CONTACT_SLOTS = in a kernel of its own whose lanes
  * follow the sorted slots of the particle grid (neighbouring lanes share their
  * neighbour cells), between a springs kernel and a response kernel;
- * BCS_CONTACT_AUTO (the default) = SLOTS.  Results are identical in every
- * mode.  No counterpart in the reference. */
+ * BCS_CONTACT_AUTO (the default) = SLOTS above 4,096 cell blocks (large
+ * workloads), else CELLS.  Results are identical in every mode.  No
+ * counterpart in the reference. */
 #define BCS_CONTACT_AUTO 0
 #define BCS_CONTACT_CELLS 1
 #define BCS_CONTACT_SLOTS 2
@@ -415,6 +416,8 @@ typedef struct bcs_ensemble_tuning {
     int32_t ray_mode;       /* BCS_RAY_SPLIT or BCS_RAY_FUSED for every member
                                (BCS_RAY_AUTO: fused when the members hold more than 4,096 cell blocks) */
     int32_t largest_first;  /* 1: the blocks of larger members come first in each launch */
+    int32_t contact_mode;   /* BCS_CONTACT_CELLS or BCS_CONTACT_SLOTS for every member
+                               (BCS_CONTACT_AUTO: slots when the members hold more than 4,096 cell blocks) */
 } bcs_ensemble_tuning;
 
 /* The members stay owned by the caller and must outlive the ensemble; all on

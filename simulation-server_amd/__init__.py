@@ -203,6 +203,7 @@ class EnsembleTuning(C.Structure):
         ("run", C.c_int32),
         ("ray_mode", C.c_int32),
         ("largest_first", C.c_int32),
+        ("contact_mode", C.c_int32),
     ]
 
 
@@ -685,15 +686,15 @@ class Ensemble:
         check(self.lib.bcs_ensemble_time_steps(self.h, int(n), C.byref(ms)), "ensemble time_steps")
         return ms.value
 
-    def set_tuning(self, blocks_per_cu=0, run=1, ray_mode=RAY_AUTO, largest_first=1):
-        t = EnsembleTuning(int(blocks_per_cu), int(run), int(ray_mode), int(largest_first))
+    def set_tuning(self, blocks_per_cu=0, run=1, ray_mode=RAY_AUTO, largest_first=1, contact_mode=CONTACT_AUTO):
+        t = EnsembleTuning(int(blocks_per_cu), int(run), int(ray_mode), int(largest_first), int(contact_mode))
         check(self.lib.bcs_ensemble_set_tuning(self.h, C.byref(t)), "ensemble set_tuning")
 
     def get_tuning(self) -> dict:
         t = EnsembleTuning()
         check(self.lib.bcs_ensemble_get_tuning(self.h, C.byref(t)), "ensemble get_tuning")
         return {"blocks_per_cu": t.blocks_per_cu, "run": t.run, "ray_mode": t.ray_mode,
-                "largest_first": t.largest_first}
+                "largest_first": t.largest_first, "contact_mode": t.contact_mode}
 
     def autotune(self, frames_per_trial=10) -> dict:
         """Times every tuning of the search space and keeps the fastest;

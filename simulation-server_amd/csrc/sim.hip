@@ -57,6 +57,9 @@ VeinPrepArgs vein_prep_args(const VeinArgs& V, int32_t ntri, const uint4* trun, 
                             float4* aabb, uint32_t* blocks);
 uint32_t ray_split(int nchunks);
 uint32_t cell_blocks(int nchunks);
+uint32_t tile_blocks(int64_t np);
+void batch_split_part(const StepArgs* a, int part, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run,
+                      hipStream_t st);
 uint32_t ray_blocks(int nchunks);
 uint32_t vein_blocks(int32_t nv);
 uint32_t order_key_blocks(const TypeTable& tt, int32_t otiles);
@@ -141,7 +144,10 @@ struct bcs_sim {
     }
     int32_t contact_mode = BCS_CONTACT_AUTO;
     float4* fslot = nullptr;  // split frames: F per sorted slot
-    bool contact_split() const { return contact_mode != BCS_CONTACT_CELLS; }
+    bool contact_split() const
+    {
+        return contact_mode == BCS_CONTACT_SLOTS || (contact_mode == BCS_CONTACT_AUTO && cell_blocks(nchunks) > 4096u);
+    }
     hipGraph_t graph = nullptr, graph_b = nullptr, graph_m = nullptr;
     std::vector<float> host_bs;
     std::vector<float> host_smallest;
@@ -269,6 +275,12 @@ int set_dev(bcs_sim* s)
 // per-triangle data and cell boxes are rebuilt every frame (k_vein_prep).
 int build_triangle_grid(bcs_sim* s)
 {
+    // host-side sizes before any launch: the grid holds every triangle, the
+    // per-triangle arrays and the cell table were allocated for this mesh
+    if (s->ntri < 0 || s->ntri > s->tg.cap || !s->tcent || !s->trun || !s->trec || !s->t_nmask ||
+        (s->ntri > 0 && (!s->tidx || !s->tg.table)) || s->tg.g.cell_count == 0 ||
+        s->tg.g.cell_count > 0xFFFFFFFFull)
+        return fail(BCS_ERR_STATE, "triangle grid: buffers do not cover the mesh");
     const bool dyn = s->phys.dynamic_triangle_grid != 0;
     if (dyn || !s->tg_built) launch_tri_centers(s->X, s->nv, s->tidx, s->ntri, s->tcent, s->stream);
     RC(grid_build(s->tg, s->tcent, s->ntri, s->ntri, nullptr, s->flags, s->stream));
@@ -1358,6 +1370,7 @@ namespace {
 enum EnsLaunch {
     EL_KEY, EL_SCAN, EL_SCATTER, EL_RANK, EL_OCC,  // grid build (GridBatch order)
     EL_PREP, EL_RAYCAND, EL_RAY, EL_CELLS, EL_VEIN,
+    EL_SPRINGS, EL_CONTACTS, EL_RESPOND,  // the split particle part (contact search by sorted slot)
     EL_OKEY, EL_OSCAN, EL_OSCAT, EL_CMAP,
     EL_COUNT
 };
@@ -1371,8 +1384,9 @@ struct bcs_ensemble {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<bcs_sim*> members;
     std::vector<int32_t> order;           // member of each argument slot
-    bcs_ensemble_tuning tuning{0, 1, BCS_RAY_AUTO, 1};
+    bcs_ensemble_tuning tuning{0, 1, BCS_RAY_AUTO, 1, BCS_CONTACT_AUTO};
     bool fused = true;
+    bool split = false;  // contact search by sorted slot (k_springs, k_contacts, k_respond)
     // device argument arrays (one entry per slot)
     StepArgs* d_step = nullptr;
     VeinArgs* d_vein = nullptr;
@@ -1447,6 +1461,9 @@ int ens_build_args(bcs_ensemble* e)
     for (bcs_sim* s : e->members) np_total += (size_t)s->nchunks;
     e->fused = e->tuning.ray_mode == BCS_RAY_FUSED ||
                (e->tuning.ray_mode == BCS_RAY_AUTO && cell_blocks((int)std::min<size_t>(np_total, INT32_MAX)) > 4096u);
+    e->split = e->tuning.contact_mode == BCS_CONTACT_SLOTS ||
+               (e->tuning.contact_mode == BCS_CONTACT_AUTO &&
+                cell_blocks((int)std::min<size_t>(np_total, INT32_MAX)) > 4096u);
     std::vector<StepArgs> step(n);
     std::vector<VeinArgs> vein(n);
     std::vector<VeinPrepArgs> prep(n);
@@ -1490,7 +1507,10 @@ int ens_build_args(bcs_ensemble* e)
         b[EL_PREP * n + k] = pb;
         b[EL_RAYCAND * n + k] = cell_blocks(s->nchunks);
         b[EL_RAY * n + k] = e->fused ? 0u : ray_blocks(s->nchunks);
-        b[EL_CELLS * n + k] = cell_blocks(s->nchunks);
+        b[EL_CELLS * n + k] = e->split ? 0u : cell_blocks(s->nchunks);
+        b[EL_SPRINGS * n + k] = e->split ? cell_blocks(s->nchunks) : 0u;
+        b[EL_CONTACTS * n + k] = e->split ? tile_blocks(s->np) : 0u;
+        b[EL_RESPOND * n + k] = e->split ? cell_blocks(s->nchunks) : 0u;
         b[EL_VEIN * n + k] = vein_blocks(s->nv);
         b[EL_OKEY * n + k] = order_key_blocks(s->tt, s->otiles);
         b[EL_OSCAN * n + k] = s->tt.nc > 0 ? (uint32_t)s->otiles : 0u;
@@ -1567,7 +1587,13 @@ int ens_enqueue_frame(bcs_ensemble* e, bool lead_vein)
                  n, run};
     grid_build_batch(gb, st);
     HIPCHK(hipStreamWaitEvent(st, e->ev_join, 0));
-    batch_cells(e->d_step, e->start(EL_CELLS), n, e->grid(EL_CELLS), run, st);
+    if (e->split) {
+        batch_split_part(e->d_step, 0, e->start(EL_SPRINGS), n, e->grid(EL_SPRINGS), run, st);
+        batch_split_part(e->d_step, 1, e->start(EL_CONTACTS), n, e->grid(EL_CONTACTS), run, st);
+        batch_split_part(e->d_step, 2, e->start(EL_RESPOND), n, e->grid(EL_RESPOND), run, st);
+    } else {
+        batch_cells(e->d_step, e->start(EL_CELLS), n, e->grid(EL_CELLS), run, st);
+    }
     HIPCHK(hipGetLastError());
     return BCS_OK;
 }
@@ -1633,6 +1659,8 @@ int ens_set_tuning(bcs_ensemble* e, const bcs_ensemble_tuning& t)
         return fail(BCS_ERR_INVALID_ARGUMENT, "ensemble tuning: blocks_per_cu in 0..64, run in 1..64");
     if (t.ray_mode != BCS_RAY_AUTO && t.ray_mode != BCS_RAY_SPLIT && t.ray_mode != BCS_RAY_FUSED)
         return fail(BCS_ERR_INVALID_ARGUMENT, "ensemble tuning: ray_mode is BCS_RAY_AUTO, _SPLIT or _FUSED");
+    if (t.contact_mode != BCS_CONTACT_AUTO && t.contact_mode != BCS_CONTACT_CELLS && t.contact_mode != BCS_CONTACT_SLOTS)
+        return fail(BCS_ERR_INVALID_ARGUMENT, "ensemble tuning: contact_mode is BCS_CONTACT_AUTO, _CELLS or _SLOTS");
     HIPCHK(hipStreamSynchronize(e->stream));
     e->tuning = t;
     e->order.resize(e->members.size());
@@ -1775,7 +1803,7 @@ extern "C" int bcs_ensemble_autotune(bcs_ensemble* e, int32_t frames_per_trial, 
             for (int32_t ray : kRay)
                 for (int32_t lf : kLargest) {
                     if (bpc == 0 && run != 1) continue;  // without persistence a run only merges blocks
-                    const bcs_ensemble_tuning t{bpc, run, ray, lf};
+                    const bcs_ensemble_tuning t{bpc, run, ray, lf, e->tuning.contact_mode};
                     RC(ens_set_tuning(e, t));
                     RC(ens_step(e, 1));  // graphs captured, caches warm
                     float ms = 0.f;

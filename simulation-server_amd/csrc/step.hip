@@ -2012,6 +2012,21 @@ __global__ void __launch_bounds__(kBlock) k_cells_batch(const StepArgs* __restri
 {
     for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { cells_body<true>(a[k], vb); });
 }
+__global__ void __launch_bounds__(kBlock) k_springs_batch(const StepArgs* __restrict__ a,
+                                                          const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { cells_body<true, false, kCellsSprings>(a[k], vb); });
+}
+__global__ void __launch_bounds__(kBlock) k_contacts_batch(const StepArgs* __restrict__ a,
+                                                           const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { contacts_body<false>(a[k], vb); });
+}
+__global__ void __launch_bounds__(kBlock) k_respond_batch(const StepArgs* __restrict__ a,
+                                                          const uint32_t* __restrict__ start, int32_t nsim, uint32_t run)
+{
+    for_virtual_blocks(start, nsim, run, [&](int32_t k, VBlock vb) { respond_body(a[k], vb); });
+}
 __global__ void __launch_bounds__(256) k_vein_batch(const VeinArgs* __restrict__ a, const uint32_t* __restrict__ start,
                                                     int32_t nsim, uint32_t run)
 {
@@ -2043,6 +2058,7 @@ __global__ void __launch_bounds__(kBlock) k_chunk_map_batch(const StepArgs* __re
 // Blocks of each single-simulation launch, as the batched launches split
 // them (the same counts the launchers above use).
 uint32_t cell_blocks(int nchunks) { return nchunks > 0 ? chunk_grid(nchunks).x : 0u; }
+uint32_t tile_blocks(int64_t np) { return np > 0 ? chunk_grid((int)((np + kBlock - 1) / kBlock)).x : 0u; }
 uint32_t ray_blocks(int nchunks) { return nchunks > 0 ? chunk_grid(nchunks).x * ray_split(nchunks) : 0u; }
 uint32_t vein_blocks(int32_t nv) { return (uint32_t)((std::max(nv, 1) + 255) / 256); }
 uint32_t order_key_blocks(const TypeTable& tt, int32_t otiles)
@@ -2066,6 +2082,14 @@ void batch_ray(const StepArgs* a, const uint32_t* split, const uint32_t* start, 
 void batch_cells(const StepArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run, hipStream_t st)
 {
     if (grid) hipLaunchKernelGGL(k_cells_batch, dim3(grid), dim3(kBlock), 0, st, a, start, nsim, run);
+}
+void batch_split_part(const StepArgs* a, int part, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run,
+                      hipStream_t st)
+{
+    if (!grid) return;
+    if (part == 0) hipLaunchKernelGGL(k_springs_batch, dim3(grid), dim3(kBlock), 0, st, a, start, nsim, run);
+    else if (part == 1) hipLaunchKernelGGL(k_contacts_batch, dim3(grid), dim3(kBlock), 0, st, a, start, nsim, run);
+    else hipLaunchKernelGGL(k_respond_batch, dim3(grid), dim3(kBlock), 0, st, a, start, nsim, run);
 }
 void batch_vein(const VeinArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run, hipStream_t st)
 {

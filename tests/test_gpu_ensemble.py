@@ -30,8 +30,9 @@ def _compare_members(sims, refs, what):
         assert a.get_info().step_index == b.get_info().step_index
 
 
-@pytest.mark.parametrize("tuning", [dict(blocks_per_cu=0, run=1, ray_mode=2, largest_first=1),
-                                    dict(blocks_per_cu=2, run=4, ray_mode=1, largest_first=0)])
+@pytest.mark.parametrize("tuning", [dict(blocks_per_cu=0, run=1, ray_mode=2, largest_first=1, contact_mode=1),
+                                    dict(blocks_per_cu=2, run=4, ray_mode=1, largest_first=0, contact_mode=2),
+                                    dict(blocks_per_cu=4, run=1, ray_mode=2, largest_first=1, contact_mode=2)])
 def test_ensemble_members_equal_solo_runs(bcs, tuning):
     scenes = _scenes(bcs)
     solo = _sims(bcs, scenes)
@@ -91,3 +92,5 @@ def test_ensemble_rejects_bad_members(bcs):
     ens = bcs.Ensemble([a])
     with pytest.raises(bcs.BcsError):
         ens.set_tuning(blocks_per_cu=-1)
+    with pytest.raises(bcs.BcsError):
+        ens.set_tuning(contact_mode=7)
