@@ -291,6 +291,8 @@ def bench_surrogate(bcs, args):
     radius, for args.surr_frames frames after the plain frames."""
     t_all = time.perf_counter()
     src = bcs.Simulation.from_scene(bcs.Scene(2, 1.0, seed=args.seed), device=0)
+    if args.no_graph:
+        src.use_graph(False)
     src.step(20)
     feats, targs = [], []
     while sum(len(f) for f in feats) < 200000:
@@ -320,6 +322,8 @@ def bench_surrogate(bcs, args):
     runs = {}
     for mode in ("plain", "surrogate"):
         sim = bcs.Simulation.from_scene(scene, device=0)
+        if args.no_graph:
+            sim.use_graph(False)
         if mode == "surrogate":
             sim.attach_surrogate(net, tol)
         sim.step(max(args.warmup, 1))
@@ -346,6 +350,8 @@ def bench_surrogate(bcs, args):
            "mfma_frac": SURR_FLOP_PER_CELL * ncell / (ms * 1e-3) / MFMA_BF16_PEAK}
     out.update({
         "metric": "cell-steps/s (42-particle cells, surrogate frames vs plain frames)",
+        "steps": args.surr_frames,
+        "warmup": max(args.warmup, 1),
         "value": runs["surrogate"]["cell_steps_per_s"],
         "unit": "cell-steps/s",
         "n_gpus": 1,

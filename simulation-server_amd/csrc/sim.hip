@@ -91,6 +91,8 @@ struct bcs_sim {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;        // second branch of the frame (vein springs, triangle data)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t third = nullptr;       // third branch: the surrogate's network (when one is attached)
+    hipEvent_t ev_join3 = nullptr;
     bcs_physics phys{};
     int32_t nt = 0, np = 0, nc = 0, nm = 0, nv = 0, ntri = 0;
     float bmin[3]{}, bmax[3]{};
@@ -181,6 +183,8 @@ struct bcs_sim {
         for (void* p : allocs) (void)hipFree(p);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_join3) (void)hipEventDestroy(ev_join3);
+        if (third) (void)hipStreamDestroy(third);
         if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -340,11 +344,17 @@ int enqueue_fused_frame(bcs_sim* s, const KernelTimer* timer = nullptr, bool lea
             launch_ray(A, s->side);
         }
         // the network's springs need only the frame's starting x, v and the
-        // last frame's contact flags: they ride on the same branch
-        if (s->surr && !s->sampling) enqueue_surrogate(s, s->side);
+        // last frame's contact flags: a third branch beside the other two
+        const bool net = s->surr && !s->sampling;
+        if (net) {
+            HIPCHK(hipStreamWaitEvent(s->third, s->ev_fork, 0));
+            enqueue_surrogate(s, s->third);
+            HIPCHK(hipEventRecord(s->ev_join3, s->third));
+        }
         HIPCHK(hipEventRecord(s->ev_join, s->side));
         RC(enqueue_particle_grid(s));
         HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
+        if (net) HIPCHK(hipStreamWaitEvent(st, s->ev_join3, 0));
     } else {
         if (lead_vein) launch_vein(Va, 7, st);
         RC(enqueue_particle_grid(s, timer));
@@ -419,6 +429,8 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
     HIPCHK(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    HIPCHK(hipStreamCreateWithFlags(&s->third, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&s->ev_join3, hipEventDisableTiming));
     s->phys = *phys;
     s->nt = n_types;
     // ---- type tables (blood_cell_factory.hpp:172-282)
@@ -1863,9 +1875,9 @@ int surr_alloc(bcs_sim* s)
         RC(s->dalloc(&s->sf_S, nc * kSurrOutPad));
         RC(s->dalloc(&s->sf_f32, nc * kSurrIn));
         RC(s->dalloc(&s->touched, (size_t)std::max(s->np, 1)));
-        RC(s->dalloc(&s->surr_stats, 4));
+        RC(s->dalloc(&s->surr_stats, (size_t)kSurrStatSlots * kSurrStatStride));
         HIPCHK(hipMemsetAsync(s->touched, 0, (size_t)std::max(s->np, 1), s->stream));
-        HIPCHK(hipMemsetAsync(s->surr_stats, 0, 16, s->stream));
+        HIPCHK(hipMemsetAsync(s->surr_stats, 0, 4 * (size_t)kSurrStatSlots * kSurrStatStride, s->stream));
     }
     return BCS_OK;
 }
@@ -1947,17 +1959,26 @@ extern "C" int bcs_sim_surrogate_stats(bcs_sim* s, int32_t reset, uint64_t* acce
     RC(set_dev(s));
     if (!s->surr_stats) return fail(BCS_ERR_STATE, "surrogate: none attached");
     HIPCHK(hipStreamSynchronize(s->stream));
-    uint32_t h[4] = {0, 0, 0, 0};
-    HIPCHK(hipMemcpy(h, s->surr_stats, 16, hipMemcpyDeviceToHost));
-    if (accepted) *accepted = h[0];
-    if (fallback) *fallback = h[1];
+    std::vector<uint32_t> h((size_t)kSurrStatSlots * kSurrStatStride);
+    HIPCHK(hipMemcpy(h.data(), s->surr_stats, 4 * h.size(), hipMemcpyDeviceToHost));
+    uint64_t acc = 0, fb = 0;
+    uint32_t e_acc = 0, e_all = 0;  // non-negative floats: bit order is value order
+    for (int k = 0; k < kSurrStatSlots; ++k) {
+        const uint32_t* w = h.data() + (size_t)k * kSurrStatStride;
+        acc += w[0];
+        fb += w[1];
+        e_acc = std::max(e_acc, w[2]);
+        e_all = std::max(e_all, w[3]);
+    }
+    if (accepted) *accepted = acc;
+    if (fallback) *fallback = fb;
     float f2, f3;
-    std::memcpy(&f2, &h[2], 4);
-    std::memcpy(&f3, &h[3], 4);
+    std::memcpy(&f2, &e_acc, 4);
+    std::memcpy(&f3, &e_all, 4);
     if (max_error_accepted) *max_error_accepted = f2;
     if (max_error_all) *max_error_all = f3;
     if (reset) {
-        HIPCHK(hipMemsetAsync(s->surr_stats, 0, 16, s->stream));
+        HIPCHK(hipMemsetAsync(s->surr_stats, 0, 4 * h.size(), s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
     }
     return BCS_OK;

@@ -1013,9 +1013,10 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
                     }
                 }
                 __syncthreads();
-                if (l < 4 && s_stat[l] != 0u) {  // one global update per tally and block
-                    if (l < 2) atomicAdd(&sa.stats[l], s_stat[l]);
-                    else atomicMax(&sa.stats[l], s_stat[l]);
+                if (l < 4 && s_stat[l] != 0u) {  // one global update per tally and block, in the block's slot
+                    uint32_t* slot = sa.stats + (size_t)(vb.id % kSurrStatSlots) * kSurrStatStride;
+                    if (l < 2) atomicAdd(&slot[l], s_stat[l]);
+                    else atomicMax(&slot[l], s_stat[l]);
                 }
             }
         }
@@ -1104,10 +1105,10 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
     // lanes); a lane past the end has an empty mask
     const int Tl = act && tt.nt > 1 ? type_of_index(tt, (int32_t)u) : 0;
     const uint32_t nc = contact_sum<kPruneCell>(A, i, Tl, key_of, mask, x1, v1, F, fl, radius, s_hit);
-    if (act) {
-        A.fslot[u] = make_float4(F.x, F.y, F.z, 0.0f);
-        if (SURR && A.surr.touched) A.surr.touched[i] = nc > 0u ? 1 : 0;
-    }
+    // w: whether the particle touched another (for the surrogate's next
+    // input, written per particle by k_respond)
+    if (act) A.fslot[u] = make_float4(F.x, F.y, F.z, __uint_as_float(nc > 0u ? 1u : 0u));
+    (void)SURR;
     if (fl) atomicOr(A.flags, fl);
 }
 template <bool SURR>
@@ -1140,6 +1141,7 @@ __device__ __forceinline__ void respond_body(const StepArgs& A, VBlock vb)
         v1 = load3(A.vel, np, i);
         const float4 f = A.fslot[A.p_slot[i]];
         F = mk(f.x, f.y, f.z);
+        if (A.surr.touched) A.surr.touched[i] = __float_as_uint(f.w) != 0u ? 1 : 0;
     }
     if (l < ncell) s_tele[l] = 0;
     respond_integrate<true>(A, act, i, T, a, cl, cell, x1, v1, F, 0u, s_tele);

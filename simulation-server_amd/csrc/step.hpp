@@ -10,6 +10,7 @@ constexpr int kBlock = 256;  // threads per cell-chunk block
 constexpr int kBsLds = 1024;       // model-vertex radii held in LDS by the contact search (more: read globally)
 constexpr int kContactBatch = 4;   // sorted-slot records loaded together by the contact search
 constexpr int kContactWindow = 16; // stream positions tested before their contacts are applied (<= 32)
+constexpr int kSurrStatSlots = 256, kSurrStatStride = 16;  // surrogate tallies: 256 slots, 64 B apart
 constexpr int kCellBatch = 6;      // neighbour-cell table entries loaded together by the contact search
 constexpr uint32_t kXcdRun = 16;   // consecutive chunks per XCD run in the block -> chunk mapping
 constexpr int kSpringLds = 1024;   // springs of one cell type held in LDS by k_cells (more: read globally)
@@ -52,8 +53,9 @@ struct SurrArgs {
                            // normalised units (times out_scale: spring displacement); null: no prediction
     float* S_out;          // sampling: (dt^2/4) S per particle of `type`, same layout; null: none
     uint8_t* touched;      // per particle: 1 if it had a contact in this frame's search (null: not kept)
-    uint32_t* stats;       // [0] accepted cells, [1] fallback cells, [2] max error of accepted cells (float
-                           // bits), [3] max error of all predicted cells (float bits)
+    uint32_t* stats;       // kSurrStatSlots slots of kSurrStatStride words (spreading the atomics): [0]
+                           // accepted cells, [1] fallback cells, [2] max error of accepted cells (float
+                           // bits), [3] max error of all predicted cells (float bits); summed on the host
     int32_t type;          // the served type (ppc 42); -1: none
     float scale;           // dt^2 / 4
     float out_scale;       // network units -> displacement

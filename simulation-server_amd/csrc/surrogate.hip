@@ -199,14 +199,28 @@ __global__ void __launch_bounds__(256) k_gelu_back(const float* __restrict__ dy,
     dzb[e] = to_bf16(v);
 }
 
-// db[n] = sum over the B rows of dz[m][n] (row stride ld), in row order.
-__global__ void __launch_bounds__(256) k_colsum(const float* __restrict__ dz, int32_t B, int32_t N, int64_t ld,
+// db[n] = sum over the B rows of dz[m][n] (row stride ld), in two
+// deterministic passes: partial sums of kColRows rows per (chunk, column),
+// then the chunks in order.
+constexpr int kColRows = 64;
+__global__ void __launch_bounds__(256) k_colsum_part(const float* __restrict__ dz, int32_t B, int32_t N, int64_t ld,
+                                                     float* __restrict__ part)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (n >= N) return;
+    const int32_t m0 = c * kColRows, m1 = min(B, m0 + kColRows);
+    float s = 0.f;
+    for (int32_t m = m0; m < m1; ++m) s += dz[(int64_t)m * ld + n];
+    part[(int64_t)c * N + n] = s;
+}
+__global__ void __launch_bounds__(256) k_colsum(const float* __restrict__ part, int32_t chunks, int32_t N,
                                                 float* __restrict__ db)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     float s = 0.f;
-    for (int32_t m = 0; m < B; ++m) s += dz[(int64_t)m * ld + n];
+    for (int32_t c = 0; c < chunks; ++c) s += part[(int64_t)c * N + n];
     db[n] = s;
 }
 
@@ -708,6 +722,9 @@ extern "C" int bcs_surrogate_train(bcs_surrogate* s, const float* features, cons
     rc = rc ? rc : talloc((void**)&wt, (size_t)kSurrInPad * kSurrHidden * 2);
     rc = rc ? rc : talloc((void**)&ta, (size_t)kSurrHidden * B * 2);
     rc = rc ? rc : talloc((void**)&tb, (size_t)kSurrInPad * B * 2);
+    const int32_t colchunks = (B + kColRows - 1) / kColRows;
+    float* colpart = nullptr;  // per row chunk: the partial bias gradients
+    rc = rc ? rc : talloc((void**)&colpart, (size_t)colchunks * kSurrHidden * 4);
     if (rc) {
         cleanup();
         return rc;
@@ -754,7 +771,9 @@ extern "C" int bcs_surrogate_train(bcs_surrogate* s, const float* features, cons
                 transpose<false>(dzb, outn, B, outn, ta, B, st);
                 transpose<false>(act[l], in, B, in, tb, B, st);
                 gemm(ta, B, tb, B, outn, in, B, nullptr, s->grad + s->off_w[l], in, nullptr, 0, 0, st);
-                hipLaunchKernelGGL(k_colsum, dim3(blocks_for(outn)), dim3(256), 0, st, dzf, B, outn, (int64_t)outn,
+                hipLaunchKernelGGL(k_colsum_part, dim3(blocks_for(outn), (unsigned)colchunks), dim3(256), 0, st, dzf, B,
+                                   outn, (int64_t)outn, colpart);
+                hipLaunchKernelGGL(k_colsum, dim3(blocks_for(outn)), dim3(256), 0, st, colpart, colchunks, outn,
                                    s->grad + s->off_b[l]);
                 if (l == 0) break;
                 // dY_{l-1} = dZ W_l : A = dZ (B x out), B = W_l^T (in x out), K = out

@@ -34,6 +34,7 @@ def _short(name: str) -> str:
     name = name.strip()
     if name.startswith("void "):
         name = name[5:]
+    name = name.replace("(anonymous namespace)::", "")
     if name.startswith("bcs::"):
         name = name[5:]
     depth = 0
