@@ -82,14 +82,32 @@ __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
     }
     if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
     if (w != kNoWord) bm.words[w] = 0;
-    uint32_t fl = 0;
+    uint32_t fl = 0, b = 0xFFFFFFFFu;
     if (have) {
-        uint32_t k = cell_key(g, x, y, z, fl);
+        const uint32_t k = cell_key(g, x, y, z, fl);
         keys[i] = k;
-        // the count before this object: its slot within the bucket (the
-        // scatter needs no atomics; slot order within a bucket is free)
-        a.arrival[i] = atomicAdd(&a.bcount[k >> a.shift], 1u);
+        b = k >> a.shift;
     }
+    // The count before this object is its slot within the bucket (the
+    // scatter needs no atomics; slot order within a bucket is free, the
+    // ranking orders by key and index).  Consecutive lanes of one bucket
+    // (spatially ordered lanes mostly are) take their slots with one add by
+    // the run's first lane: slot = the run's base + the lane's place in it.
+    const uint32_t lane = __lane_id();
+    const uint32_t up = __shfl_up(b, 1u, 64);
+    const bool head = have && (lane == 0u || up != b);
+    const uint64_t heads = __ballot(head);
+    const uint64_t breaks = heads | __ballot(!have);
+    const uint64_t upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
+    const uint32_t lead = 63u - (uint32_t)__clzll((heads & upto) | 1ull);  // this lane's run head (have lanes)
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t later = breaks & ~upto;  // the first break after this head ends its run
+        const uint32_t end = later ? (uint32_t)__ffsll((long long)later) - 1u : 64u;
+        base = atomicAdd(&a.bcount[b], end - lane);
+    }
+    base = __shfl(base, (int)lead, 64);
+    if (have) a.arrival[i] = base + (lane - lead);
     if (fl) atomicOr(a.flags, fl);
 }
 __global__ void __launch_bounds__(256) k_key(KeyArgs a) { key_body(a, hw_block()); }

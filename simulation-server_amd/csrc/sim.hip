@@ -534,6 +534,16 @@ extern "C" int bcs_sim_create(const bcs_physics* phys, const bcs_cell_type* type
             if (!(r >= 0.0f && r <= 3.0e38f)) rmax = INFINITY;
             else rmax = std::max(rmax, r);
         pc.rmax = rmax;
+        // The contact search may skip the partner radius lookup when it
+        // cannot change a result: every radius equal (so r2 = rmax) and no
+        // partner model index clamped (a partner below the launch type's
+        // first particle selects mstart - ((pstart - j) mod ppc), which stays
+        // in the table when mstart >= ppc - 1).
+        bool uni = rmax < INFINITY;
+        for (float r : s->host_bs) uni = uni && r == rmax;
+        for (int t = 0; t < n_types; ++t)
+            if (tt.pstart[t] > 0 && tt.mstart[t] < tt.ppc[t] - 1) uni = false;
+        pc.r_uniform = uni ? 1 : 0;
     }
     for (int e = 0; e < mesh->ending_count; ++e) {
         for (int d = 0; d < 3; ++d) pc.endc[3 * e + d] = mesh->ending_centers[3 * e + d];

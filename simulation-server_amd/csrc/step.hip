@@ -807,9 +807,15 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                     const f3 rel = sub(x1, mk(q[t].x, q[t].y, q[t].z));
                     const float d2 = dot(rel, rel);
                     if (d2 <= bound2) {
-                        const float r2 = radius(model_index(mi, j, fl));
-                        const float minD = r1 + r2;
-                        if (d2 <= minD * minD && d2 >= 0.0001f) {
+                        // with one radius throughout, minD = r1 + rmax and
+                        // minD^2 is bound2 itself
+                        bool touch = d2 >= 0.0001f;
+                        if (!pc.r_uniform) {
+                            const float r2 = radius(model_index(mi, j, fl));
+                            const float minD = r1 + r2;
+                            touch = touch && d2 <= minD * minD;
+                        }
+                        if (touch) {
                             hit_list[nh * kBlock] = slot[t];
                             ++nh;
                         }

@@ -38,7 +38,7 @@ struct PhysConst {
     float aabb_margin;      // absolute inflation of the triangle-cell boxes (rounding slack)
     float rmax;             // largest contact radius of the model table (+inf: unusable as a bound)
     int32_t rk4;            // particle springs and integration of the reference's Runge-Kutta build
-    int32_t pad1;
+    int32_t r_uniform;      // every radius a partner's model index can select is rmax, and none clamps
     uint64_t seed;
     float endc[3 * kMaxEndings];
     float endr[kMaxEndings];
