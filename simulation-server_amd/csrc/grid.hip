@@ -360,8 +360,8 @@ __device__ __forceinline__ void rank_write(const RankOut& o, uint32_t key, int32
     o.keys_sorted[pos] = key;
     o.perm[pos] = idx;
     if (o.pvs) {
-        o.pvs[2 * (size_t)pos] = ra;
-        o.pvs[2 * (size_t)pos + 1] = rb;
+        o.pvs[pos] = ra;
+        o.pvs[(size_t)o.pvs_stride + pos] = rb;
     }
     if (o.slot_of) o.slot_of[idx] = (int32_t)pos;
     if (first) {
@@ -955,7 +955,7 @@ int grid_launch_args(DeviceGrid& G, const float* pos, int64_t stride, int32_t n,
     L->scatter = ScatterArgs{G.keys, n, G.shift, G.bstart, G.arrival, G.items, flags, pos, vel, stride, brec};
     L->scatter_blocks = (uint32_t)blocks;
     const bool rank_map = grid_map_in_rank(G);
-    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr,
+    RankOut o{G.keys_sorted, G.perm, G.slot_of, G.table, G.runs, G.ctrl + 2, brec, brec ? G.pvs : nullptr, G.cap,
               G.bm, rank_map ? G.gpos : nullptr, G.shift, G.ibits, {G.rk[0], G.rk[1]}, {G.rq[0], G.rq[1]}};
     L->rank = RankArgs{G.items, n, G.shift, G.bstart, G.biglist, G.ctrl, G.big_cap, o};
     L->rank_blocks = (uint32_t)blocks;

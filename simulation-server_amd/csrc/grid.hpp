@@ -39,7 +39,8 @@ struct DeviceGrid {
     uint32_t* gpos = nullptr;         // per sorted slot: map word written by the last build, or kNoWord
     float4* brec = nullptr;           // per bucketed item q (optional): brec[2q] = (x, y, z, object
                                       // index as int bits), brec[2q+1] = (vx, vy, vz, 0)
-    float4* pvs = nullptr;            // the same records per sorted slot
+    float4* pvs = nullptr;            // the same records per sorted slot u: positions at pvs[u],
+                                      // velocities at pvs[cap + u]
     uint32_t* arrival = nullptr;      // per object: its slot within its bucket (histogram order)
     uint32_t* bcount = nullptr;       // bucket histogram [nb+1], zero between builds (the scan clears it)
     uint32_t* bstart = nullptr;       // bucket offsets [nb+2]
@@ -141,7 +142,8 @@ struct RankOut {
     uint32_t* runs;        // run starts of occupied cells, in no particular order (nullable)
     uint32_t* nruns;
     const float4* brec;    // per-item records (nullable) ...
-    float4* pvs;           // ... copied to their sorted slots
+    float4* pvs;           // ... copied to their sorted slots (position records, then velocity records
+    int32_t pvs_stride;    // pvs_stride entries further on)
     BrickMap bm;           // occupancy map (bm.words nullable) ...
     uint32_t* gpos;        // ... and the word each sorted slot wrote (kNoWord: none)
     int shift;             // bucket = key >> shift

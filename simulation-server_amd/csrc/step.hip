@@ -798,12 +798,14 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                 float4 q[kContactBatch];
 #pragma unroll
                 for (int t = 0; t < kContactBatch; ++t)
-                    if (t < nval) q[t] = A.pvs[2 * (size_t)slot[t]];
+                    if (t < nval) q[t] = A.pvs[slot[t]];
 #pragma unroll
                 for (int t = 0; t < kContactBatch; ++t) {
                     if (t >= nval) continue;
                     const int32_t j = __float_as_int(q[t].w);
-                    if (j == i) continue;
+                    // (with the grid of this frame x1 is the particle's own
+                    // record, whose d2 = 0 never makes a contact below)
+                    if (PRUNE != kPruneCell && j == i) continue;
                     const f3 rel = sub(x1, mk(q[t].x, q[t].y, q[t].z));
                     const float d2 = dot(rel, rel);
                     if (d2 <= bound2) {
@@ -822,9 +824,21 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                     }
                 }
             }
+            // the next contact's record is loaded before this one's force
+            // is computed (the loads of contact h + 1 overlap the math of h)
+            float4 qn = make_float4(0.f, 0.f, 0.f, 0.f), wn = qn;
+            if (nh) {
+                const uint32_t sl = hit_list[0];
+                qn = A.pvs[sl];
+                wn = A.pvs[np + sl];
+            }
             for (uint32_t h = 0; h < nh; ++h) {
-                const uint32_t sl = hit_list[h * kBlock];
-                const float4 q = A.pvs[2 * (size_t)sl], w = A.pvs[2 * (size_t)sl + 1];
+                const float4 q = qn, w = wn;
+                if (h + 1u < nh) {
+                    const uint32_t sl = hit_list[(h + 1u) * kBlock];
+                    qn = A.pvs[sl];
+                    wn = A.pvs[np + sl];
+                }
                 const f3 rel = sub(x1, mk(q.x, q.y, q.z));
                 const float d2 = dot(rel, rel);
                 const f3 rv = sub(v1, mk(w.x, w.y, w.z));
@@ -1091,7 +1105,7 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
     int32_t i = 0;
     f3 x1 = mk(0.f, 0.f, 0.f), v1 = x1, F = x1;
     if (act) {
-        const float4 q = A.pvs[2 * u], w = A.pvs[2 * u + 1], f = A.fslot[u];
+        const float4 q = A.pvs[u], w = A.pvs[np + u], f = A.fslot[u];
         i = __float_as_int(q.w);
         x1 = mk(q.x, q.y, q.z);
         v1 = mk(w.x, w.y, w.z);
@@ -1118,7 +1132,7 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
     if (fl) atomicOr(A.flags, fl);
 }
 template <bool SURR>
-__global__ void __launch_bounds__(kBlock) k_contacts(StepArgs A)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) k_contacts(StepArgs A)
 {
     contacts_body<SURR>(A, hw_block());
 }
@@ -1694,8 +1708,8 @@ __global__ void __launch_bounds__(256) k_refresh_copies(const float* __restrict_
     const int32_t i = perm[u];
     if (i < 0 || i >= n) return;
     const f3 x = load3(pos, n, i), v = load3(vel, n, i);
-    pvs[2 * (size_t)u] = make_float4(x.x, x.y, x.z, __int_as_float(i));
-    pvs[2 * (size_t)u + 1] = make_float4(v.x, v.y, v.z, 0.0f);
+    pvs[u] = make_float4(x.x, x.y, x.z, __int_as_float(i));
+    pvs[(size_t)n + u] = make_float4(v.x, v.y, v.z, 0.0f);
 }
 
 // Frame export: SoA -> xyz interleaved with a given stride (3 or 6 floats).

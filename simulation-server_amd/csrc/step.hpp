@@ -93,8 +93,8 @@ struct StepArgs {
     const uint2* p_table;   // {start, count} per cell (current where occupied in p_map)
     BrickMap p_map;         // 2-bit brick occupancy map of the particle grid
     const uint32_t* p_key_of;  // per particle: its cell key in the last build (gridCellIds by particle)
-    const float4* pvs;      // per sorted slot u, as of the last grid build: pvs[2u] = (x, y, z,
-                            // particle perm[u] as int bits), pvs[2u+1] = (vx, vy, vz, 0)
+    const float4* pvs;      // per sorted slot u, as of the last grid build: pvs[u] = (x, y, z,
+                            // particle perm[u] as int bits), pvs[np + u] = (vx, vy, vz, 0)
     float4* fslot;          // split frames: per sorted slot, F after the springs, then after the contacts
     int32_t ntiles;         // split frames: slot tiles of kBlock lanes (k_contacts blocks)
     // triangle grid + vein
