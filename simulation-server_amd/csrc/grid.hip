@@ -6,7 +6,7 @@
 // calculateStartAndEndOfCellKernel (:51-80).
 //
 // Sort: keys are split into buckets of 2^shift consecutive cell ids.
-//   k_key      key per object, bucket histogram (atomics on a small table)
+//   k_key      key per object, bucket histogram (per block in LDS, one add per bucket)
 //   k_scan     exclusive scan of the histogram (single pass, decoupled look-back)
 //   k_scatter  (key, index) pairs into their bucket segments
 //   k_rank     in-bucket rank by counting over (key, index): the output order is
@@ -43,8 +43,20 @@ constexpr uint64_t kStatusAgg = 1ull << 62;
 constexpr uint64_t kStatusInc = 2ull << 62;
 constexpr uint64_t kSpinLimit = 1ull << 22;
 
+// Lanes of the key kernel: kKeyRounds rounds of one lane per thread per
+// block, so that a block counts the objects of kKeyRounds * 256 lanes (with a
+// lane order: neighbouring blood cells) into its buckets before it adds to the
+// histogram, once per distinct bucket.
+constexpr int kKeyRounds = 4;
+constexpr uint32_t kKeyHashBits = 10;
+constexpr uint32_t kKeyHash = 1u << kKeyHashBits;  // LDS bucket table (room for every lane's bucket)
+constexpr uint32_t kKeyEmpty = 0xFFFFFFFFu;
+static_assert(kKeyHash >= 256u * kKeyRounds, "the bucket table must hold a bucket per lane");
+
 __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
 {
+    __shared__ uint32_t s_bkt[kKeyHash];  // bucket of each table entry (kKeyEmpty: free)
+    __shared__ uint32_t s_cnt[kKeyHash];  // the block's objects in it, then the block's first slot in it
     const GridGeom& g = a.g;
     const float* __restrict__ pos = a.pos;
     const int64_t stride = a.stride;
@@ -54,60 +66,81 @@ __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
     const BrickMap& bm = a.bm;
     const uint32_t* __restrict__ gpos = a.gpos;
     const int2* __restrict__ order = a.order;
-    const int64_t gid = (int64_t)vb.id * blockDim.x + threadIdx.x;
-    const int64_t nthreads = (int64_t)vb.count * blockDim.x;
-    for (int64_t t = gid; t < a.ntiles; t += nthreads) a.status[t] = 0ull;
-    if (gid == 0) {
+    const uint32_t tid = threadIdx.x;
+    {
+        const int64_t t0 = (int64_t)vb.id * blockDim.x + tid, nthreads = (int64_t)vb.count * blockDim.x;
+        for (int64_t t = t0; t < a.ntiles; t += nthreads) a.status[t] = 0ull;
+    }
+    if (vb.id == 0 && tid == 0) {
         a.ctrl[0] = 0u;  // scan ticket
         a.ctrl[1] = 0u;  // big-bucket count
         a.ctrl[2] = 0u;  // occupied cells (run list)
     }
-    // every load first (one round trip), then the clears and the new key
-    uint32_t old = 0xFFFFFFFFu, w = kNoWord;
-    float x = 0.f, y = 0.f, z = 0.f;
-    if (gid < n_prev) {  // the previous build's cell of this object
-        if (a.clear_table) old = keys[gid];
-        if (gpos) w = gpos[gid];  // map word written by the previous build (indexed by its sorted slots)
+    for (uint32_t e = tid; e < kKeyHash; e += blockDim.x) {
+        s_bkt[e] = kKeyEmpty;
+        s_cnt[e] = 0u;
     }
-    // the object of this thread: gid, or with `order` the object of lane gid
-    // (objects in spatial order, so that neighbouring lanes count into
-    // neighbouring buckets; -1: no object)
-    int64_t i = gid < n ? gid : -1;
-    if (order) i = gid < a.norder ? (int64_t)order[gid].x : -1;
-    const bool have = i >= 0 && i < n;
-    if (have) {
-        x = pos[i];
-        y = pos[stride + i];
-        z = pos[2 * stride + i];
+    __syncthreads();
+    uint32_t fl = 0;
+    int32_t obj[kKeyRounds];
+    uint32_t ent[kKeyRounds], rank[kKeyRounds];
+#pragma unroll
+    for (int r = 0; r < kKeyRounds; ++r) {
+        const int64_t gid = ((int64_t)vb.id * kKeyRounds + r) * blockDim.x + tid;
+        // every load first (one round trip), then the clears and the new key
+        uint32_t old = 0xFFFFFFFFu, w = kNoWord;
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (gid < n_prev) {  // the previous build's cell of this object
+            if (a.clear_table) old = keys[gid];
+            if (gpos) w = gpos[gid];  // map word written by the previous build (indexed by its sorted slots)
+        }
+        // the object of this lane: gid, or with `order` the object of lane
+        // gid (objects in spatial order, so that the block's lanes share
+        // buckets; -1: no object)
+        int64_t i = gid < n ? gid : -1;
+        if (order) i = gid < a.norder ? (int64_t)order[gid].x : -1;
+        const bool have = i >= 0 && i < n;
+        if (have) {
+            x = pos[i];
+            y = pos[stride + i];
+            z = pos[2 * stride + i];
+        }
+        if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
+        if (w != kNoWord) bm.words[w] = 0;
+        obj[r] = have ? (int32_t)i : -1;
+        ent[r] = kKeyHash;
+        rank[r] = 0u;
+        if (have) {
+            const uint32_t k = cell_key(g, x, y, z, fl);
+            keys[i] = k;
+            const uint32_t b = k >> a.shift;
+            // the bucket's entry in the block's table (open addressing); its
+            // count before this object is the object's place among the
+            // block's objects of that bucket
+            uint32_t h = (b * 2654435761u) >> (32u - kKeyHashBits);
+            for (uint32_t probe = 0; probe < kKeyHash; ++probe) {
+                const uint32_t prev = atomicCAS(&s_bkt[h], kKeyEmpty, b);
+                if (prev == kKeyEmpty || prev == b) {
+                    ent[r] = h;
+                    break;
+                }
+                h = (h + 1u) & (kKeyHash - 1u);
+            }
+            // (a full table cannot occur: it has an entry per lane; the
+            // histogram is then counted into directly)
+            rank[r] = ent[r] < kKeyHash ? atomicAdd(&s_cnt[ent[r]], 1u) : atomicAdd(&a.bcount[b], 1u);
+        }
     }
-    if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
-    if (w != kNoWord) bm.words[w] = 0;
-    uint32_t fl = 0, b = 0xFFFFFFFFu;
-    if (have) {
-        const uint32_t k = cell_key(g, x, y, z, fl);
-        keys[i] = k;
-        b = k >> a.shift;
-    }
-    // The count before this object is its slot within the bucket (the
-    // scatter needs no atomics; slot order within a bucket is free, the
-    // ranking orders by key and index).  Consecutive lanes of one bucket
-    // (spatially ordered lanes mostly are) take their slots with one add by
-    // the run's first lane: slot = the run's base + the lane's place in it.
-    const uint32_t lane = __lane_id();
-    const uint32_t up = __shfl_up(b, 1u, 64);
-    const bool head = have && (lane == 0u || up != b);
-    const uint64_t heads = __ballot(head);
-    const uint64_t breaks = heads | __ballot(!have);
-    const uint64_t upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
-    const uint32_t lead = 63u - (uint32_t)__clzll((heads & upto) | 1ull);  // this lane's run head (have lanes)
-    uint32_t base = 0;
-    if (head) {
-        const uint64_t later = breaks & ~upto;  // the first break after this head ends its run
-        const uint32_t end = later ? (uint32_t)__ffsll((long long)later) - 1u : 64u;
-        base = atomicAdd(&a.bcount[b], end - lane);
-    }
-    base = __shfl(base, (int)lead, 64);
-    if (have) a.arrival[i] = base + (lane - lead);
+    __syncthreads();
+    // one histogram add per bucket of the block; the entry becomes the
+    // block's first slot in the bucket (slot order within a bucket is free:
+    // the ranking orders by key and index)
+    for (uint32_t e = tid; e < kKeyHash; e += blockDim.x)
+        if (s_cnt[e]) s_cnt[e] = atomicAdd(&a.bcount[s_bkt[e]], s_cnt[e]);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kKeyRounds; ++r)
+        if (obj[r] >= 0) a.arrival[obj[r]] = ent[r] < kKeyHash ? s_cnt[ent[r]] + rank[r] : rank[r];
     if (fl) atomicOr(a.flags, fl);
 }
 __global__ void __launch_bounds__(256) k_key(KeyArgs a) { key_body(a, hw_block()); }
@@ -943,11 +976,12 @@ int grid_launch_args(DeviceGrid& G, const float* pos, int64_t stride, int32_t n,
     const int32_t n_prev = G.cap;
     G.n = n;
     const int blocks = (std::max(std::max(n, n_prev), 1) + 255) / 256;
-    const int key_blocks = std::max(blocks, order ? (norder + 255) / 256 : 0);
+    const int key_lanes = std::max(std::max(std::max(n, n_prev), order ? norder : 0), 1);
+    const int key_blocks = (key_lanes + 256 * kKeyRounds - 1) / (256 * kKeyRounds);
     *L = GridLaunch{};
     L->key = KeyArgs{G.g, pos, stride, n, n_prev, G.shift, G.clear_table ? 1 : 0, G.keys, G.table, G.bm, G.gpos,
                      G.bcount, G.status, G.ntiles, G.ctrl, flags, G.arrival, order, norder};
-    L->key_blocks = (uint32_t)std::max(key_blocks, (G.ntiles + 255) / 256);
+    L->key_blocks = (uint32_t)key_blocks;  // (the tile status clear strides over every block)
     L->scan = ScanArgs{G.bcount, G.bstart, (int64_t)G.nb, G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold,
                        G.big_cap, flags, 1};
     L->scan_blocks = (uint32_t)G.ntiles;
