@@ -760,12 +760,28 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
         // to F in visit order: the wave runs the force code once per contact
         // of its busiest lane, instead of once per stream position at which
         // any lane has a contact.
-        auto slot_at = [&](uint32_t pp) {
-            uint32_t sl = 0u;
+        // Stream cursor: the current run holds positions up to rend[0]
+        // (exclusive), at slot = position + rd[0]; the runs after it follow
+        // in rd[1..], rend[1..].  Positions only grow, so a position past
+        // the current run's end shifts the next run in (an empty run is
+        // passed over the same way).
+        uint32_t rd[kCellBatch], rend[kCellBatch];
 #pragma unroll
-            for (int k = 0; k < kCellBatch; ++k)
-                if (pp >= cum[k] && pp < cum[k + 1]) sl = ent[k].x + (pp - cum[k]);
-            return sl;
+        for (int k = 0; k < kCellBatch; ++k) {
+            rd[k] = ent[k].x - cum[k];
+            rend[k] = cum[k + 1];
+        }
+        auto cursor_slot = [&](uint32_t pp) {
+#pragma unroll
+            for (int s = 0; s < kCellBatch - 1; ++s) {  // at most kCellBatch - 1 shifts (pp < total)
+                if (pp < rend[0]) break;
+#pragma unroll
+                for (int k = 0; k < kCellBatch - 1; ++k) {
+                    rd[k] = rd[k + 1];
+                    rend[k] = rend[k + 1];
+                }
+            }
+            return pp + rd[0];
         };
         uint32_t* hit_list = s_hit + threadIdx.x;
         for (uint32_t p0 = 0; p0 < total; p0 += kContactWindow) {
@@ -775,26 +791,9 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                 const uint32_t b0 = p0 + (uint32_t)(g * kContactBatch);
                 if (b0 >= total) break;
                 const int nval = (int)min((uint32_t)kContactBatch, total - b0);
-                // the run (cell) holding position b0: slot = position + delta
-                // up to its end; positions past it are mapped one by one
-                uint32_t delta = 0u, run_end = 0u;
-#pragma unroll
-                for (int k = 0; k < kCellBatch; ++k)
-                    if (b0 >= cum[k]) {
-                        delta = ent[k].x - cum[k];
-                        run_end = cum[k + 1];
-                    }
                 uint32_t slot[kContactBatch];
-                if (__any(b0 + (uint32_t)(nval - 1) >= run_end)) {
 #pragma unroll
-                    for (int t = 0; t < kContactBatch; ++t) {
-                        const uint32_t pp = b0 + (uint32_t)t;
-                        slot[t] = pp < run_end ? pp + delta : slot_at(pp);
-                    }
-                } else {
-#pragma unroll
-                    for (int t = 0; t < kContactBatch; ++t) slot[t] = b0 + (uint32_t)t + delta;
-                }
+                for (int t = 0; t < kContactBatch; ++t) slot[t] = t < nval ? cursor_slot(b0 + (uint32_t)t) : 0u;
                 float4 q[kContactBatch];
 #pragma unroll
                 for (int t = 0; t < kContactBatch; ++t)
