@@ -129,6 +129,45 @@ __device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVe
     return smul(coef, add(add(spring, damp), shear));
 }
 
+// resilient() for a contact in the range where the correctly rounded square
+// root and quotients need none of their range handling: d2 in [2^-60, 2^60]
+// and every |relPos| component >= 2^-96 (so no operand or quotient is zero,
+// tiny, huge or denormal, and normalize() cannot produce a NaN).  There the
+// compiler's sequences reduce to what is written here -- the hardware
+// square-root estimate moved to the correctly rounded root by the sign of
+// the two neighbours' residuals, and for each quotient n / s the refined
+// reciprocal of s, then two residual corrections -- so every value is the
+// one resilient() computes.  The reciprocal is refined once for the three
+// quotients.
+__device__ __forceinline__ f3 resilient_inrange(const PhysConst& pc, f3 relPos, f3 relVel, float d2, float radius,
+                                                float coef)
+{
+    const float s0 = __builtin_amdgcn_sqrtf(d2);
+    const float sdn = __uint_as_float(__float_as_uint(s0) - 1u), sup = __uint_as_float(__float_as_uint(s0) + 1u);
+    const float edn = fmaf(-sdn, s0, d2), eup = fmaf(-sup, s0, d2);
+    float s = edn <= 0.0f ? sdn : s0;
+    s = eup > 0.0f ? sup : s;
+    float r = __builtin_amdgcn_rcpf(s);
+    r = fmaf(fmaf(-s, r, 1.0f), r, r);
+    auto quo = [&](float n) {
+        const float q0 = n * r;
+        const float q1 = fmaf(fmaf(-s, q0, n), r, q0);
+        return fmaf(fmaf(-s, q1, n), r, q1);
+    };
+    const f3 dir = mk(quo(relPos.x), quo(relPos.y), quo(relPos.z));
+    f3 tang = sub(relVel, smul(dot(relVel, dir), dir));
+    f3 spring = smul(-pc.cs * (radius * 2.0f - s), dir);
+    f3 damp = smul(pc.cd, relVel);
+    f3 shear = smul(pc.csh, tang);
+    return smul(coef, add(add(spring, damp), shear));
+}
+// Whether a contact's operands lie in that range (d2 within it is checked
+// once per particle: contacts have d2 >= 1e-4 and d2 <= bound2).
+__device__ __forceinline__ bool contact_inrange(f3 relPos)
+{
+    return fminf(fminf(fabsf(relPos.x), fabsf(relPos.y)), fabsf(relPos.z)) >= 0x1p-96f;
+}
+
 // realCollisionDetection, src/simulation/vein_collisions.cu:11-45, with
 // edge1 = v1 - v0 and edge2 = v2 - v0 precomputed (the identical float
 // subtractions, done once per frame by k_vein_prep).  Before the correctly
@@ -682,6 +721,9 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
     // the oracle does).
     const float rb = r1 + pc.rmax;
     const float bound2 = rb * rb;
+    // every contact has d2 in [1e-4, bound2]: within the range of
+    // resilient_inrange when bound2 is
+    const bool d2_inrange = bound2 <= 0x1p60f;
     // With a grid built from the current positions (every fused frame),
     // neighbour cells whose box lies farther than rb from x1 hold no such
     // partner and can be dropped from the mask.  The gaps to the cell faces
@@ -841,7 +883,8 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                 const f3 rel = sub(x1, mk(q.x, q.y, q.z));
                 const float d2 = dot(rel, rel);
                 const f3 rv = sub(v1, mk(w.x, w.y, w.z));
-                F = add(F, resilient(pc, rel, rv, d2, r1, 0.5f, fl));
+                F = add(F, d2_inrange && contact_inrange(rel) ? resilient_inrange(pc, rel, rv, d2, r1, 0.5f)
+                                                              : resilient(pc, rel, rv, d2, r1, 0.5f, fl));
             }
             ncontact += nh;
         }

@@ -308,6 +308,34 @@ def test_contact_modes_match_oracle(bcs, orc, mode):
         sim.set_contact_mode(5)
 
 
+@pytest.mark.parametrize("mode", ["cells", "slots"])
+def test_contacts_on_a_lattice_match_oracle(bcs, orc, mode):
+    """Positions snapped to a lattice of 1/4: many contact pairs then differ
+    in one or two coordinates by exactly 0, which the contact force routes
+    through the fully range-handled square root and quotients, while the
+    others take the short in-range sequences.  Two frames against the oracle
+    (after them the particles have left the lattice), with contacts checked
+    to occur at both kinds of pairs."""
+    sc = _scene(bcs, "mixed")
+    p = np.round(np.asarray(sc.positions, np.float32) * 4.0) / 4.0
+    n = p.size // 3
+    sim = bcs.Simulation.from_scene(sc)
+    sim.set_contact_mode(bcs.CONTACT_CELLS if mode == "cells" else bcs.CONTACT_SLOTS)
+    sim.set_particles(p, sc.velocities, sc.forces)
+    o = orc.Oracle(sc.physics, sc.types, sc.mesh)
+    o.set_particles(p, sc.velocities, sc.forces)
+    sim.calculate_grid()
+    counts, pairs = sim.get_contacts()
+    P = p.reshape(3, n)
+    owner = np.repeat(np.arange(n), counts)
+    zero_axis = np.any(P[:, owner] == P[:, pairs], axis=0)
+    assert zero_axis.any() and (~zero_axis).any(), "lattice scene lacks one kind of contact"
+    sim.step(2)
+    o.step(2)
+    for x, y, name in zip(sim.get_particles(), o.get_particles(), ("pos", "vel", "force")):
+        assert_bits_equal(x, y, f"lattice positions, contacts by {mode}: {name}")
+
+
 @pytest.mark.parametrize("shift", [4, 8, 11])
 def test_bucket_sizes_keep_parity(bcs, orc, shift):
     """40 graph frames with buckets of 2^4 cells (occupancy map from its own
