@@ -129,24 +129,30 @@ __device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVe
     return smul(coef, add(add(spring, damp), shear));
 }
 
-// resilient() for a contact in the range where the correctly rounded square
-// root and quotients need none of their range handling: d2 in [2^-60, 2^60]
-// and every |relPos| component >= 2^-96 (so no operand or quotient is zero,
-// tiny, huge or denormal, and normalize() cannot produce a NaN).  There the
-// compiler's sequences reduce to what is written here -- the hardware
-// square-root estimate moved to the correctly rounded root by the sign of
-// the two neighbours' residuals, and for each quotient n / s the refined
-// reciprocal of s, then two residual corrections -- so every value is the
-// one resilient() computes.  The reciprocal is refined once for the three
+// The correctly rounded square root of d2 and quotients v / s in the range
+// where they need none of their range handling: d2 in [2^-60, 2^60] and
+// every |v| component >= 2^-96, so that no operand or quotient is zero, tiny,
+// huge or denormal (and a normalize() of v cannot produce a NaN).  There the
+// compiler's sequences for sqrtf() and '/' reduce to what is written here --
+// the hardware square-root estimate moved to the correctly rounded root by
+// the signs of its two neighbours' residuals, and for n / s the refined
+// reciprocal of s, then two residual corrections -- so every value equals
+// the one sqrtf() and '/' give.  The reciprocal is refined once for the three
 // quotients.
-__device__ __forceinline__ f3 resilient_inrange(const PhysConst& pc, f3 relPos, f3 relVel, float d2, float radius,
-                                                float coef)
+__device__ __forceinline__ bool inrange_ok(f3 v, float d2)
+{
+    return d2 >= 0x1p-60f && d2 <= 0x1p60f && fminf(fminf(fabsf(v.x), fabsf(v.y)), fabsf(v.z)) >= 0x1p-96f;
+}
+__device__ __forceinline__ float root_inrange(float d2)
 {
     const float s0 = __builtin_amdgcn_sqrtf(d2);
     const float sdn = __uint_as_float(__float_as_uint(s0) - 1u), sup = __uint_as_float(__float_as_uint(s0) + 1u);
     const float edn = fmaf(-sdn, s0, d2), eup = fmaf(-sup, s0, d2);
-    float s = edn <= 0.0f ? sdn : s0;
-    s = eup > 0.0f ? sup : s;
+    const float s = edn <= 0.0f ? sdn : s0;
+    return eup > 0.0f ? sup : s;
+}
+__device__ __forceinline__ f3 quot_inrange(f3 v, float s)
+{
     float r = __builtin_amdgcn_rcpf(s);
     r = fmaf(fmaf(-s, r, 1.0f), r, r);
     auto quo = [&](float n) {
@@ -154,18 +160,21 @@ __device__ __forceinline__ f3 resilient_inrange(const PhysConst& pc, f3 relPos, 
         const float q1 = fmaf(fmaf(-s, q0, n), r, q0);
         return fmaf(fmaf(-s, q1, n), r, q1);
     };
-    const f3 dir = mk(quo(relPos.x), quo(relPos.y), quo(relPos.z));
+    return mk(quo(v.x), quo(v.y), quo(v.z));
+}
+
+// resilient() for a contact within that range (inrange_ok(relPos, d2)): the
+// same values.
+__device__ __forceinline__ f3 resilient_inrange(const PhysConst& pc, f3 relPos, f3 relVel, float d2, float radius,
+                                                float coef)
+{
+    const float s = root_inrange(d2);
+    const f3 dir = quot_inrange(relPos, s);
     f3 tang = sub(relVel, smul(dot(relVel, dir), dir));
     f3 spring = smul(-pc.cs * (radius * 2.0f - s), dir);
     f3 damp = smul(pc.cd, relVel);
     f3 shear = smul(pc.csh, tang);
     return smul(coef, add(add(spring, damp), shear));
-}
-// Whether a contact's operands lie in that range (d2 within it is checked
-// once per particle: contacts have d2 >= 1e-4 and d2 <= bound2).
-__device__ __forceinline__ bool contact_inrange(f3 relPos)
-{
-    return fminf(fminf(fabsf(relPos.x), fabsf(relPos.y)), fabsf(relPos.z)) >= 0x1p-96f;
 }
 
 // realCollisionDetection, src/simulation/vein_collisions.cu:11-45, with
@@ -721,9 +730,6 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
     // the oracle does).
     const float rb = r1 + pc.rmax;
     const float bound2 = rb * rb;
-    // every contact has d2 in [1e-4, bound2]: within the range of
-    // resilient_inrange when bound2 is
-    const bool d2_inrange = bound2 <= 0x1p60f;
     // With a grid built from the current positions (every fused frame),
     // neighbour cells whose box lies farther than rb from x1 hold no such
     // partner and can be dropped from the mask.  The gaps to the cell faces
@@ -883,8 +889,8 @@ __device__ __forceinline__ uint32_t contact_sum(const StepArgs& A, int32_t i, in
                 const f3 rel = sub(x1, mk(q.x, q.y, q.z));
                 const float d2 = dot(rel, rel);
                 const f3 rv = sub(v1, mk(w.x, w.y, w.z));
-                F = add(F, d2_inrange && contact_inrange(rel) ? resilient_inrange(pc, rel, rv, d2, r1, 0.5f)
-                                                              : resilient(pc, rel, rv, d2, r1, 0.5f, fl));
+                F = add(F, inrange_ok(rel, d2) ? resilient_inrange(pc, rel, rv, d2, r1, 0.5f)
+                                               : resilient(pc, rel, rv, d2, r1, 0.5f, fl));
             }
             ncontact += nh;
         }
@@ -1013,13 +1019,22 @@ __device__ __forceinline__ void cells_body(const StepArgs& A, VBlock vb)
             // rounding here), so one square root and three divisions serve all
             // three; a NaN quotient is replaced by (0,0,0) for both, as in
             // normalize().
-            const float sl = sqrtf(dot(dP, dP));
-            f3 nrm = sdiv(dP, sl);
-            f3 shift = smul(-1.0f, nrm);
-            if (__builtin_isnan(nrm.x) || __builtin_isnan(nrm.y) || __builtin_isnan(nrm.z)) {
-                fl |= BCS_FLAG_NAN_NORMALIZE;
-                nrm = mk(0.f, 0.f, 0.f);
-                shift = nrm;
+            const float d2s = dot(dP, dP);
+            float sl;
+            f3 nrm, shift;
+            if (inrange_ok(dP, d2s)) {
+                sl = root_inrange(d2s);
+                nrm = quot_inrange(dP, sl);
+                shift = smul(-1.0f, nrm);
+            } else {
+                sl = sqrtf(d2s);
+                nrm = sdiv(dP, sl);
+                shift = smul(-1.0f, nrm);
+                if (__builtin_isnan(nrm.x) || __builtin_isnan(nrm.y) || __builtin_isnan(nrm.z)) {
+                    fl |= BCS_FLAG_NAN_NORMALIZE;
+                    nrm = mk(0.f, 0.f, 0.f);
+                    shift = nrm;
+                }
             }
             if (pc.rk4) {
                 const f3 k1x = smul(pc.dt, dv), k1v = smul(pc.dt, F0);
