@@ -43,15 +43,17 @@ constexpr uint64_t kStatusAgg = 1ull << 62;
 constexpr uint64_t kStatusInc = 2ull << 62;
 constexpr uint64_t kSpinLimit = 1ull << 22;
 
-// Lanes of the key kernel: kKeyRounds rounds of one lane per thread per
-// block, so that a block counts the objects of kKeyRounds * 256 lanes (with a
+// Lanes of the key kernel: up to kKeyRounds rounds of one lane per thread
+// per block, so that a block counts the objects of rounds * 256 lanes (with a
 // lane order: neighbouring blood cells) into its buckets before it adds to the
-// histogram, once per distinct bucket.
+// histogram, once per distinct bucket.  Builds of fewer than kKeyRoundsFrom
+// lanes take one round (enough blocks to fill the GPU).
 constexpr int kKeyRounds = 4;
+constexpr int64_t kKeyRoundsFrom = 1 << 20;
 constexpr uint32_t kKeyHashBits = 10;
 constexpr uint32_t kKeyHash = 1u << kKeyHashBits;  // LDS bucket table (room for every lane's bucket)
 constexpr uint32_t kKeyEmpty = 0xFFFFFFFFu;
-static_assert(kKeyHash >= 256u * kKeyRounds, "the bucket table must hold a bucket per lane");
+static_assert(kKeyHash >= 256u * kKeyRounds && (1u << 8) >= 256u, "the bucket table must hold a bucket per lane");
 
 __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
 {
@@ -76,7 +78,9 @@ __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
         a.ctrl[1] = 0u;  // big-bucket count
         a.ctrl[2] = 0u;  // occupied cells (run list)
     }
-    for (uint32_t e = tid; e < kKeyHash; e += blockDim.x) {
+    // the table in use: an entry per lane of the block's rounds
+    const uint32_t hbits = a.rounds > 1 ? kKeyHashBits : 8u, hsize = 1u << hbits;
+    for (uint32_t e = tid; e < hsize; e += blockDim.x) {
         s_bkt[e] = kKeyEmpty;
         s_cnt[e] = 0u;
     }
@@ -86,7 +90,14 @@ __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
     uint32_t ent[kKeyRounds], rank[kKeyRounds];
 #pragma unroll
     for (int r = 0; r < kKeyRounds; ++r) {
-        const int64_t gid = ((int64_t)vb.id * kKeyRounds + r) * blockDim.x + tid;
+        obj[r] = -1;
+        ent[r] = kKeyHash;
+        rank[r] = 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kKeyRounds; ++r) {
+        if (r >= a.rounds) break;  // (uniform)
+        const int64_t gid = ((int64_t)vb.id * a.rounds + r) * blockDim.x + tid;
         // every load first (one round trip), then the clears and the new key
         uint32_t old = 0xFFFFFFFFu, w = kNoWord;
         float x = 0.f, y = 0.f, z = 0.f;
@@ -108,8 +119,6 @@ __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
         if ((uint64_t)old < g.cell_count) table[old] = make_uint2(0u, 0u);
         if (w != kNoWord) bm.words[w] = 0;
         obj[r] = have ? (int32_t)i : -1;
-        ent[r] = kKeyHash;
-        rank[r] = 0u;
         if (have) {
             const uint32_t k = cell_key(g, x, y, z, fl);
             keys[i] = k;
@@ -117,14 +126,14 @@ __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
             // the bucket's entry in the block's table (open addressing); its
             // count before this object is the object's place among the
             // block's objects of that bucket
-            uint32_t h = (b * 2654435761u) >> (32u - kKeyHashBits);
-            for (uint32_t probe = 0; probe < kKeyHash; ++probe) {
+            uint32_t h = (b * 2654435761u) >> (32u - hbits);
+            for (uint32_t probe = 0; probe < hsize; ++probe) {
                 const uint32_t prev = atomicCAS(&s_bkt[h], kKeyEmpty, b);
                 if (prev == kKeyEmpty || prev == b) {
                     ent[r] = h;
                     break;
                 }
-                h = (h + 1u) & (kKeyHash - 1u);
+                h = (h + 1u) & (hsize - 1u);
             }
             // (a full table cannot occur: it has an entry per lane; the
             // histogram is then counted into directly)
@@ -135,7 +144,7 @@ __device__ __forceinline__ void key_body(const KeyArgs& a, VBlock vb)
     // one histogram add per bucket of the block; the entry becomes the
     // block's first slot in the bucket (slot order within a bucket is free:
     // the ranking orders by key and index)
-    for (uint32_t e = tid; e < kKeyHash; e += blockDim.x)
+    for (uint32_t e = tid; e < hsize; e += blockDim.x)
         if (s_cnt[e]) s_cnt[e] = atomicAdd(&a.bcount[s_bkt[e]], s_cnt[e]);
     __syncthreads();
 #pragma unroll
@@ -977,10 +986,11 @@ int grid_launch_args(DeviceGrid& G, const float* pos, int64_t stride, int32_t n,
     G.n = n;
     const int blocks = (std::max(std::max(n, n_prev), 1) + 255) / 256;
     const int key_lanes = std::max(std::max(std::max(n, n_prev), order ? norder : 0), 1);
-    const int key_blocks = (key_lanes + 256 * kKeyRounds - 1) / (256 * kKeyRounds);
+    const int key_rounds = key_lanes >= kKeyRoundsFrom ? kKeyRounds : 1;
+    const int key_blocks = (key_lanes + 256 * key_rounds - 1) / (256 * key_rounds);
     *L = GridLaunch{};
     L->key = KeyArgs{G.g, pos, stride, n, n_prev, G.shift, G.clear_table ? 1 : 0, G.keys, G.table, G.bm, G.gpos,
-                     G.bcount, G.status, G.ntiles, G.ctrl, flags, G.arrival, order, norder};
+                     G.bcount, G.status, G.ntiles, G.ctrl, flags, G.arrival, order, norder, key_rounds};
     L->key_blocks = (uint32_t)key_blocks;  // (the tile status clear strides over every block)
     L->scan = ScanArgs{G.bcount, G.bstart, (int64_t)G.nb, G.status, G.ntiles, G.ctrl, G.biglist, kRankThreshold,
                        G.big_cap, flags, 1};

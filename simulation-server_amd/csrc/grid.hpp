@@ -108,6 +108,7 @@ struct KeyArgs {
     uint32_t* arrival;
     const int2* order;
     int32_t norder;
+    int32_t rounds;  // lane rounds per block (1..4)
 };
 struct ScanArgs {
     uint32_t* count;
