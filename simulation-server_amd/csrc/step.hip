@@ -67,22 +67,22 @@ __device__ __forceinline__ int type_of_index(const TypeTable& tt, int32_t s)
     return tt.nt - 1;
 }
 
-// addResilientForceOnCollision, src/simulation/physics.cuh:133-145 (increment).
 // Chunk of this block.  Workgroups are dispatched round-robin over the 8
 // XCDs, so block b runs on XCD b % 8.  The blocks of the 8 * ceil(nchunks / 8)
-// grid are cut into super-groups of 8 * kXcdRun; in each, XCD x takes a run of
-// kXcdRun consecutive chunks (the last, partial super-group splits the same
-// way with shorter runs), a bijection onto [0, grid).  With the chunks in a
+// grid are cut into super-groups of 8 * RUN; in each, XCD x takes a run of
+// RUN consecutive chunks (the last, partial super-group splits the same way
+// with shorter runs), a bijection onto [0, grid).  With the chunks in a
 // spatial (Morton) order a run is one compact region, whose grid-table lines
 // stay in that XCD's L2, while a crowded region of the vessel is shared out
 // over all XCDs.  Blocks past the last chunk return -1.
+template <uint32_t RUN = kXcdRun>
 __device__ __forceinline__ int chunk_of_block(int32_t nchunks, uint32_t block)
 {
-    constexpr uint32_t S = 8u * kXcdRun;
+    constexpr uint32_t S = 8u * RUN;
     const uint32_t nb = 8u * (((uint32_t)nchunks + 7u) >> 3);
     const uint32_t sg = block / S, w = block % S;
     const uint32_t x = w & 7u, q = w >> 3;
-    const uint32_t run = (sg + 1u) * S <= nb ? kXcdRun : (nb - sg * S) >> 3;
+    const uint32_t run = (sg + 1u) * S <= nb ? RUN : (nb - sg * S) >> 3;
     const uint32_t c = sg * S + x * run + q;
     return c < (uint32_t)nchunks ? (int)c : -1;
 }
@@ -118,6 +118,7 @@ __device__ __forceinline__ f3 position_step(const PhysConst& pc, f3 v0, f3 v)
     return smul(0.5f * pc.dt, add(v, v0));
 }
 
+// addResilientForceOnCollision, src/simulation/physics.cuh:133-145 (increment).
 __device__ __forceinline__ f3 resilient(const PhysConst& pc, f3 relPos, f3 relVel, float d2, float radius, float coef,
                                         uint32_t& fl)
 {
@@ -1147,7 +1148,10 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
     __shared__ uint32_t s_hit[kContactWindow * kBlock];  // per lane: the slots of its contacts in a window
     const TypeTable& tt = A.tt;
     const int l = threadIdx.x;
-    const int tile = chunk_of_block(A.ntiles, vb.id);
+    // runs of kXcdRunTiles tiles per XCD: a z-plane of the vessel spans tens
+    // of tiles, so a run holds a few neighbouring planes, whose records the
+    // z +- 1 neighbour cells read again from the same L2
+    const int tile = chunk_of_block<kXcdRunTiles>(A.ntiles, vb.id);
     if (tile < 0) return;
     const bool bs_lds = tt.nm <= kBsLds;
     if (bs_lds)

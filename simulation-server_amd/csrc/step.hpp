@@ -13,6 +13,7 @@ constexpr int kContactWindow = 16; // stream positions tested before their conta
 constexpr int kSurrStatSlots = 256, kSurrStatStride = 16;  // surrogate tallies: 256 slots, 64 B apart
 constexpr int kCellBatch = 6;      // neighbour-cell table entries loaded together by the contact search
 constexpr uint32_t kXcdRun = 16;   // consecutive chunks per XCD run in the block -> chunk mapping
+constexpr uint32_t kXcdRunTiles = 256;  // the same for the sorted-slot tiles of the contact search
 constexpr int kSpringLds = 1024;   // springs of one cell type held in LDS by k_cells (more: read globally)
 constexpr int kOrderBits = 15;      // Morton bits of the cell processing order
 constexpr int kReorderPeriod = 16;  // frames between updates of the order
