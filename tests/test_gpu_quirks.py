@@ -151,6 +151,40 @@ def test_mixed_derived_radii_40_frames(bcs, orc):
     assert (sim.error_flags() & bcs.FLAG_RADIUS_CLAMPED) == (o.flags() & bcs.FLAG_RADIUS_CLAMPED)
 
 
+@pytest.mark.parametrize("mode", ["cells", "slots"])
+def test_one_radius_with_clamped_partner_indices(bcs, orc, mode):
+    """One contact radius (0.5) for every model vertex, but the icosahedra
+    first: a disc's partner among them selects the model index
+    mstart - ((pstart - j) mod 42) of the discs' block, which falls below 0
+    for many, so the radius lookup must still run and clamp (the one-radius
+    shortcut of the contact search applies only when no index can clamp).
+    Contact pairs, 40 frames and the clamp flag against the oracle."""
+    p = bcs.Scene.params(0)
+    p.n_cells = 2
+    p.cells[0] = bcs.SceneCell(bcs.CELL_ICOSAHEDRON, 90, 2.0, 1.0, 0.5)
+    p.cells[1] = bcs.SceneCell(bcs.CELL_DISC42, 60, 2.0, 0.35, 0.5)
+    sc = bcs.Scene(params=p)
+    q = sc.positions.reshape(3, -1).copy()
+    q[1] = q[1] * F32(0.3) - F32(60)
+    sc.positions = q.reshape(-1)
+    o = orc.Oracle(sc.physics, sc.types, sc.mesh)
+    assert len(np.unique(o.tables()[0])) == 1, "one contact radius expected"
+    o.set_particles(sc.positions, sc.velocities, sc.forces)
+    o.calculate_grid(0)
+    oc, op = o.contacts()
+    sim = bcs.Simulation.from_scene(sc)
+    sim.set_contact_mode(bcs.CONTACT_CELLS if mode == "cells" else bcs.CONTACT_SLOTS)
+    sim.calculate_grid(0)
+    gc, gp = sim.get_contacts()
+    assert_bits_equal(gc, oc, "one radius, clamped indices: contact counts")
+    assert_bits_equal(gp, op, "one radius, clamped indices: contact pairs")
+    sim.step(40)
+    o.step(40)
+    _compare(sim, o, "one radius, clamped indices, 40 frames", rel_bar=False)
+    assert o.flags() & bcs.FLAG_RADIUS_CLAMPED, "the scene should clamp partner indices"
+    assert (sim.error_flags() & bcs.FLAG_RADIUS_CLAMPED) == (o.flags() & bcs.FLAG_RADIUS_CLAMPED)
+
+
 @pytest.mark.parametrize("switch", ["enable_reaction_force", "enable_big_cells_brake", "use_blood_flow"])
 def test_switch_off_40_frames(bcs, orc, switch):
     sc = mixed_derived_scene(bcs, seed=2)
