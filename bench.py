@@ -71,7 +71,7 @@ def kernel_bytes(name, np_, n_occ, nc, nv, nt, n_items=0, map_in_rank=False):
       k_occ        sorted key read 4, map-word slot written 4; count read 8
                    and 2-bit code written (<= 2 bytes) per occupied cell
                                                                    8 / particle + 10 / cell
-                   (map_in_rank: with buckets of 2^8 cells or more k_rank
+                   (map_in_rank: with buckets of 2^7 cells or more k_rank
                    writes the map instead -- map-word slot 4 / particle and
                    code 2 / cell, its keys and counts already in registers)
       k_raycand    x, v read 24, first-hit key written 4; work items written

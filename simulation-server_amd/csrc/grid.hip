@@ -18,9 +18,9 @@
 //              (8-bit digits of (key - bucket base) << index bits | index,
 //              stable passes through two global scratch arrays)
 // The optional 2-bit brick occupancy map (one word per 8-cell x-group):
-//   buckets of 2^8 cells or more: written by k_rank (the first run of every
+//   buckets of 2^7 cells or more: written by k_rank (the first run of every
 //              group writes the group's word; a group straddling two buckets
-//              ORs its codes in, rare at this bucket size)
+//              ORs its codes in, rare at these bucket sizes)
 //   smaller buckets (many straddling groups): k_occ from the sorted keys
 // Cell table: dense {start, count} per cell id (8 bytes, one load per lookup),
 // written for occupied cells only.  With clear_table (the triangle grid) k_key

@@ -216,9 +216,9 @@ void grid_build_batch(const GridBatch& b, hipStream_t st);
 void scan_batch(const ScanArgs* a, const uint32_t* start, int32_t nsim, uint32_t grid, uint32_t run, hipStream_t st);
 constexpr int kGridKernels = 5;
 // The occupancy map is written by k_rank when buckets span at least
-// 2^kRankMapShift cells (few 8-cell groups straddle two buckets), else by
-// k_occ after it.
-constexpr int kRankMapShift = 8;
+// 2^kRankMapShift cells (at most one 8-cell group in 16 straddles two
+// buckets), else by k_occ after it.
+constexpr int kRankMapShift = 7;
 inline bool grid_map_in_rank(const DeviceGrid& G) { return G.gpos && G.shift >= kRankMapShift; }
 
 // Bucket size from a host sample of the objects (SoA, stride apart), and the
