@@ -266,15 +266,17 @@ __global__ void __launch_bounds__(256) k_gather_rows(const uint16_t* __restrict_
 // ---- fused inference: all four layers in one kernel ------------------------
 // A block takes kFM rows (cells).  Their activations stay in LDS (bf16, row
 // pitch kHPitch) from the input to the last hidden layer; each weight matrix
-// streams through LDS 32 columns at a time (double-buffered), fetched into
-// registers two chunks ahead.  The products are formed transposed, W x H^T,
-// so that each lane ends with 4 consecutive output features of one cell: the
-// bias, GELU and bf16 rounding of a hidden layer go back to LDS as one 8-byte
-// store per 4 values.  The 4 waves split a layer's output 2 x 2: 64 cells x
-// 128 features per wave for the hidden layers (4 x 8 MFMA tiles), 64 x 64
-// for the 128-feature output layer.  Only the input rows and the output
-// rows touch HBM.
-constexpr int kFM = 128;
+// streams through one LDS buffer 32 columns at a time, fetched into registers
+// two chunks ahead.  With 62 KB of LDS and at most 128 registers per lane two
+// blocks share a CU, so that one block's waits (its barriers, the weight
+// stores) overlap the other's MFMA chain.  The products are formed
+// transposed, W x H^T, so that each lane ends with 4 consecutive output
+// features of one cell: the bias, GELU and bf16 rounding of a hidden layer go
+// back to LDS as one 8-byte store per 4 values.  The 4 waves split a layer's
+// output features 4 ways: 64 cells x 64 features per wave for the hidden
+// layers (4 x 4 MFMA tiles), 64 x 32 for the 128-feature output layer.  Only
+// the input rows and the output rows touch HBM.
+constexpr int kFM = 64;
 constexpr int kHPitch = kSurrInPad + 8;  // 328 bf16: rows 16 B apart modulo the LDS banks
 constexpr int kWPitch = kBK + 8;         // 40 bf16
 constexpr int kWBuf = kSurrHidden * kWPitch;
@@ -303,14 +305,14 @@ __device__ __forceinline__ void w_chunk_store(uint16_t* buf, int rows, int tid, 
     }
 }
 
-// acc[i][j] = W[wn + 16 j ..][0 .. K) x H[wm + 16 i ..][0 .. K)^T: lane l holds
-// features wn + 16 j + 4 (l / 16) + r of cell wm + 16 i + l % 16.  W has
-// 32 NT rows (256 or 128).  Ends with a barrier: every wave is done with H.
-template <int NT, int K>
+// acc[i][j] = W[wn + 16 j ..][0 .. K) x H[16 i ..][0 .. K)^T: lane l holds
+// features wn + 16 j + 4 (l / 16) + r of cell 16 i + l % 16.  W has ROWS
+// rows.  Ends with a barrier: every wave is done with H.
+template <int NT, int ROWS, int K>
 __device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, const uint16_t* H, uint16_t* wbuf,
-                                          f32x4 (&acc)[4][NT], int wm, int wn, int lane, int tid)
+                                          f32x4 (&acc)[4][NT], int wn, int lane, int tid)
 {
-    constexpr int rows = 32 * NT, nks = K / kBK;
+    constexpr int nks = K / kBK;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -318,37 +320,38 @@ __device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, const 
     uint4 r[kAhead][4];
 #pragma unroll
     for (int a = 0; a < kAhead; ++a)
-        if (a < nks) w_chunk_load(W, K, rows, a * kBK, tid, r[a]);
-    w_chunk_store(wbuf, rows, tid, r[0]);
+        if (a < nks) w_chunk_load(W, K, ROWS, a * kBK, tid, r[a]);
+    w_chunk_store(wbuf, ROWS, tid, r[0]);
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < nks; ++ks) {
         // chunk ks + kAhead into the registers of chunk ks (in LDS since the
         // last step); it is stored kAhead - 1 steps from now
-        if (ks + kAhead < nks) w_chunk_load(W, K, rows, (ks + kAhead) * kBK, tid, r[ks % kAhead]);
-        const uint16_t* cur = wbuf + (ks & 1) * kWBuf;
+        if (ks + kAhead < nks) w_chunk_load(W, K, ROWS, (ks + kAhead) * kBK, tid, r[ks % kAhead]);
         bf16x8 a[4], b[NT];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            a[i] = *reinterpret_cast<const bf16x8*>(H + (wm + 16 * i + (lane & 15)) * kHPitch + ks * kBK +
-                                                    8 * (lane >> 4));
+            a[i] = *reinterpret_cast<const bf16x8*>(H + (16 * i + (lane & 15)) * kHPitch + ks * kBK + 8 * (lane >> 4));
 #pragma unroll
         for (int j = 0; j < NT; ++j)
-            b[j] = *reinterpret_cast<const bf16x8*>(cur + (wn + 16 * j + (lane & 15)) * kWPitch + 8 * (lane >> 4));
+            b[j] = *reinterpret_cast<const bf16x8*>(wbuf + (wn + 16 * j + (lane & 15)) * kWPitch + 8 * (lane >> 4));
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
-        if (ks + 1 < nks) w_chunk_store(wbuf + ((ks + 1) & 1) * kWBuf, rows, tid, r[(ks + 1) % kAhead]);
-        __syncthreads();
+        __syncthreads();  // every wave is done with this chunk (and, at the end, with H)
+        if (ks + 1 < nks) {
+            w_chunk_store(wbuf, ROWS, tid, r[(ks + 1) % kAhead]);
+            __syncthreads();
+        }
     }
 }
 
-__global__ void __launch_bounds__(256) k_mlp_infer(SurrNet net, const uint16_t* __restrict__ X, int32_t n,
-                                                   float* __restrict__ out)
+__global__ void __launch_bounds__(256, 2) k_mlp_infer(SurrNet net, const uint16_t* __restrict__ X, int32_t n,
+                                                      float* __restrict__ out)
 {
     __shared__ __attribute__((aligned(16))) uint16_t H[kFM * kHPitch];
-    __shared__ __attribute__((aligned(16))) uint16_t wbuf[2 * kWBuf];
+    __shared__ __attribute__((aligned(16))) uint16_t wbuf[kWBuf];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t m0 = (int64_t)blockIdx.x * kFM;
     // input rows (zeros past the last row)
@@ -359,18 +362,17 @@ __global__ void __launch_bounds__(256) k_mlp_infer(SurrNet net, const uint16_t* 
         *reinterpret_cast<uint4*>(H + row * kHPitch + 8 * part) = v;
     }
     __syncthreads();
-    const int wm = (wave >> 1) * 64;
     const int cm = lane & 15, fq = 4 * (lane >> 4);  // the lane's cell within a tile, first feature of its 4
     {
-        f32x4 acc[4][8];
-        const int wn = (wave & 1) * 128;
+        f32x4 acc[4][4];
+        const int wn = wave * 64;
 #pragma unroll 1
         for (int l = 0; l < 3; ++l) {
-            if (l == 0) mlp_layer<8, kSurrInPad>(net.W[0], H, wbuf, acc, wm, wn, lane, tid);
-            else mlp_layer<8, kSurrHidden>(net.W[l], H, wbuf, acc, wm, wn, lane, tid);
+            if (l == 0) mlp_layer<4, kSurrHidden, kSurrInPad>(net.W[0], H, wbuf, acc, wn, lane, tid);
+            else mlp_layer<4, kSurrHidden, kSurrHidden>(net.W[l], H, wbuf, acc, wn, lane, tid);
             // bias, GELU, bf16: the next layer's input, in place
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < 4; ++j) {
                 const int nn = wn + 16 * j + fq;
                 const float4 bn = *reinterpret_cast<const float4*>(net.b[l] + nn);
 #pragma unroll
@@ -378,22 +380,22 @@ __global__ void __launch_bounds__(256) k_mlp_infer(SurrNet net, const uint16_t* 
                     const f32x4 v = acc[i][j];
                     const uint32_t lo = (uint32_t)bf16_rn(gelu(v[0] + bn.x)) | ((uint32_t)bf16_rn(gelu(v[1] + bn.y)) << 16);
                     const uint32_t hi = (uint32_t)bf16_rn(gelu(v[2] + bn.z)) | ((uint32_t)bf16_rn(gelu(v[3] + bn.w)) << 16);
-                    *reinterpret_cast<uint2*>(H + (wm + 16 * i + cm) * kHPitch + nn) = make_uint2(lo, hi);
+                    *reinterpret_cast<uint2*>(H + (16 * i + cm) * kHPitch + nn) = make_uint2(lo, hi);
                 }
             }
             __syncthreads();
         }
     }
-    f32x4 acc[4][4];
-    const int wn = (wave & 1) * 64;
-    mlp_layer<4, kSurrHidden>(net.W[3], H, wbuf, acc, wm, wn, lane, tid);
+    f32x4 acc[4][2];
+    const int wn = wave * 32;
+    mlp_layer<2, kSurrOutPad, kSurrHidden>(net.W[3], H, wbuf, acc, wn, lane, tid);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < 2; ++j) {
         const int nn = wn + 16 * j + fq;
         const float4 bn = *reinterpret_cast<const float4*>(net.b[3] + nn);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int64_t m = m0 + wm + 16 * i + cm;
+            const int64_t m = m0 + 16 * i + cm;
             const f32x4 v = acc[i][j];
             if (m < n)
                 *reinterpret_cast<float4*>(out + m * kSurrOutPad + nn) =
