@@ -1144,8 +1144,10 @@ __global__ void __launch_bounds__(kBlock) k_springs(StepArgs A)
 template <bool SURR>
 __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
 {
-    __shared__ float s_bs[kBsLds];
-    __shared__ uint32_t s_hit[kContactWindow * kBlock];  // per lane: the slots of its contacts in a window
+    // per lane: the slots of its contacts in a window (the contact radii are
+    // read from the global table: the one-radius shortcut rarely needs them,
+    // and the kernel then keeps no LDS table and no barrier for it)
+    __shared__ uint32_t s_hit[kContactWindow * kBlock];
     const TypeTable& tt = A.tt;
     const int l = threadIdx.x;
     // runs of kXcdRunTiles tiles per XCD: a z-plane of the vessel spans tens
@@ -1153,10 +1155,7 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
     // z +- 1 neighbour cells read again from the same L2
     const int tile = chunk_of_block<kXcdRunTiles>(A.ntiles, vb.id);
     if (tile < 0) return;
-    const bool bs_lds = tt.nm <= kBsLds;
-    if (bs_lds)
-        for (int t = l; t < tt.nm; t += kBlock) s_bs[t] = A.bs[t];
-    auto radius = [&](int32_t m) -> float { return bs_lds ? s_bs[m] : A.bs[m]; };
+    auto radius = [&](int32_t m) -> float { return A.bs[m]; };
     const int64_t np = tt.np;
     const int64_t u = (int64_t)tile * kBlock + l;
     const bool act = u < np;
@@ -1181,7 +1180,6 @@ __device__ __forceinline__ void contacts_body(const StepArgs& A, VBlock vb)
         axis_range_i(zId, g.cnt[2], nr.zl, nr.zh);
         mask = neighbour_mask(A.p_map, g, key_of, nr, cX, cXY, true);
     }
-    __syncthreads();  // s_bs
     // every lane of the wave takes part (the pruning exchanges masks between
     // lanes); a lane past the end has an empty mask
     const int Tl = act && tt.nt > 1 ? type_of_index(tt, (int32_t)u) : 0;
