@@ -170,7 +170,9 @@ typedef struct bcs_device_state {
     float* vein_forces;
     float* triangle_centers;
     void* stream;         /* hipStream_t the simulation enqueues on */
-    /* per model vertex (sum of ppc over the types, in type order) */
+    /* per model vertex (sum of ppc over the types, in type order); read
+     * only: the contact search takes its reach and whether every radius is
+     * the same from these radii at creation */
     float* model_radii;   /* collision radii: cellModelsBoundingSpheres
                              (src/simulation/simulation_controller.cuh:42) */
     float* initial_radii; /* distance to the model centroid: BloodCells::initialRadiuses

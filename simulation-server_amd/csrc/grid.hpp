@@ -189,8 +189,9 @@ void grid_free(DeviceGrid& G);
 // layout, nullable) with a grid allocated with copies also writes the
 // sorted-slot records pvs.  order (nullable): norder lanes naming every
 // object in [0, n) exactly once (.x, -1 for an empty lane), in spatial order;
-// the key kernel visits the objects in that order, so that runs of lanes
-// share one bucket counter add.  The result does not depend on it.
+// the key kernel visits the objects in that order, so that a block's lanes
+// share buckets (one histogram add per bucket and block).  The result does
+// not depend on it.
 int grid_build(DeviceGrid& G, const float* pos, int64_t stride, int32_t n, const float* vel, uint32_t* flags,
                hipStream_t st, const KernelTimer* timer = nullptr, const int2* order = nullptr, int32_t norder = 0);
 // The launches grid_build enqueues (sets G.n = n).
